@@ -1,0 +1,347 @@
+"""Pins the CPU oracle before it is trusted as the checker of the HIP engine.
+
+Every oracle function is checked against at least one of: the reference core's
+own outputs (committed golden fixtures from tests/golden/make_golden.py, plus a
+live cross-check against oracle/_ref when it is built), closed forms, brute-force
+dense operators, invariants, and the C++ standard's mt19937_64 known answer.
+The reference's own closed-form tests restated here: tests/test_core_math.py:10-69,
+tests/test_statevector.py:14-60, tests/test_independent_reference.py:44-61,
+tests/test_verification_metamorphic.py:47-77.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from invariantq_amd import gates, workloads
+from oracle import oracle as orc
+from tests.golden.replay import replay
+
+TOL = 1e-12
+
+
+def _u(rng, d):
+    return orc.haar_unitary(rng, d)
+
+
+# ----------------------------------------------------------- closed forms
+
+def test_single_qubit_closed_forms(oracle_lib):
+    sv = oracle_lib.statevector(1)
+    sv.apply_1q(0, gates.H)
+    a = sv.get_amplitudes()
+    assert np.allclose(a, [1 / math.sqrt(2)] * 2, atol=TOL)
+    sv.apply_1q(0, gates.X)
+    assert np.allclose(sv.get_amplitudes(), [1 / math.sqrt(2)] * 2, atol=TOL)
+    sv.reset()
+    sv.apply_1q(0, gates.X)
+    assert np.allclose(sv.get_amplitudes(), [0, 1], atol=TOL)
+
+
+def test_bell_and_ghz(oracle_lib):
+    sv = oracle_lib.statevector(3)
+    sv.apply_1q(0, gates.H)
+    sv.apply_2q(0, 1, gates.CX)
+    sv.apply_2q(1, 2, gates.CX)
+    a = sv.get_amplitudes()
+    exp = np.zeros(8, complex)
+    exp[0] = exp[7] = 1 / math.sqrt(2)
+    assert np.allclose(a, exp, atol=TOL)
+    # Bell pair entropy is exactly 1 bit (reference tests/test_core_math.py:24-34)
+    b = oracle_lib.statevector(2)
+    b.apply_1q(0, gates.H)
+    b.apply_2q(0, 1, gates.CX)
+    assert abs(b.von_neumann_entropy([0]) - 1.0) < 1e-12
+    p = oracle_lib.statevector(2)
+    p.apply_1q(0, gates.H)
+    p.apply_1q(1, gates.H)
+    assert abs(p.von_neumann_entropy([0])) < 1e-12
+
+
+def test_cx_control_is_first_wire(oracle_lib):
+    sv = oracle_lib.statevector(2)
+    sv.apply_1q(1, gates.X)          # |q1=1, q0=0> = index 2
+    sv.apply_2q(1, 0, gates.CX)      # control q1 -> flips q0
+    assert np.allclose(sv.get_amplitudes(), [0, 0, 0, 1], atol=TOL)
+
+
+def test_pauli_and_measure_closed_forms(oracle_lib):
+    sv = oracle_lib.statevector(1)
+    assert abs(sv.expectation_value_pauli("Z", [0]) - 1) < TOL
+    sv.apply_1q(0, gates.H)
+    assert abs(sv.expectation_value_pauli("X", [0]) - 1) < TOL
+    assert abs(sv.expectation_value_pauli("Z", [0])) < TOL
+    assert sv.measure(0, 0.1) == 0
+    assert np.allclose(sv.get_amplitudes(), [1, 0], atol=TOL)
+    sv.apply_1q(0, gates.H)
+    assert sv.measure(0, 0.9) == 1
+    assert np.allclose(sv.get_amplitudes(), [0, 1], atol=TOL)
+    sv.reset_qubit(0, 0.5)
+    assert np.allclose(sv.get_amplitudes(), [1, 0], atol=TOL)
+
+
+def test_zero_probability_collapse_is_an_error_not_an_abort(oracle_lib):
+    sv = oracle_lib.statevector(2)
+    assert sv.measure(1, 0.9999) == 0  # p0 = 1 > rand -> outcome 0
+    sv2 = oracle_lib.statevector(2)
+    out = sv2.measure(0, 1.5)  # rand >= p0 = 1 picks the zero-probability branch
+    assert out == 1
+    assert oracle_lib.lib.iqo_last_error() == 2
+    assert np.allclose(sv2.get_amplitudes(), [1, 0, 0, 0])  # left unchanged
+
+
+# ------------------------------------------------------- brute-force dense
+
+@pytest.mark.parametrize("qubits", [[0], [5], [2, 4], [4, 2], [0, 5], [5, 1, 3], [0, 6, 2, 4],
+                                    [6, 5, 4, 3, 2], [1, 3, 0, 6, 4], [0, 1, 2, 3, 4]])
+def test_gate_application_matches_dense_operator(oracle_lib, qubits):
+    n = 7
+    rng = np.random.default_rng(len(qubits) * 100 + qubits[0])
+    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    psi /= np.linalg.norm(psi)
+    U = _u(rng, 1 << len(qubits))
+    sv = oracle_lib.statevector(n)
+    sv.set_amplitudes(psi)
+    if len(qubits) == 1:
+        sv.apply_1q(qubits[0], U)
+    elif len(qubits) == 2:
+        sv.apply_2q(qubits[0], qubits[1], U)
+    else:
+        sv.apply_fused(qubits, U)
+    expect = orc.full_operator(n, qubits, U) @ psi
+    assert np.max(np.abs(sv.get_amplitudes() - expect)) < TOL
+    # the tensor-axis restatement (binding.py:603-620 scheme) agrees too
+    assert np.max(np.abs(orc.apply_tensor(psi, n, qubits, U) - expect)) < TOL
+
+
+def test_fused_k1_k2_forward_to_1q_2q(oracle_lib):
+    rng = np.random.default_rng(3)
+    for qs in ([3], [1, 4], [4, 1]):
+        U = _u(rng, 1 << len(qs))
+        a = oracle_lib.statevector(5)
+        b = oracle_lib.statevector(5)
+        for s in (a, b):
+            for q in range(5):
+                s.apply_1q(q, gates.u3(0.3 + q, 0.7, 1.1))
+        a.apply_fused(qs, U)
+        (b.apply_1q(qs[0], U) if len(qs) == 1 else b.apply_2q(qs[0], qs[1], U))
+        assert np.max(np.abs(a.get_amplitudes() - b.get_amplitudes())) < TOL
+
+
+def test_pauli_expval_matches_dense(oracle_lib):
+    n = 6
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    psi /= np.linalg.norm(psi)
+    sv = oracle_lib.statevector(n)
+    sv.set_amplitudes(psi)
+    for ops, qs in [("XYZ", [0, 3, 5]), ("YY", [1, 2]), ("ZZ", [4, 5]), ("XIY", [5, 0, 2]), ("Y", [3])]:
+        assert abs(sv.expectation_value_pauli(ops, qs) - orc.pauli_expval_dense(psi, n, ops, qs)) < TOL
+
+
+def test_tfim_dense_matches_index_arithmetic():
+    n = 5
+    rng = np.random.default_rng(8)
+    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    assert np.allclose(orc.tfim_dense(n) @ psi, orc.tfim_apply(psi, n), atol=1e-12)
+
+
+# --------------------------------------------------------------- sampling
+
+def test_mt19937_64_known_answer(oracle_lib):
+    # C++ [rand.predef]: the 10000th consecutive invocation of a default-constructed
+    # std::mt19937_64 (seed 5489) produces 9981545732273789042.
+    raw = oracle_lib.mt19937_64_raw(5489, 10000)
+    assert int(raw[-1]) == 9981545732273789042
+
+
+def test_uniform_map_is_word_over_two_to_the_64(oracle_lib):
+    seed = orc.seed_tree(42, "cfg0", 0)
+    raw = oracle_lib.mt19937_64_raw(seed, 4096)
+    u = oracle_lib.uniforms(seed, 4096)
+    expect = raw.astype(np.float64) / 2.0 ** 64
+    expect = np.where(expect >= 1.0, np.nextafter(1.0, 0.0), expect)
+    assert np.array_equal(u, expect)
+
+
+def test_seed_tree_is_sha256_prefix():
+    import hashlib
+    assert orc.seed_tree(42, "cfg0", 0) == int(hashlib.sha256(b"42:cfg0:0").hexdigest()[:16], 16)
+
+
+def test_sampling_is_lower_bound_on_serial_cdf(oracle_lib):
+    n = 6
+    rng = np.random.default_rng(2)
+    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    psi /= np.linalg.norm(psi)
+    sv = oracle_lib.statevector(n)
+    sv.set_amplitudes(psi)
+    s = sv.sample(3000, 77)
+    cdf = np.cumsum(np.abs(psi) ** 2)
+    u = oracle_lib.uniforms(77, 3000)
+    expect = np.minimum(np.searchsorted(cdf, u, side="left"), (1 << n) - 1)
+    assert np.array_equal(s, expect.astype(np.uint64))
+
+
+# ----------------------------------------------------------- density matrix
+
+def test_dm_matches_pure_state_outer_product(oracle_lib):
+    # reference tests/test_independent_reference.py:44-61 (rho = |psi><psi|)
+    n = 4
+    rng = np.random.default_rng(4)
+    sv = oracle_lib.statevector(n)
+    dm = oracle_lib.density_matrix(n)
+    for q in range(n):
+        U = _u(rng, 2)
+        sv.apply_1q(q, U)
+        dm.apply_1q_unitary(q, U)
+    for a, b in [(0, 1), (3, 1), (2, 0)]:
+        U = _u(rng, 4)
+        sv.apply_2q(a, b, U)
+        dm.apply_2q_unitary(a, b, U)
+    psi = sv.get_amplitudes()
+    assert np.max(np.abs(dm.get_matrix() - np.outer(psi, psi.conj()))) < TOL
+    assert abs(dm.trace() - 1) < TOL and abs(dm.purity() - 1) < 1e-12
+
+
+def test_depolarizing_purity_closed_form(oracle_lib):
+    # depolarizing(p) maps rho -> (1-p) rho + p I/2; on a pure state purity = 1 - p(1 - p/2)
+    for p in (0.05, 0.2, 0.7):
+        dm = oracle_lib.density_matrix(1)
+        dm.apply_1q_unitary(0, gates.u3(1.1, 0.4, 2.0))
+        dm.apply_1q_kraus(0, gates.depolarizing(p))
+        assert abs(dm.purity() - (1 - p * (1 - p / 2))) < 1e-12
+        assert abs(dm.trace() - 1) < 1e-12
+
+
+def test_amplitude_damping_on_one(oracle_lib):
+    g = 0.3
+    dm = oracle_lib.density_matrix(2)
+    dm.apply_1q_unitary(1, gates.X)
+    dm.apply_1q_kraus(1, gates.amplitude_damping(g))
+    z = dm.expval_z()
+    assert abs(z[1] - (1 - 2 * (1 - g))) < 1e-12   # P(1) = 1 - gamma
+    assert abs(z[0] - 1) < 1e-12
+
+
+def test_kraus_matches_superoperator(oracle_lib):
+    n = 3
+    rng = np.random.default_rng(6)
+    dm = oracle_lib.density_matrix(n)
+    for q in range(n):
+        dm.apply_1q_unitary(q, _u(rng, 2))
+    dm.apply_2q_unitary(0, 2, _u(rng, 4))
+    rho = dm.get_matrix()
+    ks = gates.amplitude_damping(0.25) + []
+    dm.apply_1q_kraus(1, ks)
+    expect = sum(orc.full_operator(n, [1], K) @ rho @ orc.full_operator(n, [1], K).conj().T for K in ks)
+    assert np.max(np.abs(dm.get_matrix() - expect)) < TOL
+
+
+def test_dm_measure_and_reset(oracle_lib):
+    dm = oracle_lib.density_matrix(2)
+    dm.apply_1q_unitary(0, gates.H)
+    assert dm.measure(0, 0.3) == 0
+    assert abs(dm.trace() - 1) < TOL
+    dm.apply_1q_unitary(1, gates.X)
+    dm.reset_qubit(1)
+    rho = dm.get_matrix()
+    assert abs(rho[0, 0] - 1) < TOL
+
+
+# ------------------------------------------------- reference golden fixtures
+
+def test_golden_u3cx_n8(oracle_lib, golden):
+    g = golden("sv_u3cx_n8.npz")
+    sv = oracle_lib.statevector(int(g["n"]))
+    replay(g, sv)
+    assert np.max(np.abs(sv.get_amplitudes() - g["amps"])) < TOL
+    assert np.max(np.abs(sv.get_probabilities() - g["probs"])) < TOL
+    assert np.array_equal(sv.sample(2000, int(g["sample_seed"])), g["samples"])
+    for ops, qs, val in zip(g["pauli_ops"], g["pauli_qubits"], g["pauli_vals"]):
+        q = [int(v) for v in qs if v >= 0]
+        assert abs(sv.expectation_value_pauli(str(ops), q) - val) < TOL
+    for q in range(int(g["n"])):
+        for o in (0, 1):
+            assert abs(sv.measure_prob(q, o) - g["measure_prob"][q][o]) < TOL
+    for sub, e in zip(g["entropy_subsets"], g["entropy"]):
+        assert abs(sv.von_neumann_entropy([int(v) for v in sub if v >= 0]) - e) < 1e-10
+
+
+def test_golden_sampling_n10(oracle_lib, golden):
+    g = golden("sv_sample_n10.npz")
+    sv = oracle_lib.statevector(int(g["n"]))
+    replay(g, sv)
+    for seed, expect in zip(g["seeds"], g["samples"]):
+        assert np.array_equal(sv.sample(expect.size, int(seed)), expect)
+
+
+def test_golden_fused_msb_first(oracle_lib, golden):
+    g = golden("sv_fused_n7.npz")
+    sv = oracle_lib.statevector(int(g["n"]))
+    replay(g, sv)
+    assert np.max(np.abs(sv.get_amplitudes() - g["amps"])) < TOL
+    # the reference core's native k=3,4 indexing (LSB-first) is a different map
+    assert np.max(np.abs(g["amps"] - g["amps_lsb_first"])) > 1e-3
+
+
+def test_golden_measure_n5(oracle_lib, golden):
+    g = golden("sv_measure_n5.npz")
+    sv = oracle_lib.statevector(int(g["n"]))
+    assert replay(g, sv) == list(g["outcomes"])
+    assert np.max(np.abs(sv.get_amplitudes() - g["amps"])) < TOL
+
+
+@pytest.mark.parametrize("name", ["dm_noisy_n4.npz", "dm_noisy_n7.npz"])
+def test_golden_dm(oracle_lib, golden, name):
+    g = golden(name)
+    dm = oracle_lib.density_matrix(int(g["n"]))
+    assert replay(g, dm) == list(g["outcomes"])
+    assert np.max(np.abs(dm.get_matrix() - g["rho"])) < TOL
+    assert abs(dm.trace() - float(g["trace"])) < TOL
+    assert abs(dm.purity() - float(g["purity"])) < TOL
+    assert abs(dm.von_neumann_entropy() - float(g["entropy"])) < 1e-10
+
+
+# --------------------------------- live cross-check against the compiled reference
+
+@pytest.mark.skipif(not orc.REF_LIB.exists(), reason="reference core not compiled here (oracle/_ref)")
+def test_live_reference_random_circuits(oracle_lib):
+    ref = orc.CoreLib(orc.REF_LIB)
+    for seed in range(3):
+        ops = workloads.random_u3_cx(9, 5, seed)
+        a, b = oracle_lib.statevector(9), ref.statevector(9)
+        for op in ops:
+            for s in (a, b):
+                (s.apply_1q(op.qubits[0], op.matrix) if len(op.qubits) == 1
+                 else s.apply_2q(op.qubits[0], op.qubits[1], op.matrix))
+        assert np.max(np.abs(a.get_amplitudes() - b.get_amplitudes())) < TOL
+        assert np.array_equal(a.sample(500, seed), b.sample(500, seed))
+
+
+@pytest.mark.skipif(not orc.REF_LIB.exists(), reason="reference core not compiled here (oracle/_ref)")
+def test_live_reference_fused_k3_k4():
+    ref = orc.CoreLib(orc.REF_LIB)
+    o = orc.oracle()
+    rng = np.random.default_rng(17)
+    for qs in ([4, 0, 2], [1, 5, 3, 0], [2, 3, 1, 4]):
+        U = _u(rng, 1 << len(qs))
+        a, b = o.statevector(6), ref.statevector(6)
+        for s in (a, b):
+            for q in range(6):
+                s.apply_1q(q, gates.u3(0.2 * q + 0.1, 0.5, 0.9))
+        a.apply_fused(qs, U)
+        b.apply_fused(list(reversed(qs)), U)
+        assert np.max(np.abs(a.get_amplitudes() - b.get_amplitudes())) < TOL
+
+
+def test_gate_builders_match_matrix_exponentials():
+    for th in (0.3, 1.7, 4.1):
+        assert np.allclose(gates.rx(th), scipy.linalg.expm(-0.5j * th * gates.X), atol=1e-14)
+        assert np.allclose(gates.ry(th), scipy.linalg.expm(-0.5j * th * gates.Y), atol=1e-14)
+        assert np.allclose(gates.rz(th), scipy.linalg.expm(-0.5j * th * gates.Z), atol=1e-14)
+    th, ph, la = 0.7, 1.9, 2.6
+    decomposed = np.exp(0.5j * (ph + la)) * gates.rz(ph) @ gates.ry(th) @ gates.rz(la)
+    assert np.allclose(gates.u3(th, ph, la), decomposed, atol=1e-14)
