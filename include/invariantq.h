@@ -1,0 +1,142 @@
+/*
+ * libinvariantq.so — MI355X (gfx950) statevector / density-matrix engine.
+ *
+ * Drop-in C ABI for the reference InvariantQ native core.  Every name in the
+ * first three sections replaces the same name of reference
+ * core/include/c_api.h (line numbers cited), with identical C signatures, as
+ * bound by invariantq/core/binding.py:54-299.  Semantics follow
+ * core/src/api.cpp:14-316 with two documented changes:
+ *   - apply_fused uses the MSB-first local index for every k (qubits[0] is the
+ *     most significant bit of the row index), and k may be 1..5;
+ *   - invalid input never aborts: the call leaves the state unchanged and sets
+ *     a thread-local code read with iq_last_error().
+ * Matrices are split real/imag, row-major, copied before the call returns.
+ * Gate calls are asynchronous on the handle's HIP stream; calls returning data
+ * wait for it.  Handles are not thread-safe (as in the reference).
+ */
+#ifndef INVARIANTQ_H
+#define INVARIANTQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------- StateVector (c_api.h:10-24) */
+void*  iq_statevector_create(size_t num_qubits);                                                     /* c_api.h:11 */
+void   iq_statevector_free(void* sv);                                                                /* c_api.h:12 */
+void   iq_statevector_reset(void* sv);                                                               /* c_api.h:13 */
+void   iq_statevector_apply_1q(void* sv, size_t target, const double* u_real, const double* u_imag); /* c_api.h:14 */
+void   iq_statevector_apply_2q(void* sv, size_t q0, size_t q1, const double* u_real,
+                               const double* u_imag);                                                /* c_api.h:15 */
+void   iq_statevector_apply_fused(void* sv, const size_t* qubits, size_t k, const double* u_real,
+                                  const double* u_imag);                                             /* c_api.h:16 */
+int    iq_statevector_measure(void* sv, size_t target, double rand_val);                             /* c_api.h:17 */
+void   iq_statevector_reset_qubit(void* sv, size_t target, double rand_val);                         /* c_api.h:18 */
+double iq_statevector_measure_prob(void* sv, size_t target, int outcome);                            /* c_api.h:19 */
+void   iq_statevector_get_amplitudes(void* sv, double* real_out, double* imag_out);                  /* c_api.h:20 */
+void   iq_statevector_get_probabilities(void* sv, double* probs_out);                                /* c_api.h:21 */
+void   iq_statevector_sample(void* sv, size_t num_shots, uint64_t seed, uint64_t* results_out);      /* c_api.h:22 */
+double iq_statevector_entropy(void* sv, const size_t* qubits, size_t num_subsys_qubits);             /* c_api.h:23 */
+void   iq_statevector_expval_pauli(void* sv, const char* pauli_str, const size_t* qubits, size_t len,
+                                   double* val_real, double* val_imag);                              /* c_api.h:24 */
+
+/* ------------------------------------------------- DensityMatrix (c_api.h:26-38) */
+void*  iq_density_matrix_create(size_t num_qubits);                                                  /* c_api.h:27 */
+void   iq_density_matrix_free(void* dm);                                                             /* c_api.h:28 */
+void   iq_density_matrix_reset(void* dm);                                                            /* c_api.h:29 */
+void   iq_density_matrix_apply_1q_unitary(void* dm, size_t target, const double* u_real,
+                                          const double* u_imag);                                     /* c_api.h:30 */
+void   iq_density_matrix_apply_2q_unitary(void* dm, size_t q0, size_t q1, const double* u_real,
+                                          const double* u_imag);                                     /* c_api.h:31 */
+void   iq_density_matrix_apply_kraus_1q(void* dm, size_t target, size_t num_kraus, const double* k_real,
+                                        const double* k_imag);                                       /* c_api.h:32 */
+void   iq_density_matrix_get_elements(void* dm, double* real_out, double* imag_out);                 /* c_api.h:33 */
+double iq_density_matrix_trace(void* dm);                                                            /* c_api.h:34 */
+double iq_density_matrix_purity(void* dm);                                                           /* c_api.h:35 */
+double iq_density_matrix_entropy(void* dm);                                                          /* c_api.h:36 */
+int    iq_density_matrix_measure(void* dm, size_t target, double rand_val);                          /* c_api.h:37 */
+void   iq_density_matrix_reset_qubit(void* dm, size_t target);                                       /* c_api.h:38 */
+
+/* ------------------- Stabilizer / MPS (c_api.h:40-65): exported, not provided.
+ * *_create returns NULL with iq_last_error() == IQ_ERR_UNSUPPORTED; the rest are no-ops. */
+void*  iq_stabilizer_create(size_t num_qubits);                                                      /* c_api.h:41 */
+void   iq_stabilizer_free(void* stab);                                                               /* c_api.h:42 */
+void   iq_stabilizer_reset(void* stab);                                                              /* c_api.h:43 */
+void   iq_stabilizer_reset_qubit(void* stab, size_t a, double rand_val);                             /* c_api.h:44 */
+void   iq_stabilizer_apply_h(void* stab, size_t a);                                                  /* c_api.h:45 */
+void   iq_stabilizer_apply_s(void* stab, size_t a);                                                  /* c_api.h:46 */
+void   iq_stabilizer_apply_sdg(void* stab, size_t a);                                                /* c_api.h:47 */
+void   iq_stabilizer_apply_cnot(void* stab, size_t a, size_t b);                                     /* c_api.h:48 */
+void   iq_stabilizer_apply_cz(void* stab, size_t a, size_t b);                                       /* c_api.h:49 */
+void   iq_stabilizer_apply_swap(void* stab, size_t a, size_t b);                                     /* c_api.h:50 */
+void   iq_stabilizer_apply_x(void* stab, size_t a);                                                  /* c_api.h:51 */
+void   iq_stabilizer_apply_y(void* stab, size_t a);                                                  /* c_api.h:52 */
+void   iq_stabilizer_apply_z(void* stab, size_t a);                                                  /* c_api.h:53 */
+int    iq_stabilizer_measure(void* stab, size_t a, double rand_val);                                 /* c_api.h:54 */
+void   iq_stabilizer_to_string(void* stab, char* buffer, size_t max_len);                            /* c_api.h:55 */
+void*  iq_mps_create(size_t num_qubits, size_t max_bond_dim, double cutoff);                         /* c_api.h:58 */
+void   iq_mps_free(void* mps);                                                                       /* c_api.h:59 */
+void   iq_mps_reset(void* mps);                                                                      /* c_api.h:60 */
+void   iq_mps_apply_1q(void* mps, size_t target, const double* u_real, const double* u_imag);        /* c_api.h:61 */
+void   iq_mps_apply_2q(void* mps, size_t q0, size_t q1, const double* u_real, const double* u_imag);  /* c_api.h:62 */
+double iq_mps_entropy(void* mps, size_t bond_idx);                                                   /* c_api.h:63 */
+void   iq_mps_to_statevector(void* mps, double* real_out, double* imag_out);                         /* c_api.h:64 */
+int    iq_mps_measure(void* mps, size_t target, double rand_val);                                    /* c_api.h:65 */
+
+/* ------------------------------------------ Additions (no reference counterpart) */
+#define IQ_OK 0
+#define IQ_ERR_INVALID_ARGUMENT 1
+#define IQ_ERR_ZERO_PROBABILITY 2
+#define IQ_ERR_OUT_OF_MEMORY 3
+#define IQ_ERR_DEVICE 4
+#define IQ_ERR_UNSUPPORTED 5
+
+#define IQ_PRECISION_C128 0 /* complex128 amplitudes: the exact mode */
+#define IQ_PRECISION_C64 1  /* complex64 amplitudes, FP64 reductions: the fast mode */
+
+/* Error reporting in place of the reference's C++ exceptions
+ * (core/src/statevector.cpp:27-29,50-52,100-102,173-175). */
+int         iq_last_error(void);
+const char* iq_last_error_message(void);
+void        iq_clear_error(void);
+const char* iq_backend_name(void);
+/* Gate kernel selection: 0 automatic, 1 direct (group per lane), 2 LDS-tiled. */
+void        iq_set_gate_path(int path);
+
+void*  iq_statevector_create_precision(size_t num_qubits, int precision);
+int    iq_statevector_get_precision(void* sv);
+void   iq_statevector_set_amplitudes(void* sv, const double* real_in, const double* imag_in);
+double iq_statevector_norm_squared(void* sv);     /* StateVector::norm_squared, statevector.cpp:206-213 */
+void   iq_statevector_normalize(void* sv);        /* StateVector::normalize, statevector.cpp:215-224 */
+void   iq_statevector_apply_diagonal(void* sv, const size_t* qubits, size_t k, const double* d_real,
+                                     const double* d_imag);
+/* Sampling with caller-supplied uniforms in [0,1) (e.g. drawn from a seed tree). */
+void   iq_statevector_sample_uniforms(void* sv, size_t num_shots, const double* uniforms, uint64_t* results_out);
+/* Inclusive FP64 cumulative distribution used by sampling (dim doubles). */
+void   iq_statevector_get_cdf(void* sv, double* cdf_out);
+/* sum_t coeffs[t] <P_t>; term t uses lens[t] consecutive characters of ops and
+ * entries of qubits.  per_term (optional) receives Re/Im <P_t> pairs. */
+void   iq_statevector_expval_pauli_sum(void* sv, size_t nterms, const double* coeffs, const char* ops,
+                                       const size_t* qubits, const size_t* lens, double* out_real,
+                                       double* out_imag, double* per_term);
+void   iq_statevector_synchronize(void* sv);
+void*  iq_statevector_device_ptr(void* sv);  /* device address of the amplitudes (re, im interleaved) */
+void*  iq_statevector_stream(void* sv);      /* hipStream_t the handle's work is ordered on */
+
+void   iq_density_matrix_set_elements(void* dm, const double* real_in, const double* imag_in);
+/* 1- or 2-qubit Kraus channel (class-only in the reference, density_matrix.cpp:165-191). */
+void   iq_density_matrix_apply_kraus(void* dm, const size_t* qubits, size_t k, size_t num_kraus,
+                                     const double* k_real, const double* k_imag);
+/* <Z_q> for every qubit q from the diagonal (nq doubles). */
+void   iq_density_matrix_expval_z(void* dm, double* out);
+void   iq_density_matrix_synchronize(void* dm);
+void*  iq_density_matrix_stream(void* dm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INVARIANTQ_H */

@@ -1,0 +1,939 @@
+// C-ABI of libinvariantq.so on gfx950: statevector and density-matrix handles
+// whose amplitudes live in HBM.
+//
+// Boundary contract (reference core/include/c_api.h:10-38, core/src/api.cpp:14-192,
+// invariantq/core/binding.py:302-476):
+//   * same names and signatures; matrices arrive split re/im, row-major, and are
+//     copied before the call returns (the caller keeps ownership);
+//   * gate applications are enqueued on the handle's stream and return at once;
+//     every call that hands data back (amplitudes, probabilities, samples,
+//     scalars) waits for the stream first, so the observable behaviour is the
+//     reference's synchronous one;
+//   * errors never cross extern "C" as exceptions: invalid qubits, k > 5 and
+//     zero-probability collapses leave the state unchanged and set a
+//     thread-local code read by iq_last_error() (the reference throws there,
+//     core/src/statevector.cpp:27-29,50-52,100-102,173-175, which aborts the process).
+//
+// A density matrix is the 2n-qubit vector of row-major rho (element (r, c) at
+// r * 2^n + c, reference core/src/density_matrix.cpp:9): row qubit q is bit n+q,
+// column qubit q is bit q.  U rho U^dagger is the superoperator U (x) conj(U) on
+// bits (n+q...), (q...) and a Kraus set is sum_k K (x) conj(K) — one HBM pass
+// each instead of the reference's K+2 full copies.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "kernels.hpp"
+
+using namespace iqg;
+
+namespace iqg {
+
+static thread_local int t_err = kOk;
+static thread_local char t_msg[256] = "";
+static int g_gate_path = kPathAuto;
+
+void set_error(int code, const char* fmt, ...) {
+    t_err = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t_msg, sizeof(t_msg), fmt, ap);
+    va_end(ap);
+}
+
+bool hip_ok(hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    set_error(kErrDevice, "%s: %s", what, hipGetErrorString(e));
+    return false;
+}
+
+int gate_path_override() { return g_gate_path; }
+
+DeviceGuard::DeviceGuard(int dev) {
+    hipGetDevice(&prev_);
+    if (prev_ != dev) hipSetDevice(dev);
+    dev_ = dev;
+}
+DeviceGuard::~DeviceGuard() {
+    if (prev_ != dev_) hipSetDevice(prev_);
+}
+
+// ----------------------------------------------------------------- Engine
+
+bool Engine::init(int nqubits, int precision) {
+    n = nqubits;
+    prec = precision;
+    dim = 1ULL << n;
+    elem = prec == kC128 ? 16 : 8;
+    hipGetDevice(&dev);
+    if (!hip_ok(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream create")) return false;
+    if (hipMalloc(&amp, dim * elem) != hipSuccess) {
+        set_error(kErrOutOfMemory, "cannot allocate %llu bytes of device memory", (unsigned long long)(dim * elem));
+        amp = nullptr;
+        return false;
+    }
+    const size_t red = (size_t)kRedBlocks * 64 * sizeof(double);
+    if (!hip_ok(hipMalloc(&red_part, red), "alloc") || !hip_ok(hipMalloc(&red_out, 128 * sizeof(double)), "alloc") ||
+        !hip_ok(hipHostMalloc((void**)&host_out, 128 * sizeof(double)), "pinned alloc"))
+        return false;
+    slot_bytes = gate_matrix_bytes(kC128, kMaxK, true);
+    slots.resize(kSlots);
+    for (auto& s : slots) {
+        if (!hip_ok(hipHostMalloc(&s.host, slot_bytes), "pinned alloc") || !hip_ok(hipMalloc(&s.dev, slot_bytes), "alloc") ||
+            !hip_ok(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "event"))
+            return false;
+    }
+    return reset_basis(0);
+}
+
+Engine::~Engine() {
+    if (stream) hipStreamSynchronize(stream);
+    for (auto& s : slots) {
+        if (s.host) hipHostFree(s.host);
+        if (s.dev) hipFree(s.dev);
+        if (s.ev) hipEventDestroy(s.ev);
+    }
+    if (amp) hipFree(amp);
+    if (red_part) hipFree(red_part);
+    if (red_out) hipFree(red_out);
+    if (host_out) hipHostFree(host_out);
+    if (work) hipFree(work);
+    if (stage) hipFree(stage);
+    if (stream) hipStreamDestroy(stream);
+}
+
+bool Engine::reset_basis(uint64_t index) {
+    return hip_ok(launch_init_basis(amp, n, prec, index, stream), "init");
+}
+
+const void* Engine::upload_matrix(int K, bool m3, const double* re, const double* im) {
+    Slot& s = slots[next_slot];
+    next_slot = (next_slot + 1) % kSlots;
+    if (s.pending) hipEventSynchronize(s.ev);
+    pack_gate_matrix(prec, K, m3, re, im, s.host);
+    const size_t bytes = gate_matrix_bytes(prec, K, m3);
+    if (!hip_ok(hipMemcpyAsync(s.dev, s.host, bytes, hipMemcpyHostToDevice, stream), "matrix upload")) return nullptr;
+    hip_ok(hipEventRecord(s.ev, stream), "event record");
+    s.pending = true;
+    return s.dev;
+}
+
+static bool is_diagonal(int K, const double* re, const double* im) {
+    const int d = 1 << K;
+    for (int r = 0; r < d; ++r)
+        for (int c = 0; c < d; ++c)
+            if (r != c && (re[r * d + c] != 0.0 || im[r * d + c] != 0.0)) return false;
+    return true;
+}
+
+bool Engine::apply(int K, const uint32_t* q, const double* re, const double* im, int path) {
+    if (K < 1 || K > kMaxK || K > n) {
+        set_error(kErrInvalidArgument, "gate on %d qubits is not supported (1..5, <= n)", K);
+        return false;
+    }
+    for (int m = 0; m < K; ++m) {
+        if ((int)q[m] >= n) {
+            set_error(kErrInvalidArgument, "qubit index %u out of range for %d qubits", q[m], n);
+            return false;
+        }
+        for (int j = 0; j < m; ++j)
+            if (q[j] == q[m]) {
+                set_error(kErrInvalidArgument, "repeated qubit %u in gate", q[m]);
+                return false;
+            }
+    }
+    if (path == kPathAuto && is_diagonal(K, re, im)) {
+        const int d = 1 << K;
+        double dre[32], dimv[32];
+        for (int l = 0; l < d; ++l) {
+            dre[l] = re[l * d + l];
+            dimv[l] = im[l * d + l];
+        }
+        return hip_ok(launch_diag(amp, n, prec, stream, K, q, dre, dimv), "diagonal gate launch");
+    }
+    GateDesc g;
+    g.K = K;
+    for (int m = 0; m < K; ++m) g.q[m] = q[m];
+    g.re = re;
+    g.im = im;
+    g.path = path;
+    const int chosen = choose_path(n, prec, g);
+    if (K >= 4 || chosen == kPathTiled) {
+        g.dmat = upload_matrix(K, use_three_mult(K), re, im);
+        if (!g.dmat) return false;
+    }
+    return hip_ok(launch_gate(amp, n, prec, stream, g), "gate launch");
+}
+
+bool Engine::sync() { return hip_ok(hipStreamSynchronize(stream), "stream synchronize"); }
+
+bool Engine::ensure_work(size_t doubles) {
+    if (work_doubles >= doubles) return true;
+    if (work) hipFree(work);
+    work = nullptr;
+    work_doubles = 0;
+    if (hipMalloc(&work, doubles * sizeof(double)) != hipSuccess) {
+        set_error(kErrOutOfMemory, "cannot allocate %zu scratch doubles", doubles);
+        return false;
+    }
+    work_doubles = doubles;
+    return true;
+}
+
+bool Engine::ensure_stage() {
+    if (stage) return true;
+    if (hipMalloc(&stage, 2 * kStage * sizeof(double)) != hipSuccess) {
+        set_error(kErrOutOfMemory, "cannot allocate export staging");
+        stage = nullptr;
+        return false;
+    }
+    return true;
+}
+
+bool Engine::export_split(double* re, double* im) {
+    if (!ensure_stage()) return false;
+    for (uint64_t first = 0; first < dim; first += kStage) {
+        const uint64_t cnt = std::min<uint64_t>(kStage, dim - first);
+        if (!hip_ok(launch_split(amp, first, cnt, prec, stage, stage + kStage, stream), "split")) return false;
+        if (!hip_ok(hipMemcpyAsync(re + first, stage, cnt * 8, hipMemcpyDeviceToHost, stream), "D2H")) return false;
+        if (!hip_ok(hipMemcpyAsync(im + first, stage + kStage, cnt * 8, hipMemcpyDeviceToHost, stream), "D2H")) return false;
+        if (!sync()) return false;
+    }
+    return true;
+}
+
+bool Engine::import_split(const double* re, const double* im) {
+    if (!ensure_stage() || !sync()) return false;
+    for (uint64_t first = 0; first < dim; first += kStage) {
+        const uint64_t cnt = std::min<uint64_t>(kStage, dim - first);
+        if (!hip_ok(hipMemcpyAsync(stage, re + first, cnt * 8, hipMemcpyHostToDevice, stream), "H2D")) return false;
+        if (!hip_ok(hipMemcpyAsync(stage + kStage, im + first, cnt * 8, hipMemcpyHostToDevice, stream), "H2D")) return false;
+        if (!hip_ok(launch_join(amp, first, cnt, prec, stage, stage + kStage, stream), "join")) return false;
+        if (!sync()) return false;
+    }
+    return true;
+}
+
+bool Engine::export_probs(double* out) {
+    if (!ensure_stage()) return false;
+    const size_t esz = elem;
+    for (uint64_t first = 0; first < dim; first += kStage) {
+        const uint64_t cnt = std::min<uint64_t>(kStage, dim - first);
+        const void* src = static_cast<const char*>(amp) + first * esz;
+        if (!hip_ok(launch_probs(src, cnt, prec, stage, stream), "probs")) return false;
+        if (!hip_ok(hipMemcpyAsync(out + first, stage, cnt * 8, hipMemcpyDeviceToHost, stream), "D2H")) return false;
+        if (!sync()) return false;
+    }
+    return true;
+}
+
+bool Engine::fetch(int m, double* out) {
+    if (!hip_ok(hipMemcpyAsync(host_out, red_out, m * sizeof(double), hipMemcpyDeviceToHost, stream), "D2H") || !sync())
+        return false;
+    for (int i = 0; i < m; ++i) out[i] = host_out[i];
+    return true;
+}
+
+bool Engine::norm2(double* out) {
+    return hip_ok(reduce_norm(amp, dim, prec, red_part, red_out, stream), "norm") && fetch(1, out);
+}
+
+bool Engine::bit_probs(int target, double* p01) {
+    return hip_ok(reduce_bit_probs(amp, n, prec, target, red_part, red_out, stream), "bit probability") && fetch(2, p01);
+}
+
+bool Engine::pauli_terms(size_t nterms, const PauliTerm* terms, double* vals) {
+    // group strings by flip mask, up to 8 per pass
+    std::vector<int> done(nterms, 0);
+    for (size_t i = 0; i < nterms; ++i) {
+        if (done[i]) continue;
+        std::vector<size_t> grp;
+        for (size_t j = i; j < nterms && grp.size() < 8; ++j)
+            if (!done[j] && terms[j].xmask == terms[i].xmask) {
+                grp.push_back(j);
+                done[j] = 1;
+            }
+        uint64_t zy[8];
+        int ny[8];
+        for (size_t s = 0; s < grp.size(); ++s) {
+            zy[s] = terms[grp[s]].zymask;
+            ny[s] = terms[grp[s]].ny;
+        }
+        double tmp[16];
+        if (!hip_ok(reduce_pauli_group(amp, n, prec, terms[i].xmask, (int)grp.size(), zy, ny, red_part, red_out, stream),
+                    "pauli expectation") ||
+            !fetch(2 * (int)grp.size(), tmp))
+            return false;
+        for (size_t s = 0; s < grp.size(); ++s) {
+            vals[2 * grp[s]] = tmp[2 * s];
+            vals[2 * grp[s] + 1] = tmp[2 * s + 1];
+        }
+    }
+    return true;
+}
+
+bool parse_pauli(const char* ops, const size_t* qubits, size_t len, int n, PauliTerm& t) {
+    t.xmask = 0;
+    t.zymask = 0;
+    t.ny = 0;
+    for (size_t k = 0; k < len; ++k) {
+        const size_t q = qubits[k];
+        const char c = ops[k];
+        if (c != 'X' && c != 'Y' && c != 'Z') continue;  // other characters act as identity
+        if (q >= (size_t)n) {
+            set_error(kErrInvalidArgument, "Pauli qubit %zu out of range for %d qubits", q, n);
+            return false;
+        }
+        const uint64_t b = 1ULL << q;
+        if (c == 'X') t.xmask ^= b;
+        else if (c == 'Y') {
+            t.xmask ^= b;
+            t.zymask ^= b;
+            t.ny += 1;
+        } else t.zymask ^= b;
+    }
+    return true;
+}
+
+// Largest-pivot complex Jacobi with an iteration cap and an off-diagonal
+// threshold: the eigenvalue method of reference core/src/statevector.cpp:333-393
+// and core/src/density_matrix.cpp:322-370, kept so entropies agree with it.
+std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> A, size_t d, int max_iter, double tol) {
+    using cplx = std::complex<double>;
+    auto at = [&](size_t i, size_t j) -> cplx& { return A[i * d + j]; };
+    for (int it = 0; it < max_iter; ++it) {
+        double best = 0.0;
+        size_t p = 0, q = 1;
+        for (size_t i = 0; i < d; ++i)
+            for (size_t j = i + 1; j < d; ++j) {
+                const double v = std::abs(at(i, j));
+                if (v > best) {
+                    best = v;
+                    p = i;
+                    q = j;
+                }
+            }
+        if (best < tol) break;
+        const double app = at(p, p).real(), aqq = at(q, q).real();
+        const cplx apq = at(p, q);
+        const double th = 0.5 * std::atan2(2.0 * std::abs(apq), app - aqq);
+        const cplx ph = apq / std::abs(apq);
+        const double c = std::cos(th);
+        const cplx sn = std::sin(th) * ph;
+        for (size_t i = 0; i < d; ++i) {
+            if (i == p || i == q) continue;
+            const cplx aip = at(i, p), aiq = at(i, q);
+            at(i, p) = c * aip + std::conj(sn) * aiq;
+            at(p, i) = std::conj(at(i, p));
+            at(i, q) = -sn * aip + c * aiq;
+            at(q, i) = std::conj(at(i, q));
+        }
+        const double cross = 2.0 * (c * std::conj(sn) * apq).real();
+        at(p, p) = c * c * app + std::norm(sn) * aqq + cross;
+        at(q, q) = std::norm(sn) * app + c * c * aqq - cross;
+        at(p, q) = 0.0;
+        at(q, p) = 0.0;
+    }
+    std::vector<double> ev(d);
+    for (size_t i = 0; i < d; ++i) ev[i] = at(i, i).real();
+    return ev;
+}
+
+double entropy_bits(const std::vector<double>& ev) {
+    double h = 0.0;
+    for (double l : ev)
+        if (l > 1e-14) h -= l * std::log2(l);
+    return std::max(0.0, h);
+}
+
+}  // namespace iqg
+
+// ------------------------------------------------------------------ C ABI
+
+namespace {
+
+Engine* sv_of(void* p) { return static_cast<Engine*>(p); }
+
+struct DMHandle {
+    Engine eng;  // 2n-qubit vectorised rho, complex128
+    int nq = 0;
+};
+
+DMHandle* dm_of(void* p) { return static_cast<DMHandle*>(p); }
+
+void to_u32(const size_t* q, size_t k, uint32_t* out) {
+    for (size_t i = 0; i < k; ++i) out[i] = (uint32_t)std::min<size_t>(q[i], 0xffffffffu);
+}
+
+// S = sum_k K (x) conj(K), both d x d; S is (d*d) x (d*d) row-major (MSB-first: row part high).
+void superop(size_t d, size_t nk, const double* kre, const double* kim, std::vector<double>& sre, std::vector<double>& sim) {
+    const size_t D = d * d;
+    sre.assign(D * D, 0.0);
+    sim.assign(D * D, 0.0);
+    for (size_t k = 0; k < nk; ++k) {
+        const double* ar = kre + k * d * d;
+        const double* ai = kim + k * d * d;
+        for (size_t r = 0; r < d; ++r)
+            for (size_t c = 0; c < d; ++c)
+                for (size_t r2 = 0; r2 < d; ++r2)
+                    for (size_t c2 = 0; c2 < d; ++c2) {
+                        // K[r, r2] * conj(K[c, c2])
+                        const double xr = ar[r * d + r2], xi = ai[r * d + r2];
+                        const double yr = ar[c * d + c2], yi = -ai[c * d + c2];
+                        const size_t row = r * d + c, col = r2 * d + c2;
+                        sre[row * D + col] += xr * yr - xi * yi;
+                        sim[row * D + col] += xr * yi + xi * yr;
+                    }
+    }
+}
+
+bool dm_apply_superop(DMHandle* h, const uint32_t* q, int k, size_t nk, const double* kre, const double* kim) {
+    std::vector<double> sre, sim;
+    superop((size_t)1 << k, nk, kre, kim, sre, sim);
+    uint32_t bits[4];
+    for (int i = 0; i < k; ++i) {
+        bits[i] = q[i] + (uint32_t)h->nq;  // row qubits
+        bits[k + i] = q[i];                 // column qubits
+    }
+    return h->eng.apply(2 * k, bits, sre.data(), sim.data(), gate_path_override());
+}
+
+}  // namespace
+
+extern "C" {
+
+int iq_last_error(void) { return t_err; }
+const char* iq_last_error_message(void) { return t_msg; }
+void iq_clear_error(void) {
+    t_err = kOk;
+    t_msg[0] = '\0';
+}
+const char* iq_backend_name(void) { return "hip-gfx950"; }
+void iq_set_gate_path(int path) { g_gate_path = (path >= 0 && path <= 2) ? path : 0; }
+
+// ------------------------------------------------------------- statevector
+
+void* iq_statevector_create_precision(size_t num_qubits, int precision) {
+    if (num_qubits > (size_t)kMaxN || (precision != kC128 && precision != kC64)) {
+        set_error(kErrInvalidArgument, "unsupported statevector shape (n=%zu, precision=%d)", num_qubits, precision);
+        return nullptr;
+    }
+    Engine* e = new (std::nothrow) Engine();
+    if (!e) return nullptr;
+    if (!e->init((int)num_qubits, precision)) {
+        delete e;
+        return nullptr;
+    }
+    return e;
+}
+
+void* iq_statevector_create(size_t num_qubits) { return iq_statevector_create_precision(num_qubits, kC128); }
+
+int iq_statevector_get_precision(void* sv) { return sv ? sv_of(sv)->prec : -1; }
+
+void iq_statevector_free(void* sv) {
+    if (!sv) return;
+    DeviceGuard g(sv_of(sv)->dev);
+    delete sv_of(sv);
+}
+
+void iq_statevector_reset(void* sv) {
+    if (!sv) return;
+    DeviceGuard g(sv_of(sv)->dev);
+    sv_of(sv)->reset_basis(0);
+}
+
+void iq_statevector_apply_1q(void* sv, size_t target, const double* u_real, const double* u_imag) {
+    if (!sv || !u_real || !u_imag) return;
+    DeviceGuard g(sv_of(sv)->dev);
+    uint32_t q = (uint32_t)std::min<size_t>(target, 0xffffffffu);
+    sv_of(sv)->apply(1, &q, u_real, u_imag, gate_path_override());
+}
+
+void iq_statevector_apply_2q(void* sv, size_t q0, size_t q1, const double* u_real, const double* u_imag) {
+    if (!sv || !u_real || !u_imag) return;
+    DeviceGuard g(sv_of(sv)->dev);
+    const size_t qs[2] = {q0, q1};
+    uint32_t q[2];
+    to_u32(qs, 2, q);
+    sv_of(sv)->apply(2, q, u_real, u_imag, gate_path_override());
+}
+
+void iq_statevector_apply_fused(void* sv, const size_t* qubits, size_t k, const double* u_real, const double* u_imag) {
+    if (!sv || !qubits || !u_real || !u_imag) return;
+    DeviceGuard g(sv_of(sv)->dev);
+    if (k < 1 || k > (size_t)kMaxK) {
+        set_error(kErrInvalidArgument, "fused gate on %zu qubits is not supported (1..5)", k);
+        return;
+    }
+    uint32_t q[kMaxK];
+    to_u32(qubits, k, q);
+    sv_of(sv)->apply((int)k, q, u_real, u_imag, gate_path_override());
+}
+
+void iq_statevector_apply_diagonal(void* sv, const size_t* qubits, size_t k, const double* d_real, const double* d_imag) {
+    if (!sv || !qubits || !d_real || !d_imag) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    if (k < 1 || k > (size_t)kMaxK || k > (size_t)e->n) {
+        set_error(kErrInvalidArgument, "diagonal gate on %zu qubits is not supported", k);
+        return;
+    }
+    uint32_t q[kMaxK];
+    to_u32(qubits, k, q);
+    for (size_t i = 0; i < k; ++i)
+        if ((int)q[i] >= e->n) {
+            set_error(kErrInvalidArgument, "qubit index out of range");
+            return;
+        }
+    hip_ok(launch_diag(e->amp, e->n, e->prec, e->stream, (int)k, q, d_real, d_imag), "diagonal gate launch");
+}
+
+double iq_statevector_measure_prob(void* sv, size_t target, int outcome) {
+    if (!sv) return 0.0;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    if (target >= (size_t)e->n) {
+        set_error(kErrInvalidArgument, "target qubit %zu out of range", target);
+        return 0.0;
+    }
+    double p[2];
+    if (!e->bit_probs((int)target, p)) return 0.0;
+    return (outcome == 0 || outcome == 1) ? p[outcome] : 0.0;
+}
+
+// reference core/src/statevector.cpp:171-195: outcome = rand < P0 ? 0 : 1,
+// renormalise by 1/sqrt(P(outcome)); P(outcome) < 1e-15 is an error.
+int iq_statevector_measure(void* sv, size_t target, double rand_val) {
+    if (!sv) return 0;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    if (target >= (size_t)e->n) {
+        set_error(kErrInvalidArgument, "target qubit %zu out of range", target);
+        return 0;
+    }
+    double p[2];
+    if (!e->bit_probs((int)target, p)) return 0;
+    const int outcome = (rand_val < p[0]) ? 0 : 1;
+    if (p[outcome] < 1e-15) {
+        set_error(kErrZeroProbability, "attempted to collapse to a zero-probability state");
+        return outcome;
+    }
+    hip_ok(launch_collapse(e->amp, e->n, e->prec, (int)target, outcome, 1.0 / std::sqrt(p[outcome]), e->stream), "collapse");
+    return outcome;
+}
+
+void iq_statevector_reset_qubit(void* sv, size_t target, double rand_val) {
+    if (!sv) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    if (target >= (size_t)e->n) {
+        set_error(kErrInvalidArgument, "target qubit %zu out of range", target);
+        return;
+    }
+    const int before = t_err;
+    iq_clear_error();
+    const int outcome = iq_statevector_measure(sv, target, rand_val);
+    if (t_err != kOk) return;
+    t_err = before;
+    if (outcome == 1) {
+        const double xr[4] = {0, 1, 1, 0}, xi[4] = {0, 0, 0, 0};
+        uint32_t q = (uint32_t)target;
+        e->apply(1, &q, xr, xi, gate_path_override());
+    }
+}
+
+void iq_statevector_get_amplitudes(void* sv, double* real_out, double* imag_out) {
+    if (!sv || !real_out || !imag_out) return;
+    DeviceGuard g(sv_of(sv)->dev);
+    sv_of(sv)->export_split(real_out, imag_out);
+}
+
+void iq_statevector_set_amplitudes(void* sv, const double* real_in, const double* imag_in) {
+    if (!sv || !real_in || !imag_in) return;
+    DeviceGuard g(sv_of(sv)->dev);
+    sv_of(sv)->import_split(real_in, imag_in);
+}
+
+void iq_statevector_get_probabilities(void* sv, double* probs_out) {
+    if (!sv || !probs_out) return;
+    DeviceGuard g(sv_of(sv)->dev);
+    sv_of(sv)->export_probs(probs_out);
+}
+
+double iq_statevector_norm_squared(void* sv) {
+    if (!sv) return 0.0;
+    DeviceGuard g(sv_of(sv)->dev);
+    double v = 0.0;
+    sv_of(sv)->norm2(&v);
+    return v;
+}
+
+void iq_statevector_normalize(void* sv) {
+    if (!sv) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    double v = 0.0;
+    if (!e->norm2(&v) || !(v > 0.0)) return;
+    hip_ok(launch_scale(e->amp, e->dim, e->prec, 1.0 / std::sqrt(v), e->stream), "scale");
+}
+
+// reference core/src/statevector.cpp:235-256: probabilities, inclusive CDF,
+// uniform_real_distribution<double>(0,1) over mt19937_64(seed), lower_bound,
+// clamp to dim-1.  The uniforms are drawn here on the host with the same
+// standard-library engine and distribution; the CDF and the searches run on the GPU.
+void iq_statevector_sample(void* sv, size_t num_shots, uint64_t seed, uint64_t* results_out) {
+    if (!sv || !results_out) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    const size_t nchunks = scan_scratch_doubles(e->dim);
+    if (!e->ensure_work(e->dim + nchunks + 2 * std::max<size_t>(num_shots, 1))) return;
+    double* cdf = e->work;
+    double* scratch = cdf + e->dim;
+    double* du = scratch + nchunks;
+    uint64_t* dout = reinterpret_cast<uint64_t*>(du + std::max<size_t>(num_shots, 1));
+    std::vector<double> u(num_shots);
+    std::mt19937_64 gen(seed);
+    std::uniform_real_distribution<double> uni(0.0, 1.0);
+    for (size_t s = 0; s < num_shots; ++s) u[s] = uni(gen);
+    if (!hip_ok(launch_probs(e->amp, e->dim, e->prec, cdf, e->stream), "probs") ||
+        !hip_ok(launch_scan(cdf, cdf, e->dim, scratch, e->stream), "scan"))
+        return;
+    if (num_shots == 0) {
+        e->sync();
+        return;
+    }
+    if (!hip_ok(hipMemcpyAsync(du, u.data(), num_shots * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
+        !hip_ok(launch_search(cdf, e->dim, du, num_shots, dout, e->stream), "search") ||
+        !hip_ok(hipMemcpyAsync(results_out, dout, num_shots * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
+        return;
+    e->sync();
+}
+
+// Device-side uniforms variant: the caller passes the uniforms (e.g. from a
+// seed tree); lower_bound semantics as iq_statevector_sample.
+void iq_statevector_sample_uniforms(void* sv, size_t num_shots, const double* uniforms, uint64_t* results_out) {
+    if (!sv || !results_out || (!uniforms && num_shots)) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    const size_t nchunks = scan_scratch_doubles(e->dim);
+    if (!e->ensure_work(e->dim + nchunks + 2 * std::max<size_t>(num_shots, 1))) return;
+    double* cdf = e->work;
+    double* scratch = cdf + e->dim;
+    double* du = scratch + nchunks;
+    uint64_t* dout = reinterpret_cast<uint64_t*>(du + std::max<size_t>(num_shots, 1));
+    if (!hip_ok(launch_probs(e->amp, e->dim, e->prec, cdf, e->stream), "probs") ||
+        !hip_ok(launch_scan(cdf, cdf, e->dim, scratch, e->stream), "scan"))
+        return;
+    if (num_shots) {
+        if (!hip_ok(hipMemcpyAsync(du, uniforms, num_shots * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
+            !hip_ok(launch_search(cdf, e->dim, du, num_shots, dout, e->stream), "search") ||
+            !hip_ok(hipMemcpyAsync(results_out, dout, num_shots * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
+            return;
+    }
+    e->sync();
+}
+
+// Inclusive FP64 CDF of the probabilities, exported (for boundary-margin checks).
+void iq_statevector_get_cdf(void* sv, double* cdf_out) {
+    if (!sv || !cdf_out) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    const size_t nchunks = scan_scratch_doubles(e->dim);
+    if (!e->ensure_work(e->dim + nchunks + 2)) return;
+    if (!hip_ok(launch_probs(e->amp, e->dim, e->prec, e->work, e->stream), "probs") ||
+        !hip_ok(launch_scan(e->work, e->work, e->dim, e->work + e->dim, e->stream), "scan") ||
+        !hip_ok(hipMemcpyAsync(cdf_out, e->work, e->dim * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
+        return;
+    e->sync();
+}
+
+// reference core/src/statevector.cpp:293-330 (partial trace, LSB-first local
+// index over `qubits`) and :395-410 (entropy from capped Jacobi eigenvalues).
+double iq_statevector_entropy(void* sv, const size_t* qubits, size_t num_subsys_qubits) {
+    if (!sv || !qubits) return 0.0;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    const size_t k = num_subsys_qubits;
+    if (k == 0 || k == (size_t)e->n) return 0.0;
+    if (k > 12) {
+        set_error(kErrUnsupported, "entropy of more than 12 subsystem qubits is not supported");
+        return 0.0;
+    }
+    uint32_t q[16];
+    to_u32(qubits, k, q);
+    for (size_t i = 0; i < k; ++i) {
+        if ((int)q[i] >= e->n) {
+            set_error(kErrInvalidArgument, "subsystem qubit out of range");
+            return 0.0;
+        }
+        for (size_t j = 0; j < i; ++j)
+            if (q[i] == q[j]) {
+                set_error(kErrInvalidArgument, "repeated subsystem qubit");
+                return 0.0;
+            }
+    }
+    const size_t kd = (size_t)1 << k;
+    if (!e->ensure_work(2 * kd * kd)) return 0.0;
+    std::vector<std::complex<double>> red(kd * kd);
+    if (!hip_ok(launch_partial_trace(e->amp, e->n, e->prec, q, (int)k, reinterpret_cast<double2*>(e->work), e->stream),
+                "partial trace") ||
+        !hip_ok(hipMemcpyAsync(red.data(), e->work, kd * kd * 16, hipMemcpyDeviceToHost, e->stream), "D2H") || !e->sync())
+        return 0.0;
+    return entropy_bits(jacobi_eigenvalues(std::move(red), kd, 100, 1e-12));
+}
+
+void iq_statevector_expval_pauli(void* sv, const char* pauli_str, const size_t* qubits, size_t len, double* val_real,
+                                 double* val_imag) {
+    if (!sv || !pauli_str || !qubits || !val_real || !val_imag) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    PauliTerm t;
+    if (!parse_pauli(pauli_str, qubits, len, e->n, t)) return;
+    double v[2] = {0.0, 0.0};
+    if (!e->pauli_terms(1, &t, v)) return;
+    *val_real = v[0];
+    *val_imag = v[1];
+}
+
+// Sum of Pauli strings: term t has lens[t] characters of `ops` and qubits,
+// consumed consecutively.  Writes sum_t coeff[t] <P_t> and, if per_term is
+// non-null, the 2*nterms values Re/Im <P_t>.
+void iq_statevector_expval_pauli_sum(void* sv, size_t nterms, const double* coeffs, const char* ops, const size_t* qubits,
+                                     const size_t* lens, double* out_real, double* out_imag, double* per_term) {
+    if (!sv || !ops || !qubits || !lens || !out_real || !out_imag) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    std::vector<PauliTerm> terms(nterms);
+    size_t off = 0;
+    for (size_t t = 0; t < nterms; ++t) {
+        if (!parse_pauli(ops + off, qubits + off, lens[t], e->n, terms[t])) return;
+        off += lens[t];
+    }
+    std::vector<double> vals(2 * std::max<size_t>(nterms, 1), 0.0);
+    if (nterms && !e->pauli_terms(nterms, terms.data(), vals.data())) return;
+    double sr = 0.0, si = 0.0;
+    for (size_t t = 0; t < nterms; ++t) {
+        const double c = coeffs ? coeffs[t] : 1.0;
+        sr += c * vals[2 * t];
+        si += c * vals[2 * t + 1];
+    }
+    *out_real = sr;
+    *out_imag = si;
+    if (per_term)
+        for (size_t i = 0; i < 2 * nterms; ++i) per_term[i] = vals[i];
+}
+
+void iq_statevector_synchronize(void* sv) {
+    if (!sv) return;
+    DeviceGuard g(sv_of(sv)->dev);
+    sv_of(sv)->sync();
+}
+
+void* iq_statevector_device_ptr(void* sv) { return sv ? sv_of(sv)->amp : nullptr; }
+void* iq_statevector_stream(void* sv) { return sv ? (void*)sv_of(sv)->stream : nullptr; }
+
+// ---------------------------------------------------------- density matrix
+
+void* iq_density_matrix_create(size_t num_qubits) {
+    if (num_qubits > 16) {
+        set_error(kErrInvalidArgument, "density matrices above 16 qubits are not supported");
+        return nullptr;
+    }
+    DMHandle* h = new (std::nothrow) DMHandle();
+    if (!h) return nullptr;
+    h->nq = (int)num_qubits;
+    if (!h->eng.init(2 * (int)num_qubits, kC128)) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void iq_density_matrix_free(void* dm) {
+    if (!dm) return;
+    DeviceGuard g(dm_of(dm)->eng.dev);
+    delete dm_of(dm);
+}
+
+void iq_density_matrix_reset(void* dm) {
+    if (!dm) return;
+    DeviceGuard g(dm_of(dm)->eng.dev);
+    dm_of(dm)->eng.reset_basis(0);
+}
+
+void iq_density_matrix_apply_1q_unitary(void* dm, size_t target, const double* u_real, const double* u_imag) {
+    if (!dm || !u_real || !u_imag) return;
+    DMHandle* h = dm_of(dm);
+    DeviceGuard g(h->eng.dev);
+    if (target >= (size_t)h->nq) {
+        set_error(kErrInvalidArgument, "target qubit %zu out of range", target);
+        return;
+    }
+    const uint32_t q = (uint32_t)target;
+    dm_apply_superop(h, &q, 1, 1, u_real, u_imag);
+}
+
+void iq_density_matrix_apply_2q_unitary(void* dm, size_t q0, size_t q1, const double* u_real, const double* u_imag) {
+    if (!dm || !u_real || !u_imag) return;
+    DMHandle* h = dm_of(dm);
+    DeviceGuard g(h->eng.dev);
+    if (q0 >= (size_t)h->nq || q1 >= (size_t)h->nq || q0 == q1) {
+        set_error(kErrInvalidArgument, "invalid qubit indices for a 2-qubit unitary");
+        return;
+    }
+    const uint32_t q[2] = {(uint32_t)q0, (uint32_t)q1};
+    dm_apply_superop(h, q, 2, 1, u_real, u_imag);
+}
+
+void iq_density_matrix_apply_kraus_1q(void* dm, size_t target, size_t num_kraus, const double* k_real, const double* k_imag) {
+    if (!dm || !k_real || !k_imag) return;
+    DMHandle* h = dm_of(dm);
+    DeviceGuard g(h->eng.dev);
+    if (target >= (size_t)h->nq) {
+        set_error(kErrInvalidArgument, "target qubit %zu out of range", target);
+        return;
+    }
+    const uint32_t q = (uint32_t)target;
+    dm_apply_superop(h, &q, 1, num_kraus, k_real, k_imag);
+}
+
+// Multi-qubit Kraus channel on 1 or 2 qubits (reference
+// core/src/density_matrix.cpp:165-191 has it as a class method only).
+void iq_density_matrix_apply_kraus(void* dm, const size_t* qubits, size_t k, size_t num_kraus, const double* k_real,
+                                   const double* k_imag) {
+    if (!dm || !qubits || !k_real || !k_imag) return;
+    DMHandle* h = dm_of(dm);
+    DeviceGuard g(h->eng.dev);
+    if (k < 1 || k > 2) {
+        set_error(kErrInvalidArgument, "Kraus channels act on 1 or 2 qubits");
+        return;
+    }
+    uint32_t q[2];
+    to_u32(qubits, k, q);
+    for (size_t i = 0; i < k; ++i)
+        if ((int)q[i] >= h->nq || (i == 1 && q[0] == q[1])) {
+            set_error(kErrInvalidArgument, "invalid Kraus qubits");
+            return;
+        }
+    dm_apply_superop(h, q, (int)k, num_kraus, k_real, k_imag);
+}
+
+void iq_density_matrix_get_elements(void* dm, double* real_out, double* imag_out) {
+    if (!dm || !real_out || !imag_out) return;
+    DeviceGuard g(dm_of(dm)->eng.dev);
+    dm_of(dm)->eng.export_split(real_out, imag_out);
+}
+
+void iq_density_matrix_set_elements(void* dm, const double* real_in, const double* imag_in) {
+    if (!dm || !real_in || !imag_in) return;
+    DeviceGuard g(dm_of(dm)->eng.dev);
+    dm_of(dm)->eng.import_split(real_in, imag_in);
+}
+
+static bool dm_diag(DMHandle* h, int mode, int target, double* out) {
+    const int m = mode == 1 ? h->nq : (mode == 2 ? 2 : 1);
+    return hip_ok(reduce_dm_diag(h->eng.amp, h->nq, mode, target, h->eng.red_part, h->eng.red_out, h->eng.stream),
+                  "diagonal reduction") &&
+           h->eng.fetch(m, out);
+}
+
+double iq_density_matrix_trace(void* dm) {
+    if (!dm) return 0.0;
+    DeviceGuard g(dm_of(dm)->eng.dev);
+    double v = 0.0;
+    dm_diag(dm_of(dm), 0, 0, &v);
+    return v;
+}
+
+double iq_density_matrix_purity(void* dm) {
+    if (!dm) return 0.0;
+    DeviceGuard g(dm_of(dm)->eng.dev);
+    double v = 0.0;
+    dm_of(dm)->eng.norm2(&v);
+    return v;
+}
+
+void iq_density_matrix_expval_z(void* dm, double* out) {
+    if (!dm || !out) return;
+    DMHandle* h = dm_of(dm);
+    DeviceGuard g(h->eng.dev);
+    if (h->nq == 0) return;
+    dm_diag(h, 1, 0, out);
+}
+
+// reference core/src/density_matrix.cpp:310-380
+double iq_density_matrix_entropy(void* dm) {
+    if (!dm) return 0.0;
+    DMHandle* h = dm_of(dm);
+    DeviceGuard g(h->eng.dev);
+    const size_t d = (size_t)1 << h->nq;
+    if (d > 64) {
+        double v = 0.0;
+        dm_diag(h, 3, 0, &v);
+        return v;
+    }
+    std::vector<double> re(d * d), im(d * d);
+    if (!h->eng.export_split(re.data(), im.data())) return 0.0;
+    std::vector<std::complex<double>> A(d * d);
+    for (size_t i = 0; i < d * d; ++i) A[i] = {re[i], im[i]};
+    return entropy_bits(jacobi_eigenvalues(std::move(A), d, 120, 1e-13));
+}
+
+// reference core/src/density_matrix.cpp:218-256
+int iq_density_matrix_measure(void* dm, size_t target, double rand_val) {
+    if (!dm) return 0;
+    DMHandle* h = dm_of(dm);
+    DeviceGuard g(h->eng.dev);
+    if (target >= (size_t)h->nq) {
+        set_error(kErrInvalidArgument, "target qubit %zu out of range", target);
+        return 0;
+    }
+    double p[2];
+    if (!dm_diag(h, 2, (int)target, p)) return 0;
+    p[0] = std::max(0.0, p[0]);
+    p[1] = std::max(0.0, p[1]);
+    const int outcome = (rand_val < p[0]) ? 0 : 1;
+    if (p[outcome] < 1e-15) {
+        set_error(kErrZeroProbability, "cannot collapse to a zero-probability outcome");
+        return outcome;
+    }
+    hip_ok(launch_dm_collapse(h->eng.amp, h->nq, (int)target, outcome, 1.0 / p[outcome], h->eng.stream), "collapse");
+    return outcome;
+}
+
+// reference core/src/density_matrix.cpp:258-269: Kraus {|0><0|, |0><1|}.
+void iq_density_matrix_reset_qubit(void* dm, size_t target) {
+    if (!dm) return;
+    DMHandle* h = dm_of(dm);
+    DeviceGuard g(h->eng.dev);
+    if (target >= (size_t)h->nq) {
+        set_error(kErrInvalidArgument, "target qubit %zu out of range", target);
+        return;
+    }
+    const double kr[8] = {1, 0, 0, 0, 0, 1, 0, 0}, ki[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t q = (uint32_t)target;
+    dm_apply_superop(h, &q, 1, 2, kr, ki);
+}
+
+void iq_density_matrix_synchronize(void* dm) {
+    if (!dm) return;
+    DeviceGuard g(dm_of(dm)->eng.dev);
+    dm_of(dm)->eng.sync();
+}
+
+void* iq_density_matrix_stream(void* dm) { return dm ? (void*)dm_of(dm)->eng.stream : nullptr; }
+
+}  // extern "C"

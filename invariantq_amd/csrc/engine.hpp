@@ -1,0 +1,87 @@
+// Device-resident state handle shared by the statevector, density-matrix,
+// adjoint and batch entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdarg>
+#include <vector>
+
+#include "common.hpp"
+
+namespace iqg {
+
+void set_error(int code, const char* fmt, ...);
+bool hip_ok(hipError_t e, const char* what);
+int gate_path_override();
+
+class DeviceGuard {
+public:
+    explicit DeviceGuard(int dev);
+    ~DeviceGuard();
+
+private:
+    int prev_ = 0, dev_ = 0;
+};
+
+struct PauliTerm {
+    uint64_t xmask = 0;   // X and Y positions (flip mask)
+    uint64_t zymask = 0;  // Y and Z positions (sign mask)
+    int ny = 0;           // number of Y factors
+};
+
+bool parse_pauli(const char* ops, const size_t* qubits, size_t len, int n, PauliTerm& t);
+std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> A, size_t d, int max_iter, double tol);
+double entropy_bits(const std::vector<double>& ev);
+
+struct Engine {
+    static constexpr int kSlots = 16;
+    static constexpr uint64_t kStage = 1ULL << 22;  // export chunk, doubles
+
+    struct Slot {
+        void* host = nullptr;
+        void* dev = nullptr;
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+    };
+
+    int n = 0;
+    int prec = kC128;
+    uint64_t dim = 0;
+    size_t elem = 16;
+    int dev = 0;
+    void* amp = nullptr;
+    hipStream_t stream = nullptr;
+    double* red_part = nullptr;
+    double* red_out = nullptr;
+    double* host_out = nullptr;
+    std::vector<Slot> slots;
+    int next_slot = 0;
+    size_t slot_bytes = 0;
+    double* work = nullptr;
+    size_t work_doubles = 0;
+    double* stage = nullptr;
+
+    Engine() = default;
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+    ~Engine();
+
+    bool init(int nqubits, int precision);
+    bool reset_basis(uint64_t index);
+    const void* upload_matrix(int K, bool m3, const double* re, const double* im);
+    bool apply(int K, const uint32_t* q, const double* re, const double* im, int path);
+    bool sync();
+    bool ensure_work(size_t doubles);
+    bool ensure_stage();
+    bool export_split(double* re, double* im);
+    bool import_split(const double* re, const double* im);
+    bool export_probs(double* out);
+    bool fetch(int m, double* out);
+    bool norm2(double* out);
+    bool bit_probs(int target, double* p01);
+    bool pauli_terms(size_t nterms, const PauliTerm* terms, double* vals);
+};
+
+}  // namespace iqg

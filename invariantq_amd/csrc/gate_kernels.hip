@@ -1,0 +1,322 @@
+// Gate application kernels for gfx950: the HBM-bound pass over 2^n amplitudes.
+//
+// A K-qubit gate (K = 1..5) splits the state into 2^(n-K) independent groups of
+// 2^K amplitudes; each group is multiplied by the 2^K x 2^K matrix.  This is
+// the algorithm of reference core/src/statevector.cpp:26-152 (1q pairs, 2q
+// quads, fused gather/matvec/scatter) with the documented MSB-first local
+// index for every K: local bit (K-1-m) of a group member is qubit q[m].
+//
+// Two kernels:
+//   direct  one lane = one group.  The group is loaded straight into VGPRs.
+//           Lanes of a wave span the lowest non-target qubits, so each load
+//           instruction is coalesced when the lowest target is >= 3 (c128) /
+//           >= 4 (c64), i.e. runs of >= 128 B.
+//   tiled   one wave stages a 2^(K+6)-amplitude tile in LDS: the targets plus
+//           the six lowest non-target qubits.  Those six always include qubits
+//           0..5, so every global access is a contiguous run of >= 64
+//           amplitudes.  Lanes then gather their group from LDS (XOR swizzle
+//           keeps the 16-lane phases bank-conflict free), multiply, write back
+//           to LDS and the wave stores the tile coalesced.
+// The matrix is read with uniform (scalar) loads, so every FMA takes its
+// matrix operand from an SGPR pair.  K >= 4 blocks use the 3-multiply complex
+// product (Re = A.u - B.v, Im = (A+B).(u+v) - A.u - B.v): 3 real matvecs
+// instead of 4, which moves the FP64 compute roof from 9.8 to ~13 TB/s.
+
+#include "kernels.hpp"
+#include "layout.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace iqg {
+
+namespace {
+
+template <typename R, int N>
+struct ValMat {
+    R v[N];
+};
+
+template <typename R, int D, bool M3, typename Get, typename Put>
+__device__ __forceinline__ void matvec(const R (&xr)[D], const R (&xi)[D], Get m, Put put) {
+    if constexpr (M3) {
+        R xs[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) xs[c] = xr[c] + xi[c];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            R t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                t1 = fma(m(r * D + c), xr[c], t1);
+                t2 = fma(m(D * D + r * D + c), xi[c], t2);
+                t3 = fma(m(2 * D * D + r * D + c), xs[c], t3);
+            }
+            put(r, t1 - t2, t3 - t1 - t2);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            R yr = 0, yi = 0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const R a = m(r * D + c);
+                const R b = m(D * D + r * D + c);
+                yr = fma(a, xr[c], yr);
+                yr = fma(-b, xi[c], yr);
+                yi = fma(a, xi[c], yi);
+                yi = fma(b, xr[c], yi);
+            }
+            put(r, yr, yi);
+        }
+    }
+}
+
+template <typename R, int K, bool M3>
+__global__ void __launch_bounds__(256) k_gate_direct_ptr(cv<R>* __restrict__ psi, uint64_t ngroups, DirectArgs<K> a,
+                                                         const R* __restrict__ mat) {
+    constexpr int D = 1 << K;
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint64_t base = insert_zero_bits<K>(g, a.pos);
+    R xr[D], xi[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        const cv<R> v = psi[base + a.off[s]];
+        xr[s] = v.x;
+        xi[s] = v.y;
+    }
+    matvec<R, D, M3>(
+        xr, xi, [&](int i) { return mat[i]; },
+        [&](int r, R yr, R yi) {
+            cv<R> o;
+            o.x = yr;
+            o.y = yi;
+            psi[base + a.off[r]] = o;
+        });
+}
+
+template <typename R, int K, bool M3>
+__global__ void __launch_bounds__(256) k_gate_direct_val(cv<R>* __restrict__ psi, uint64_t ngroups, DirectArgs<K> a,
+                                                         ValMat<R, (M3 ? 3 : 2) * (1 << (2 * K))> mat) {
+    constexpr int D = 1 << K;
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint64_t base = insert_zero_bits<K>(g, a.pos);
+    R xr[D], xi[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        const cv<R> v = psi[base + a.off[s]];
+        xr[s] = v.x;
+        xi[s] = v.y;
+    }
+    matvec<R, D, M3>(
+        xr, xi, [&](int i) { return mat.v[i]; },
+        [&](int r, R yr, R yi) {
+            cv<R> o;
+            o.x = yr;
+            o.y = yi;
+            psi[base + a.off[r]] = o;
+        });
+}
+
+template <typename R, int K, bool M3>
+__global__ void __launch_bounds__(64) k_gate_tiled(cv<R>* __restrict__ psi, TiledArgs<K> a, const R* __restrict__ mat) {
+    constexpr int D = 1 << K;
+    constexpr int T = 64 * D;
+    __shared__ cv<R> tile[T];
+    const uint32_t lane = threadIdx.x;
+    uint32_t pl, pg;
+    tile_lane_positions<K>(lane, a, pl, pg);
+    for (uint64_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+        const uint64_t base = deposit_bits(t, a.rest) + lane;
+        cv<R> v[D];
+#pragma unroll
+        for (int m = 0; m < D; ++m) v[m] = psi[base + a.off[m]];
+#pragma unroll
+        for (int m = 0; m < D; ++m) tile[pl ^ a.pm[m]] = v[m];
+        __syncthreads();
+        R xr[D], xi[D];
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const cv<R> x = tile[pg ^ a.ts[s]];
+            xr[s] = x.x;
+            xi[s] = x.y;
+        }
+        matvec<R, D, M3>(
+            xr, xi, [&](int i) { return mat[i]; },
+            [&](int r, R yr, R yi) {
+                cv<R> o;
+                o.x = yr;
+                o.y = yi;
+                tile[pg ^ a.ts[r]] = o;
+            });
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < D; ++m) psi[base + a.off[m]] = tile[pl ^ a.pm[m]];
+        __syncthreads();
+    }
+}
+
+template <typename R, int K>
+__global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t count, DirectArgs<K> a,
+                                              ValMat<double, 2 * (1 << K)> d) {
+    constexpr int D = 1 << K;
+    __shared__ double sre[D], sim[D];
+    if (threadIdx.x < D) {
+        sre[threadIdx.x] = d.v[threadIdx.x];
+        sim[threadIdx.x] = d.v[D + threadIdx.x];
+    }
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        uint32_t loc = 0;
+#pragma unroll
+        for (int m = 0; m < K; ++m) loc |= (uint32_t)((i >> a.pos[m]) & 1u) << (K - 1 - m);
+        const cv<R> v = psi[i];
+        const R dr = (R)sre[loc], di = (R)sim[loc];
+        cv<R> o;
+        o.x = dr * v.x - di * v.y;
+        o.y = dr * v.y + di * v.x;
+        psi[i] = o;
+    }
+}
+
+inline unsigned grid_for(uint64_t items, unsigned block) {
+    const uint64_t g = (items + block - 1) / block;
+    return (unsigned)std::max<uint64_t>(1, g);
+}
+
+template <typename R, int K, bool M3>
+hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
+    DirectArgs<K> a;
+    fill_direct_args<K>(a, g.q);
+    const uint64_t ngroups = 1ULL << (n - K);
+    const unsigned blocks = grid_for(ngroups, 256);
+    constexpr int D = 1 << K;
+    if constexpr (K <= 3) {
+        ValMat<R, (M3 ? 3 : 2) * D * D> mv;
+        pack_gate_matrix(std::is_same<R, double>::value ? kC128 : kC64, K, M3, g.re, g.im, mv.v);
+        hipLaunchKernelGGL((k_gate_direct_val<R, K, M3>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups, a, mv);
+    } else {
+        hipLaunchKernelGGL((k_gate_direct_ptr<R, K, M3>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups, a,
+                           (const R*)g.dmat);
+    }
+    return hipGetLastError();
+}
+
+template <typename R, int K, bool M3>
+hipError_t run_tiled(void* psi, int n, hipStream_t st, const GateDesc& g) {
+    TiledArgs<K> a;
+    fill_tiled_args<K>(a, n, g.q);
+    const unsigned blocks = (unsigned)std::min<uint64_t>(a.ntiles, 256ULL * 32ULL);
+    hipLaunchKernelGGL((k_gate_tiled<R, K, M3>), dim3(blocks), dim3(64), 0, st, (cv<R>*)psi, a, (const R*)g.dmat);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t dispatch(void* psi, int n, hipStream_t st, const GateDesc& g, int path) {
+    const bool m3 = use_three_mult(g.K);
+    if (path == kPathTiled) {
+        switch (g.K) {
+            case 3: return m3 ? run_tiled<R, 3, true>(psi, n, st, g) : run_tiled<R, 3, false>(psi, n, st, g);
+            case 4: return m3 ? run_tiled<R, 4, true>(psi, n, st, g) : run_tiled<R, 4, false>(psi, n, st, g);
+            case 5: return m3 ? run_tiled<R, 5, true>(psi, n, st, g) : run_tiled<R, 5, false>(psi, n, st, g);
+            default: break;
+        }
+    }
+    switch (g.K) {
+        case 1: return run_direct<R, 1, false>(psi, n, st, g);
+        case 2: return run_direct<R, 2, false>(psi, n, st, g);
+        case 3: return m3 ? run_direct<R, 3, true>(psi, n, st, g) : run_direct<R, 3, false>(psi, n, st, g);
+        case 4: return m3 ? run_direct<R, 4, true>(psi, n, st, g) : run_direct<R, 4, false>(psi, n, st, g);
+        case 5: return m3 ? run_direct<R, 5, true>(psi, n, st, g) : run_direct<R, 5, false>(psi, n, st, g);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename R, int K>
+hipError_t run_diag(void* psi, int n, hipStream_t st, const uint32_t* q, const double* dre, const double* dim) {
+    DirectArgs<K> a;
+    for (int m = 0; m < K; ++m) a.pos[m] = q[m];  // kept in list order: local bit K-1-m <-> q[m]
+    for (int l = 0; l < (1 << K); ++l) a.off[l] = 0;
+    ValMat<double, 2 * (1 << K)> d;
+    for (int l = 0; l < (1 << K); ++l) {
+        d.v[l] = dre[l];
+        d.v[(1 << K) + l] = dim[l];
+    }
+    const uint64_t count = 1ULL << n;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(grid_for(count, 256), 256ULL * 64ULL);
+    hipLaunchKernelGGL((k_diag<R, K>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+bool use_three_mult(int K) { return K >= 4; }
+
+size_t gate_matrix_bytes(int prec, int K, bool three_mult) {
+    const size_t d = 1ULL << K;
+    return (three_mult ? 3 : 2) * d * d * (prec == kC128 ? sizeof(double) : sizeof(float));
+}
+
+void pack_gate_matrix(int prec, int K, bool three_mult, const double* re, const double* im, void* out) {
+    const size_t d = 1ULL << K, dd = d * d;
+    if (prec == kC128) {
+        double* o = static_cast<double*>(out);
+        for (size_t i = 0; i < dd; ++i) {
+            o[i] = re[i];
+            o[dd + i] = im[i];
+            if (three_mult) o[2 * dd + i] = re[i] + im[i];
+        }
+    } else {
+        float* o = static_cast<float*>(out);
+        for (size_t i = 0; i < dd; ++i) {
+            o[i] = (float)re[i];
+            o[dd + i] = (float)im[i];
+            if (three_mult) o[2 * dd + i] = (float)(re[i] + im[i]);
+        }
+    }
+}
+
+int choose_path(int n, int prec, const GateDesc& g) {
+    if (g.path != kPathAuto) return g.path;
+    if (g.K <= 2 || n < g.K + 6) return kPathDirect;
+    uint32_t lowest = 64;
+    for (int m = 0; m < g.K; ++m) lowest = std::min(lowest, g.q[m]);
+    const uint32_t coalesce_bits = (prec == kC128) ? 3u : 4u;  // 128 B runs
+    return lowest >= coalesce_bits ? kPathDirect : kPathTiled;
+}
+
+hipError_t launch_gate(void* psi, int n, int prec, hipStream_t st, const GateDesc& g) {
+    if (g.K < 1 || g.K > kMaxK || g.K > n) return hipErrorInvalidValue;
+    int path = choose_path(n, prec, g);
+    if (path == kPathTiled && (g.K < 3 || n < g.K + 6)) path = kPathDirect;
+    if (g.K >= 4 && !g.dmat) return hipErrorInvalidValue;
+    if (path == kPathTiled && !g.dmat) path = kPathDirect;
+    return prec == kC128 ? dispatch<double>(psi, n, st, g, path) : dispatch<float>(psi, n, st, g, path);
+}
+
+hipError_t launch_diag(void* psi, int n, int prec, hipStream_t st, int K, const uint32_t* q, const double* dre,
+                       const double* dim) {
+    if (K < 1 || K > kMaxK || K > n) return hipErrorInvalidValue;
+    if (prec == kC128) {
+        switch (K) {
+            case 1: return run_diag<double, 1>(psi, n, st, q, dre, dim);
+            case 2: return run_diag<double, 2>(psi, n, st, q, dre, dim);
+            case 3: return run_diag<double, 3>(psi, n, st, q, dre, dim);
+            case 4: return run_diag<double, 4>(psi, n, st, q, dre, dim);
+            default: return run_diag<double, 5>(psi, n, st, q, dre, dim);
+        }
+    }
+    switch (K) {
+        case 1: return run_diag<float, 1>(psi, n, st, q, dre, dim);
+        case 2: return run_diag<float, 2>(psi, n, st, q, dre, dim);
+        case 3: return run_diag<float, 3>(psi, n, st, q, dre, dim);
+        case 4: return run_diag<float, 4>(psi, n, st, q, dre, dim);
+        default: return run_diag<float, 5>(psi, n, st, q, dre, dim);
+    }
+}
+
+}  // namespace iqg

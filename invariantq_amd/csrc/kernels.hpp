@@ -1,0 +1,64 @@
+// Host-side launch interface of the statevector / density-matrix kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace iqg {
+
+// Matrix layout consumed by the gate kernels for one 2^K x 2^K block:
+//   [A | B]      (4-multiply form: A = Re U, B = Im U), or
+//   [A | B | S]  (3-multiply form, S = A + B),
+// each D x D row-major in the state's real type.
+size_t gate_matrix_bytes(int prec, int K, bool three_mult);
+void pack_gate_matrix(int prec, int K, bool three_mult, const double* re, const double* im, void* out);
+bool use_three_mult(int K);
+
+enum GatePath : int { kPathAuto = 0, kPathDirect = 1, kPathTiled = 2 };
+
+struct GateDesc {
+    int K = 0;                 // number of target qubits, 1..5
+    uint32_t q[kMaxK] = {};    // target list, MSB-first: q[0] is the most significant local bit
+    const double* re = nullptr;  // host row-major D x D (kept for by-value launches, K <= 3)
+    const double* im = nullptr;
+    const void* dmat = nullptr;  // device copy in pack_gate_matrix layout (required for K >= 4)
+    int path = kPathAuto;
+};
+
+// psi <- U psi on the K target qubits.  Returns the launch status.
+hipError_t launch_gate(void* psi, int n, int prec, hipStream_t st, const GateDesc& g);
+// Diagonal gate: psi[i] *= d[local(i)] (d host, 2^K entries, MSB-first local index).
+hipError_t launch_diag(void* psi, int n, int prec, hipStream_t st, int K, const uint32_t* q,
+                       const double* dre, const double* dim);
+int choose_path(int n, int prec, const GateDesc& g);
+
+// State initialisation and element-wise passes.
+hipError_t launch_init_basis(void* psi, int n, int prec, uint64_t index, hipStream_t st);
+hipError_t launch_collapse(void* psi, int n, int prec, int target, int outcome, double factor, hipStream_t st);
+hipError_t launch_scale(void* psi, uint64_t count, int prec, double factor, hipStream_t st);
+hipError_t launch_split(const void* psi, uint64_t first, uint64_t count, int prec, double* re, double* im, hipStream_t st);
+hipError_t launch_join(void* psi, uint64_t first, uint64_t count, int prec, const double* re, const double* im, hipStream_t st);
+hipError_t launch_probs(const void* psi, uint64_t count, int prec, double* out, hipStream_t st);
+
+// Deterministic FP64 reductions.  Each writes `m` values into `out` (device),
+// using `part` (device, kRedBlocks * m doubles) as per-block partial sums.
+hipError_t reduce_norm(const void* psi, uint64_t count, int prec, double* part, double* out, hipStream_t st);
+hipError_t reduce_bit_probs(const void* psi, int n, int prec, int target, double* part, double* out, hipStream_t st);
+// Pauli strings sharing one X/Y flip mask: out[2s], out[2s+1] = Re, Im <psi|P_s|psi>.
+hipError_t reduce_pauli_group(const void* psi, int n, int prec, uint64_t xmask, int nstr, const uint64_t* zy_masks,
+                              const int* ny, double* part, double* out, hipStream_t st);
+
+// Density matrix (vectorised rho, row-major: element (r, c) is index r * 2^nq + c).
+// mode 0: trace; mode 1: <Z_q> for every q (nq values); mode 2: P(bit t = 0), P(bit t = 1) clamped later;
+// mode 3: diagonal Shannon entropy -sum p log2 p over p > 1e-14.
+hipError_t reduce_dm_diag(const void* rho, int nq, int mode, int target, double* part, double* out, hipStream_t st);
+hipError_t launch_dm_collapse(void* rho, int nq, int target, int outcome, double factor, hipStream_t st);
+
+// Sampling: inclusive FP64 prefix sum of probs into cdf, then lower_bound per uniform.
+size_t scan_scratch_doubles(uint64_t count);
+hipError_t launch_scan(const double* probs, double* cdf, uint64_t count, double* scratch, hipStream_t st);
+hipError_t launch_search(const double* cdf, uint64_t count, const double* u, uint64_t shots, uint64_t* out, hipStream_t st);
+
+// Reduced density matrix of `keep` (LSB-first local index), complex128 out [kd * kd].
+hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t* keep, int k, double2* out, hipStream_t st);
+
+}  // namespace iqg

@@ -1,0 +1,506 @@
+// Element-wise passes, FP64 reductions, sampling and partial trace.
+//
+// Reductions accumulate in FP64 whatever the amplitude precision, write one
+// partial per block, and a single-block finaliser sums the partials in a fixed
+// order: results are bit-reproducible run to run (no atomics on the result path).
+
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace iqg {
+
+namespace {
+
+inline unsigned blocks_for(uint64_t items, unsigned block, uint64_t cap) {
+    uint64_t g = (items + block - 1) / block;
+    g = std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+    return (unsigned)g;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_init_basis(cv<R>* __restrict__ psi, uint64_t count, uint64_t index) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        cv<R> v;
+        v.x = (i == index) ? (R)1 : (R)0;
+        v.y = 0;
+        psi[i] = v;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_collapse(cv<R>* __restrict__ psi, uint64_t count, int target, int outcome,
+                                                  double factor) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const R f = (R)factor;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        cv<R> v = psi[i];
+        if ((int)((i >> target) & 1ULL) == outcome) {
+            v.x *= f;
+            v.y *= f;
+        } else {
+            v.x = 0;
+            v.y = 0;
+        }
+        psi[i] = v;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_scale(cv<R>* __restrict__ psi, uint64_t count, double factor) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const R f = (R)factor;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        cv<R> v = psi[i];
+        v.x *= f;
+        v.y *= f;
+        psi[i] = v;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_split(const cv<R>* __restrict__ psi, uint64_t count, double* __restrict__ re,
+                                               double* __restrict__ im) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const cv<R> v = psi[i];
+        re[i] = (double)v.x;
+        im[i] = (double)v.y;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_join(cv<R>* __restrict__ psi, uint64_t count, const double* __restrict__ re,
+                                              const double* __restrict__ im) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        cv<R> v;
+        v.x = (R)re[i];
+        v.y = (R)im[i];
+        psi[i] = v;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_probs(const cv<R>* __restrict__ psi, uint64_t count, double* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const cv<R> v = psi[i];
+        const double x = (double)v.x, y = (double)v.y;
+        out[i] = x * x + y * y;
+    }
+}
+
+// ------------------------------------------------------------ reductions
+
+template <int M>
+__device__ __forceinline__ void block_sum_store(double (&v)[M], int m_used, double* __restrict__ part) {
+    __shared__ double sh[kRedThreads / 64][M];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        if (m < m_used) {
+            double x = v[m];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+            if (lane == 0) sh[wave][m] = x;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < m_used) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kRedThreads / 64; ++w) s += sh[w][threadIdx.x];
+        part[(uint64_t)blockIdx.x * m_used + threadIdx.x] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_finalize(const double* __restrict__ part, int nblocks, int m_used,
+                                                  double* __restrict__ out) {
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int m = 0; m < m_used; ++m) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += part[(uint64_t)b * m_used + m];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) sh[wave] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) out[m] = sh[0] + sh[1] + sh[2] + sh[3];
+        __syncthreads();
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_red_norm(const cv<R>* __restrict__ psi, uint64_t count, double* __restrict__ part) {
+    double v[1] = {0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const cv<R> a = psi[i];
+        const double x = (double)a.x, y = (double)a.y;
+        v[0] += x * x + y * y;
+    }
+    block_sum_store<1>(v, 1, part);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_red_bitprob(const cv<R>* __restrict__ psi, uint64_t count, int target,
+                                                     double* __restrict__ part) {
+    double v[2] = {0.0, 0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const cv<R> a = psi[i];
+        const double x = (double)a.x, y = (double)a.y;
+        const double p = x * x + y * y;
+        if ((i >> target) & 1ULL) v[1] += p;
+        else v[0] += p;
+    }
+    block_sum_store<2>(v, 2, part);
+}
+
+constexpr int kPauliPerGroup = 8;
+
+struct PauliGroup {
+    uint64_t xmask;
+    uint64_t zy[kPauliPerGroup];
+    int ny[kPauliPerGroup];
+    int nstr;
+};
+
+// <psi|P|psi> = sum_i conj(psi[i ^ x]) * i^{nY} * (-1)^{popcount(i & (Y|Z))} * psi[i],
+// skipping |psi_i|^2 < 1e-18 (reference core/src/statevector.cpp:265-288).
+template <typename R>
+__global__ void __launch_bounds__(256) k_red_pauli(const cv<R>* __restrict__ psi, uint64_t count, PauliGroup g,
+                                                   double* __restrict__ part) {
+    double v[2 * kPauliPerGroup];
+#pragma unroll
+    for (int m = 0; m < 2 * kPauliPerGroup; ++m) v[m] = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const cv<R> a = psi[i];
+        const double ar = (double)a.x, ai = (double)a.y;
+        if (ar * ar + ai * ai < 1e-18) continue;
+        const cv<R> b = psi[i ^ g.xmask];
+        const double br = (double)b.x, bi = (double)b.y;
+        // conj(b) * a
+        const double pr = br * ar + bi * ai;
+        const double pi = br * ai - bi * ar;
+#pragma unroll
+        for (int s = 0; s < kPauliPerGroup; ++s) {
+            if (s >= g.nstr) break;
+            const double sg = (__popcll(i & g.zy[s]) & 1) ? -1.0 : 1.0;
+            double cr = sg * pr, ci = sg * pi;
+            switch (g.ny[s] & 3) {
+                case 1: { const double t = cr; cr = -ci; ci = t; break; }    // * i
+                case 2: cr = -cr; ci = -ci; break;                           // * -1
+                case 3: { const double t = cr; cr = ci; ci = -t; break; }    // * -i
+                default: break;
+            }
+            v[2 * s] += cr;
+            v[2 * s + 1] += ci;
+        }
+    }
+    block_sum_store<2 * kPauliPerGroup>(v, 2 * g.nstr, part);
+}
+
+__global__ void __launch_bounds__(256) k_red_dm_diag(const double2* __restrict__ rho, int nq, int mode, int target,
+                                                     double* __restrict__ part) {
+    double v[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = 0.0;
+    const uint64_t dim = 1ULL << nq;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        const double d = rho[i * (dim + 1)].x;
+        if (mode == 0) {
+            v[0] += d;
+        } else if (mode == 1) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if (q < nq) v[q] += ((i >> q) & 1ULL) ? -d : d;
+        } else if (mode == 2) {
+            if ((i >> target) & 1ULL) v[1] += d;
+            else v[0] += d;
+        } else {
+            const double p = d > 0.0 ? d : 0.0;
+            if (p > 1e-14) v[0] -= p * log2(p);
+        }
+    }
+    const int m_used = mode == 1 ? nq : (mode == 2 ? 2 : 1);
+    block_sum_store<16>(v, m_used, part);
+}
+
+__global__ void __launch_bounds__(256) k_dm_collapse(double2* __restrict__ rho, int nq, int target, int outcome,
+                                                     double factor) {
+    const uint64_t count = 1ULL << (2 * nq);
+    const uint64_t cmask = (1ULL << nq) - 1;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint64_t r = i >> nq, c = i & cmask;
+        double2 v = rho[i];
+        if ((int)((r >> target) & 1ULL) == outcome && (int)((c >> target) & 1ULL) == outcome) {
+            v.x *= factor;
+            v.y *= factor;
+        } else {
+            v.x = 0.0;
+            v.y = 0.0;
+        }
+        rho[i] = v;
+    }
+}
+
+// --------------------------------------------------------------- scan
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr uint64_t kScanChunk = (uint64_t)kScanThreads * kScanItems;
+
+__device__ __forceinline__ double block_exclusive_scan(double x, double* total) {
+    __shared__ double sh[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double incl = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) sh[wave] = incl;
+    __syncthreads();
+    double before = 0.0;
+    for (int w = 0; w < wave; ++w) before += sh[w];
+    if (total) *total = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return before + incl - x;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_chunks(const double* __restrict__ p, double* __restrict__ cdf,
+                                                              uint64_t count, double* __restrict__ totals) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
+    double loc[kScanItems];
+    double run = 0.0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k;
+        run += (i < count) ? p[i] : 0.0;
+        loc[k] = run;
+    }
+    double total = 0.0;
+    const double off = block_exclusive_scan(run, &total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k;
+        if (i < count) cdf[i] = off + loc[k];
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
+// Exclusive scan of the chunk totals, in place, by one block.
+__global__ void __launch_bounds__(kScanThreads) k_scan_totals(double* __restrict__ totals, uint64_t nchunks) {
+    double carry = 0.0;
+    for (uint64_t b0 = 0; b0 < nchunks; b0 += kScanThreads) {
+        const uint64_t b = b0 + threadIdx.x;
+        const double x = (b < nchunks) ? totals[b] : 0.0;
+        double t = 0.0;
+        const double ex = block_exclusive_scan(x, &t);
+        if (b < nchunks) totals[b] = carry + ex;
+        carry += t;
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_add(double* __restrict__ cdf, uint64_t count,
+                                                           const double* __restrict__ offsets) {
+    const double off = offsets[blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    for (uint64_t k = threadIdx.x; k < kScanChunk; k += kScanThreads) {
+        const uint64_t i = base + k;
+        if (i < count) cdf[i] += off;
+    }
+}
+
+// lower_bound (first index with cdf[idx] >= u), clamped to count - 1
+// (reference core/src/statevector.cpp:248-254).
+__global__ void __launch_bounds__(256) k_search(const double* __restrict__ cdf, uint64_t count, const double* __restrict__ u,
+                                                uint64_t shots, uint64_t* __restrict__ out) {
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= shots) return;
+    const double x = u[s];
+    uint64_t lo = 0, hi = count;
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        if (cdf[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    out[s] = lo < count ? lo : count - 1;
+}
+
+// ---------------------------------------------------------- partial trace
+
+struct KeepList {
+    uint32_t q[16];
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_partial_trace(const cv<R>* __restrict__ psi, KeepList keep, int k,
+                                                       uint64_t envmask, double2* __restrict__ out) {
+    const uint64_t kd = 1ULL << k;
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= kd * kd) return;
+    const uint64_t r = id / kd, c = id % kd;
+    uint64_t ir = 0, ic = 0;
+    for (int b = 0; b < k; ++b) {
+        if ((r >> b) & 1ULL) ir |= 1ULL << keep.q[b];
+        if ((c >> b) & 1ULL) ic |= 1ULL << keep.q[b];
+    }
+    double sr = 0.0, si = 0.0;
+    uint64_t e = 0;
+    do {
+        const cv<R> a = psi[ir | e];
+        const cv<R> b = psi[ic | e];
+        const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+        sr += ar * br + ai * bi;  // a * conj(b)
+        si += ai * br - ar * bi;
+        e = (e - envmask) & envmask;
+    } while (e != 0);
+    out[id] = make_double2(sr, si);
+}
+
+}  // namespace
+
+hipError_t launch_init_basis(void* psi, int n, int prec, uint64_t index, hipStream_t st) {
+    const uint64_t count = 1ULL << n;
+    const unsigned b = blocks_for(count, 256, 65536);
+    if (prec == kC128) hipLaunchKernelGGL(k_init_basis<double>, dim3(b), dim3(256), 0, st, (double2*)psi, count, index);
+    else hipLaunchKernelGGL(k_init_basis<float>, dim3(b), dim3(256), 0, st, (float2*)psi, count, index);
+    return hipGetLastError();
+}
+
+hipError_t launch_collapse(void* psi, int n, int prec, int target, int outcome, double factor, hipStream_t st) {
+    const uint64_t count = 1ULL << n;
+    const unsigned b = blocks_for(count, 256, 65536);
+    if (prec == kC128)
+        hipLaunchKernelGGL(k_collapse<double>, dim3(b), dim3(256), 0, st, (double2*)psi, count, target, outcome, factor);
+    else hipLaunchKernelGGL(k_collapse<float>, dim3(b), dim3(256), 0, st, (float2*)psi, count, target, outcome, factor);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(void* psi, uint64_t count, int prec, double factor, hipStream_t st) {
+    const unsigned b = blocks_for(count, 256, 65536);
+    if (prec == kC128) hipLaunchKernelGGL(k_scale<double>, dim3(b), dim3(256), 0, st, (double2*)psi, count, factor);
+    else hipLaunchKernelGGL(k_scale<float>, dim3(b), dim3(256), 0, st, (float2*)psi, count, factor);
+    return hipGetLastError();
+}
+
+hipError_t launch_split(const void* psi, uint64_t first, uint64_t count, int prec, double* re, double* im, hipStream_t st) {
+    const unsigned b = blocks_for(count, 256, 65536);
+    if (prec == kC128)
+        hipLaunchKernelGGL(k_split<double>, dim3(b), dim3(256), 0, st, (const double2*)psi + first, count, re, im);
+    else hipLaunchKernelGGL(k_split<float>, dim3(b), dim3(256), 0, st, (const float2*)psi + first, count, re, im);
+    return hipGetLastError();
+}
+
+hipError_t launch_join(void* psi, uint64_t first, uint64_t count, int prec, const double* re, const double* im,
+                       hipStream_t st) {
+    const unsigned b = blocks_for(count, 256, 65536);
+    if (prec == kC128) hipLaunchKernelGGL(k_join<double>, dim3(b), dim3(256), 0, st, (double2*)psi + first, count, re, im);
+    else hipLaunchKernelGGL(k_join<float>, dim3(b), dim3(256), 0, st, (float2*)psi + first, count, re, im);
+    return hipGetLastError();
+}
+
+hipError_t launch_probs(const void* psi, uint64_t count, int prec, double* out, hipStream_t st) {
+    const unsigned b = blocks_for(count, 256, 65536);
+    if (prec == kC128) hipLaunchKernelGGL(k_probs<double>, dim3(b), dim3(256), 0, st, (const double2*)psi, count, out);
+    else hipLaunchKernelGGL(k_probs<float>, dim3(b), dim3(256), 0, st, (const float2*)psi, count, out);
+    return hipGetLastError();
+}
+
+static hipError_t finalize(const double* part, unsigned nblocks, int m, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, part, (int)nblocks, m, out);
+    return hipGetLastError();
+}
+
+hipError_t reduce_norm(const void* psi, uint64_t count, int prec, double* part, double* out, hipStream_t st) {
+    const unsigned b = blocks_for(count, kRedThreads, kRedBlocks);
+    if (prec == kC128) hipLaunchKernelGGL(k_red_norm<double>, dim3(b), dim3(kRedThreads), 0, st, (const double2*)psi, count, part);
+    else hipLaunchKernelGGL(k_red_norm<float>, dim3(b), dim3(kRedThreads), 0, st, (const float2*)psi, count, part);
+    return finalize(part, b, 1, out, st);
+}
+
+hipError_t reduce_bit_probs(const void* psi, int n, int prec, int target, double* part, double* out, hipStream_t st) {
+    const uint64_t count = 1ULL << n;
+    const unsigned b = blocks_for(count, kRedThreads, kRedBlocks);
+    if (prec == kC128)
+        hipLaunchKernelGGL(k_red_bitprob<double>, dim3(b), dim3(kRedThreads), 0, st, (const double2*)psi, count, target, part);
+    else hipLaunchKernelGGL(k_red_bitprob<float>, dim3(b), dim3(kRedThreads), 0, st, (const float2*)psi, count, target, part);
+    return finalize(part, b, 2, out, st);
+}
+
+hipError_t reduce_pauli_group(const void* psi, int n, int prec, uint64_t xmask, int nstr, const uint64_t* zy_masks,
+                              const int* ny, double* part, double* out, hipStream_t st) {
+    if (nstr < 1 || nstr > kPauliPerGroup) return hipErrorInvalidValue;
+    PauliGroup g{};
+    g.xmask = xmask;
+    g.nstr = nstr;
+    for (int s = 0; s < nstr; ++s) {
+        g.zy[s] = zy_masks[s];
+        g.ny[s] = ny[s];
+    }
+    const uint64_t count = 1ULL << n;
+    const unsigned b = blocks_for(count, kRedThreads, kRedBlocks);
+    if (prec == kC128)
+        hipLaunchKernelGGL(k_red_pauli<double>, dim3(b), dim3(kRedThreads), 0, st, (const double2*)psi, count, g, part);
+    else hipLaunchKernelGGL(k_red_pauli<float>, dim3(b), dim3(kRedThreads), 0, st, (const float2*)psi, count, g, part);
+    return finalize(part, b, 2 * nstr, out, st);
+}
+
+hipError_t reduce_dm_diag(const void* rho, int nq, int mode, int target, double* part, double* out, hipStream_t st) {
+    if (nq > 16) return hipErrorInvalidValue;
+    const uint64_t dim = 1ULL << nq;
+    const unsigned b = blocks_for(dim, kRedThreads, kRedBlocks);
+    hipLaunchKernelGGL(k_red_dm_diag, dim3(b), dim3(kRedThreads), 0, st, (const double2*)rho, nq, mode, target, part);
+    const int m = mode == 1 ? nq : (mode == 2 ? 2 : 1);
+    return finalize(part, b, m, out, st);
+}
+
+hipError_t launch_dm_collapse(void* rho, int nq, int target, int outcome, double factor, hipStream_t st) {
+    const uint64_t count = 1ULL << (2 * nq);
+    const unsigned b = blocks_for(count, 256, 65536);
+    hipLaunchKernelGGL(k_dm_collapse, dim3(b), dim3(256), 0, st, (double2*)rho, nq, target, outcome, factor);
+    return hipGetLastError();
+}
+
+size_t scan_scratch_doubles(uint64_t count) { return (size_t)((count + kScanChunk - 1) / kScanChunk); }
+
+hipError_t launch_scan(const double* probs, double* cdf, uint64_t count, double* scratch, hipStream_t st) {
+    const uint64_t nchunks = (count + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)nchunks), dim3(kScanThreads), 0, st, probs, cdf, count, scratch);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, st, scratch, nchunks);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nchunks), dim3(kScanThreads), 0, st, cdf, count, scratch);
+    return hipGetLastError();
+}
+
+hipError_t launch_search(const double* cdf, uint64_t count, const double* u, uint64_t shots, uint64_t* out, hipStream_t st) {
+    if (shots == 0) return hipSuccess;
+    const unsigned b = blocks_for(shots, 256, 1ULL << 30);
+    hipLaunchKernelGGL(k_search, dim3(b), dim3(256), 0, st, cdf, count, u, shots, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t* keep, int k, double2* out, hipStream_t st) {
+    if (k < 1 || k > 16) return hipErrorInvalidValue;
+    KeepList kl{};
+    uint64_t kmask = 0;
+    for (int b = 0; b < k; ++b) {
+        kl.q[b] = keep[b];
+        kmask |= 1ULL << keep[b];
+    }
+    const uint64_t envmask = ((1ULL << n) - 1) & ~kmask;
+    const uint64_t total = (1ULL << k) * (1ULL << k);
+    const unsigned b = blocks_for(total, 256, 1ULL << 30);
+    if (prec == kC128)
+        hipLaunchKernelGGL(k_partial_trace<double>, dim3(b), dim3(256), 0, st, (const double2*)psi, kl, k, envmask, out);
+    else hipLaunchKernelGGL(k_partial_trace<float>, dim3(b), dim3(256), 0, st, (const float2*)psi, kl, k, envmask, out);
+    return hipGetLastError();
+}
+
+}  // namespace iqg

@@ -1,0 +1,142 @@
+// Host-only check of the gate kernels' index layout (invariantq_amd/csrc/layout.hpp),
+// built with g++ by tests/test_layout.py.  For every target list given on stdin
+// ("n K q0 .. qK-1" per line) it replays the kernels' address arithmetic and
+// verifies, without touching a GPU:
+//   direct: every group member index is < 2^n, each amplitude belongs to exactly
+//           one (group, member) pair, and member s has the MSB-first target bits of s;
+//   tiled:  every global address is < 2^n and each amplitude is loaded exactly once;
+//           the LDS positions of a tile are a permutation of [0, 64*2^K);
+//           the element a lane gathers for local index s is the amplitude whose
+//           target bits spell s and whose other bits are that lane's group;
+//           16 consecutive lanes hit 16 distinct 16-element slots in both passes.
+// Prints "ok" or the first violation.
+#include <cstdio>
+#include <cstdlib>
+#include <set>
+#include <vector>
+
+#include "layout.hpp"
+
+using namespace iqg;
+
+static int fail(const char* what, int n, int K) {
+    std::printf("FAIL %s (n=%d K=%d)\n", what, n, K);
+    return 1;
+}
+
+template <int K>
+static int check(int n, const uint32_t* q) {
+    const uint64_t N = 1ULL << n;
+    const int D = 1 << K;
+    auto local_of = [&](uint64_t idx) {
+        uint32_t l = 0;
+        for (int m = 0; m < K; ++m) l |= (uint32_t)((idx >> q[m]) & 1ULL) << (K - 1 - m);
+        return l;
+    };
+    uint64_t tmask = 0;
+    for (int m = 0; m < K; ++m) tmask |= 1ULL << q[m];
+    {  // direct
+        DirectArgs<K> a;
+        fill_direct_args<K>(a, q);
+        std::vector<unsigned char> seen(N, 0);
+        for (uint64_t g = 0; g < (N >> K); ++g) {
+            const uint64_t base = insert_zero_bits<K>(g, a.pos);
+            if (base & tmask) return fail("direct base has target bits", n, K);
+            for (int s = 0; s < D; ++s) {
+                const uint64_t idx = base + a.off[s];
+                if (idx >= N) return fail("direct index out of range", n, K);
+                if (seen[idx]++) return fail("direct index repeated", n, K);
+                if (local_of(idx) != (uint32_t)s) return fail("direct local index", n, K);
+            }
+        }
+    }
+    if (n < K + 6) return 0;
+    TiledArgs<K> a;
+    fill_tiled_args<K>(a, n, q);
+    const int T = 64 * D;
+    std::vector<unsigned char> seen(N, 0);
+    for (uint64_t t = 0; t < a.ntiles; ++t) {
+        std::vector<uint64_t> lds(T, ~0ULL);  // LDS position -> global index
+        for (uint32_t lane = 0; lane < 64; ++lane) {
+            uint32_t pl, pg;
+            tile_lane_positions<K>(lane, a, pl, pg);
+            const uint64_t base = deposit_bits(t, a.rest) + lane;
+            for (int m = 0; m < D; ++m) {
+                const uint64_t gidx = base + a.off[m];
+                const uint32_t pos = pl ^ a.pm[m];
+                if (gidx >= N) return fail("tiled global index out of range", n, K);
+                if (pos >= (uint32_t)T) return fail("tiled LDS position out of range", n, K);
+                if (seen[gidx]++) return fail("tiled global index repeated", n, K);
+                if (lds[pos] != ~0ULL) return fail("tiled LDS position repeated", n, K);
+                lds[pos] = gidx;
+            }
+        }
+        std::set<uint64_t> groups;
+        for (uint32_t lane = 0; lane < 64; ++lane) {
+            uint32_t pl, pg;
+            tile_lane_positions<K>(lane, a, pl, pg);
+            uint64_t rest_bits = ~0ULL;
+            for (int s = 0; s < D; ++s) {
+                const uint32_t pos = pg ^ a.ts[s];
+                if (pos >= (uint32_t)T) return fail("gather position out of range", n, K);
+                const uint64_t gidx = lds[pos];
+                if (local_of(gidx) != (uint32_t)s) return fail("gather local index", n, K);
+                const uint64_t r = gidx & ~tmask;
+                if (s == 0) rest_bits = r;
+                else if (r != rest_bits) return fail("gather mixes groups", n, K);
+            }
+            if (!groups.insert(rest_bits).second) return fail("two lanes share a group", n, K);
+        }
+        // bank slots (16 elements of 16 B = 256 B) for 16-lane phases
+        for (uint32_t l0 = 0; l0 < 64; l0 += 16) {
+            for (int m = 0; m < D; ++m) {
+                std::set<uint32_t> slots;
+                for (uint32_t lane = l0; lane < l0 + 16; ++lane) {
+                    uint32_t pl, pg;
+                    tile_lane_positions<K>(lane, a, pl, pg);
+                    slots.insert((pl ^ a.pm[m]) & 15u);
+                }
+                if (slots.size() != 16) return fail("coalesced pass bank conflict", n, K);
+            }
+            for (int s = 0; s < D; ++s) {
+                std::set<uint32_t> slots;
+                for (uint32_t lane = l0; lane < l0 + 16; ++lane) {
+                    uint32_t pl, pg;
+                    tile_lane_positions<K>(lane, a, pl, pg);
+                    slots.insert((pg ^ a.ts[s]) & 15u);
+                }
+                if (slots.size() != 16) return fail("gather pass bank conflict", n, K);
+            }
+        }
+    }
+    for (uint64_t i = 0; i < N; ++i)
+        if (seen[i] != 1) return fail("tiled pass missed an amplitude", n, K);
+    return 0;
+}
+
+int main() {
+    int n, K, cases = 0;
+    while (std::scanf("%d %d", &n, &K) == 2) {
+        uint32_t q[8];
+        for (int m = 0; m < K; ++m)
+            if (std::scanf("%u", &q[m]) != 1) return 1;
+        int r = 0;
+        switch (K) {
+            case 1: r = check<1>(n, q); break;
+            case 2: r = check<2>(n, q); break;
+            case 3: r = check<3>(n, q); break;
+            case 4: r = check<4>(n, q); break;
+            case 5: r = check<5>(n, q); break;
+            default: std::printf("FAIL bad K\n"); return 1;
+        }
+        if (r) {
+            std::printf("case n=%d K=%d q=", n, K);
+            for (int m = 0; m < K; ++m) std::printf("%u ", q[m]);
+            std::printf("\n");
+            return 1;
+        }
+        ++cases;
+    }
+    std::printf("ok %d\n", cases);
+    return 0;
+}
