@@ -135,6 +135,38 @@ void   iq_density_matrix_expval_z(void* dm, double* out);
 void   iq_density_matrix_synchronize(void* dm);
 void*  iq_density_matrix_stream(void* dm);
 
+/* ------------------------------------------------------- parametric circuits
+ * No reference counterpart (the reference's VQE is gradient-free,
+ * invariantq/benchmarks/vqe.py:81-98).  A circuit handle owns two device states
+ * (psi and the adjoint state) and a gate tape; gates follow
+ * invariantq/ir/qir.py:90-136, Pauli strings core/src/statevector.cpp:258-291. */
+#define IQ_GATE_RX 0      /* param_index[0] = theta */
+#define IQ_GATE_RY 1
+#define IQ_GATE_RZ 2
+#define IQ_GATE_U3 3      /* param_index[0..2] = theta, phi, lambda */
+#define IQ_GATE_CZ 4
+#define IQ_GATE_CX 5      /* q0 = control */
+#define IQ_GATE_MATRIX1 6 /* fixed 2x2, m_re/m_im row-major */
+#define IQ_GATE_MATRIX2 7 /* fixed 4x4, local index 2*bit(q0) + bit(q1) */
+
+void*  iq_circuit_create(size_t num_qubits, int precision);
+void   iq_circuit_free(void* c);
+int    iq_circuit_add_gate(void* c, int kind, size_t q0, size_t q1, const int64_t* param_index,
+                           const double* m_re, const double* m_im);
+size_t iq_circuit_num_gates(void* c);
+size_t iq_circuit_num_segments(void* c);
+/* Observable sum_t coeffs[t] P_t, strings packed as in iq_statevector_expval_pauli_sum. */
+int    iq_circuit_set_observable(void* c, size_t nterms, const double* coeffs, const char* ops,
+                                 const size_t* qubits, const size_t* lens);
+double iq_circuit_expectation(void* c, const double* params, size_t nparams);
+/* Adjoint-method <H> and d<H>/dparams (FP64 accumulation on the GPU). */
+int    iq_circuit_gradient(void* c, const double* params, size_t nparams, double* energy_out, double* grad_out);
+/* Parameter-shift gradient on the same engine for params listed in `which` (all if NULL). */
+int    iq_circuit_parameter_shift(void* c, const double* params, size_t nparams, const int64_t* which,
+                                  size_t nwhich, double* grad_out);
+void   iq_circuit_get_amplitudes(void* c, double* real_out, double* imag_out);
+void*  iq_circuit_stream(void* c);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,16 @@ _LIB_DIRS = [
 ]
 _LIB_NAMES = ("libinvariantq.so",)
 
+# One HIP runtime per process: load the HSA and HIP runtimes under their bare
+# names first.  A later `import torch` (whose libraries ask for these bare
+# names) then binds to the same copies instead of loading the ones it bundles;
+# if torch was imported first, these calls return its already-loaded copies.
+for _rt in ("libhsa-runtime64.so", "libamdhip64.so"):
+    try:
+        ctypes.CDLL(_rt, mode=ctypes.RTLD_GLOBAL)
+    except OSError:
+        pass
+
 _NATIVE_LIB = None
 _NATIVE_LOAD_ERRORS: List[str] = []
 _LIB_PATH: Optional[Path] = None
@@ -134,6 +144,17 @@ SIGNATURES = {
     "iq_density_matrix_expval_z": ([_vp, _d], None),
     "iq_density_matrix_synchronize": ([_vp], None),
     "iq_density_matrix_stream": ([_vp], _vp),
+    "iq_circuit_create": ([_sz, ctypes.c_int], _vp),
+    "iq_circuit_free": ([_vp], None),
+    "iq_circuit_add_gate": ([_vp, ctypes.c_int, _sz, _sz, ctypes.POINTER(ctypes.c_int64), _d, _d], ctypes.c_int),
+    "iq_circuit_num_gates": ([_vp], _sz),
+    "iq_circuit_num_segments": ([_vp], _sz),
+    "iq_circuit_set_observable": ([_vp, _sz, _d, ctypes.c_char_p, _s, _s], ctypes.c_int),
+    "iq_circuit_expectation": ([_vp, _d, _sz], ctypes.c_double),
+    "iq_circuit_gradient": ([_vp, _d, _sz, _d, _d], ctypes.c_int),
+    "iq_circuit_parameter_shift": ([_vp, _d, _sz, ctypes.POINTER(ctypes.c_int64), _sz, _d], ctypes.c_int),
+    "iq_circuit_get_amplitudes": ([_vp, _d, _d], None),
+    "iq_circuit_stream": ([_vp], _vp),
 }
 
 REFERENCE_NAMES = list(SIGNATURES)[:49]

@@ -38,37 +38,57 @@ struct ValMat {
     R v[N];
 };
 
+// y = U x for one group held in registers.  Rows run in a runtime loop for
+// D >= 8: each iteration streams one matrix row through SGPRs (scalar loads),
+// which bounds SGPR pressure; a fully unrolled 32x32 product would hoist all
+// 2048-3072 coefficients and spill.  Two partial sums per chain shorten the
+// dependent-FMA chains.
 template <typename R, int D, bool M3, typename Get, typename Put>
 __device__ __forceinline__ void matvec(const R (&xr)[D], const R (&xi)[D], Get m, Put put) {
+    constexpr int kW = M3 ? 3 * D : 2 * D;  // coefficients per row
     if constexpr (M3) {
         R xs[D];
 #pragma unroll
         for (int c = 0; c < D; ++c) xs[c] = xr[c] + xi[c];
-#pragma unroll
+#pragma unroll(D >= 8 ? 1 : D)
         for (int r = 0; r < D; ++r) {
-            R t1 = 0, t2 = 0, t3 = 0;
+            R t1a = 0, t2a = 0, t3a = 0, t1b = 0, t2b = 0, t3b = 0;
 #pragma unroll
-            for (int c = 0; c < D; ++c) {
-                t1 = fma(m(r * D + c), xr[c], t1);
-                t2 = fma(m(D * D + r * D + c), xi[c], t2);
-                t3 = fma(m(2 * D * D + r * D + c), xs[c], t3);
+            for (int c = 0; c < D; c += 2) {
+                t1a = fma(m(r * kW + c), xr[c], t1a);
+                t2a = fma(m(r * kW + D + c), xi[c], t2a);
+                t3a = fma(m(r * kW + 2 * D + c), xs[c], t3a);
+                if (c + 1 < D) {
+                    t1b = fma(m(r * kW + c + 1), xr[c + 1], t1b);
+                    t2b = fma(m(r * kW + D + c + 1), xi[c + 1], t2b);
+                    t3b = fma(m(r * kW + 2 * D + c + 1), xs[c + 1], t3b);
+                }
             }
+            const R t1 = t1a + t1b, t2 = t2a + t2b, t3 = t3a + t3b;
             put(r, t1 - t2, t3 - t1 - t2);
         }
     } else {
-#pragma unroll
+#pragma unroll(D >= 8 ? 1 : D)
         for (int r = 0; r < D; ++r) {
-            R yr = 0, yi = 0;
+            R yra = 0, yia = 0, yrb = 0, yib = 0;
 #pragma unroll
-            for (int c = 0; c < D; ++c) {
-                const R a = m(r * D + c);
-                const R b = m(D * D + r * D + c);
-                yr = fma(a, xr[c], yr);
-                yr = fma(-b, xi[c], yr);
-                yi = fma(a, xi[c], yi);
-                yi = fma(b, xr[c], yi);
+            for (int c = 0; c < D; c += 2) {
+                const R a = m(r * kW + c);
+                const R b = m(r * kW + D + c);
+                yra = fma(a, xr[c], yra);
+                yra = fma(-b, xi[c], yra);
+                yia = fma(a, xi[c], yia);
+                yia = fma(b, xr[c], yia);
+                if (c + 1 < D) {
+                    const R a1 = m(r * kW + c + 1);
+                    const R b1 = m(r * kW + D + c + 1);
+                    yrb = fma(a1, xr[c + 1], yrb);
+                    yrb = fma(-b1, xi[c + 1], yrb);
+                    yib = fma(a1, xi[c + 1], yib);
+                    yib = fma(b1, xr[c + 1], yib);
+                }
             }
-            put(r, yr, yi);
+            put(r, yra + yrb, yia + yib);
         }
     }
 }
@@ -129,34 +149,34 @@ __global__ void __launch_bounds__(64) k_gate_tiled(cv<R>* __restrict__ psi, Tile
     const uint32_t lane = threadIdx.x;
     uint32_t pl, pg;
     tile_lane_positions<K>(lane, a, pl, pg);
-    for (uint64_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-        const uint64_t base = deposit_bits(t, a.rest) + lane;
+    const uint64_t t = blockIdx.x;
+    const uint64_t base = deposit_bits(t, a.rest) + lane;
+    {
         cv<R> v[D];
 #pragma unroll
         for (int m = 0; m < D; ++m) v[m] = psi[base + a.off[m]];
 #pragma unroll
         for (int m = 0; m < D; ++m) tile[pl ^ a.pm[m]] = v[m];
-        __syncthreads();
-        R xr[D], xi[D];
-#pragma unroll
-        for (int s = 0; s < D; ++s) {
-            const cv<R> x = tile[pg ^ a.ts[s]];
-            xr[s] = x.x;
-            xi[s] = x.y;
-        }
-        matvec<R, D, M3>(
-            xr, xi, [&](int i) { return mat[i]; },
-            [&](int r, R yr, R yi) {
-                cv<R> o;
-                o.x = yr;
-                o.y = yi;
-                tile[pg ^ a.ts[r]] = o;
-            });
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < D; ++m) psi[base + a.off[m]] = tile[pl ^ a.pm[m]];
-        __syncthreads();
     }
+    __syncthreads();
+    R xr[D], xi[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        const cv<R> x = tile[pg ^ a.ts[s]];
+        xr[s] = x.x;
+        xi[s] = x.y;
+    }
+    matvec<R, D, M3>(
+        xr, xi, [&](int i) { return mat[i]; },
+        [&](int r, R yr, R yi) {
+            cv<R> o;
+            o.x = yr;
+            o.y = yi;
+            tile[pg ^ a.ts[r]] = o;
+        });
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < D; ++m) psi[base + a.off[m]] = tile[pl ^ a.pm[m]];
 }
 
 template <typename R, int K>
@@ -210,8 +230,9 @@ template <typename R, int K, bool M3>
 hipError_t run_tiled(void* psi, int n, hipStream_t st, const GateDesc& g) {
     TiledArgs<K> a;
     fill_tiled_args<K>(a, n, g.q);
-    const unsigned blocks = (unsigned)std::min<uint64_t>(a.ntiles, 256ULL * 32ULL);
-    hipLaunchKernelGGL((k_gate_tiled<R, K, M3>), dim3(blocks), dim3(64), 0, st, (cv<R>*)psi, a, (const R*)g.dmat);
+    if (a.ntiles > 0x7fffffffULL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_gate_tiled<R, K, M3>), dim3((unsigned)a.ntiles), dim3(64), 0, st, (cv<R>*)psi, a,
+                       (const R*)g.dmat);
     return hipGetLastError();
 }
 
@@ -261,23 +282,20 @@ size_t gate_matrix_bytes(int prec, int K, bool three_mult) {
     return (three_mult ? 3 : 2) * d * d * (prec == kC128 ? sizeof(double) : sizeof(float));
 }
 
+// Row r occupies [w*r, w*(r+1)) with w = 2D (A_r, B_r) or 3D (A_r, B_r, S_r),
+// so one matrix row is one contiguous run of scalar loads.
 void pack_gate_matrix(int prec, int K, bool three_mult, const double* re, const double* im, void* out) {
-    const size_t d = 1ULL << K, dd = d * d;
-    if (prec == kC128) {
-        double* o = static_cast<double*>(out);
-        for (size_t i = 0; i < dd; ++i) {
-            o[i] = re[i];
-            o[dd + i] = im[i];
-            if (three_mult) o[2 * dd + i] = re[i] + im[i];
+    const size_t d = 1ULL << K, w = (three_mult ? 3 : 2) * d;
+    for (size_t r = 0; r < d; ++r)
+        for (size_t c = 0; c < d; ++c) {
+            const double a = re[r * d + c], b = im[r * d + c];
+            const double v[3] = {a, b, a + b};
+            for (int part = 0; part < (three_mult ? 3 : 2); ++part) {
+                const size_t at = r * w + part * d + c;
+                if (prec == kC128) static_cast<double*>(out)[at] = v[part];
+                else static_cast<float*>(out)[at] = (float)v[part];
+            }
         }
-    } else {
-        float* o = static_cast<float*>(out);
-        for (size_t i = 0; i < dd; ++i) {
-            o[i] = (float)re[i];
-            o[dd + i] = (float)im[i];
-            if (three_mult) o[2 * dd + i] = (float)(re[i] + im[i]);
-        }
-    }
 }
 
 int choose_path(int n, int prec, const GateDesc& g) {

@@ -41,6 +41,8 @@ hipError_t launch_probs(const void* psi, uint64_t count, int prec, double* out, 
 
 // Deterministic FP64 reductions.  Each writes `m` values into `out` (device),
 // using `part` (device, kRedBlocks * m doubles) as per-block partial sums.
+// Sum `nblocks` per-block partials of m values each (layout part[b * m + j]) into out[0..m).
+hipError_t launch_finalize(const double* part, unsigned nblocks, int m, double* out, hipStream_t st);
 hipError_t reduce_norm(const void* psi, uint64_t count, int prec, double* part, double* out, hipStream_t st);
 hipError_t reduce_bit_probs(const void* psi, int n, int prec, int target, double* part, double* out, hipStream_t st);
 // Pauli strings sharing one X/Y flip mask: out[2s], out[2s+1] = Re, Im <psi|P_s|psi>.

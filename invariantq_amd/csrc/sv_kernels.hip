@@ -5,6 +5,7 @@
 // order: results are bit-reproducible run to run (no atomics on the result path).
 
 #include "kernels.hpp"
+#include "reduce.hpp"
 
 #include <algorithm>
 
@@ -93,28 +94,6 @@ __global__ void __launch_bounds__(256) k_probs(const cv<R>* __restrict__ psi, ui
 }
 
 // ------------------------------------------------------------ reductions
-
-template <int M>
-__device__ __forceinline__ void block_sum_store(double (&v)[M], int m_used, double* __restrict__ part) {
-    __shared__ double sh[kRedThreads / 64][M];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        if (m < m_used) {
-            double x = v[m];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-            if (lane == 0) sh[wave][m] = x;
-        }
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < m_used) {
-        double s = 0.0;
-#pragma unroll
-        for (int w = 0; w < kRedThreads / 64; ++w) s += sh[w][threadIdx.x];
-        part[(uint64_t)blockIdx.x * m_used + threadIdx.x] = s;
-    }
-}
 
 __global__ void __launch_bounds__(256) k_finalize(const double* __restrict__ part, int nblocks, int m_used,
                                                   double* __restrict__ out) {
@@ -417,6 +396,10 @@ hipError_t launch_probs(const void* psi, uint64_t count, int prec, double* out, 
 static hipError_t finalize(const double* part, unsigned nblocks, int m, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, part, (int)nblocks, m, out);
     return hipGetLastError();
+}
+
+hipError_t launch_finalize(const double* part, unsigned nblocks, int m, double* out, hipStream_t st) {
+    return finalize(part, nblocks, m, out, st);
 }
 
 hipError_t reduce_norm(const void* psi, uint64_t count, int prec, double* part, double* out, hipStream_t st) {

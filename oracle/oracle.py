@@ -386,3 +386,69 @@ def tfim_dense(n: int, hx: float = 0.5) -> np.ndarray:
     for i in range(n):
         H += hx * kron_on({i: PAULI["X"]})
     return H
+
+
+# ------------------------------------------------ parametric circuits (gradients)
+
+def _ry(t):
+    c, s = math.cos(t / 2), math.sin(t / 2)
+    return np.array([[c, -s], [s, c]], dtype=np.complex128)
+
+
+def _rz(t):
+    return np.array([[np.exp(-0.5j * t), 0], [0, np.exp(0.5j * t)]], dtype=np.complex128)
+
+
+def _rx(t):
+    c, s = math.cos(t / 2), math.sin(t / 2)
+    return np.array([[c, -1j * s], [-1j * s, c]], dtype=np.complex128)
+
+
+_CZ = np.diag([1, 1, 1, -1]).astype(np.complex128)
+
+
+def hea_ops(n: int, layers: int, params, ring: bool = True):
+    """RY on every qubit, RZ on every qubit, then CZ on (q, q+1) and (n-1, 0), per layer.
+    Parameter layout: layer l uses params[2nl + q] (RY) and params[2nl + n + q] (RZ)."""
+    p = np.asarray(params, dtype=np.float64)
+    ops = []
+    edges = [(q, q + 1) for q in range(n - 1)] + ([(n - 1, 0)] if ring and n > 2 else [])
+    for l in range(layers):
+        b = 2 * n * l
+        ops += [([q], _ry(p[b + q])) for q in range(n)]
+        ops += [([q], _rz(p[b + n + q])) for q in range(n)]
+        ops += [([a, c], _CZ) for a, c in edges]
+    return ops
+
+
+def energy(core: CoreLib, n: int, ops, terms) -> float:
+    """sum_t c_t Re <psi|P_t|psi> for psi = ops|0>, on the oracle (complex128)."""
+    sv = core.statevector(n)
+    for qs, m in ops:
+        (sv.apply_1q(qs[0], m) if len(qs) == 1 else sv.apply_2q(qs[0], qs[1], m))
+    return float(sum(c * sv.expectation_value_pauli(o, q).real for c, o, q in terms))
+
+
+def grad_parameter_shift(core: CoreLib, n: int, make_ops, terms, params, which=None) -> np.ndarray:
+    """FP64 parameter-shift rule, exact for one Pauli rotation per parameter."""
+    p = np.asarray(params, dtype=np.float64)
+    idx = range(p.size) if which is None else which
+    out = []
+    for j in idx:
+        a, b = p.copy(), p.copy()
+        a[j] += math.pi / 2
+        b[j] -= math.pi / 2
+        out.append(0.5 * (energy(core, n, make_ops(a), terms) - energy(core, n, make_ops(b), terms)))
+    return np.asarray(out)
+
+
+def grad_central_difference(core: CoreLib, n: int, make_ops, terms, params, h: float = 1e-6, which=None) -> np.ndarray:
+    p = np.asarray(params, dtype=np.float64)
+    idx = range(p.size) if which is None else which
+    out = []
+    for j in idx:
+        a, b = p.copy(), p.copy()
+        a[j] += h
+        b[j] -= h
+        out.append((energy(core, n, make_ops(a), terms) - energy(core, n, make_ops(b), terms)) / (2 * h))
+    return np.asarray(out)

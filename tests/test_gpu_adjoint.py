@@ -1,0 +1,154 @@
+"""Adjoint gradients on the GPU against FP64 references.
+
+References: the oracle's FP64 parameter-shift and central differences
+(h = 1e-6) on the CPU at small sizes; at the full 24-qubit size, the engine's
+own complex128 parameter-shift (its complex128 path is pinned to the oracle
+by tests/test_gpu_parity.py).  Gradient error bar: ||g - g_ref||_2 / ||g_ref||_2 <= 1e-4
+for the complex64 state with FP64 accumulation; 1e-10 absolute for complex128.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from invariantq_amd import gates, workloads
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def A():
+    from invariantq_amd import adjoint
+    from invariantq_amd.core import binding
+    assert binding.HAS_NATIVE_CORE
+    return adjoint
+
+
+def _hea(A, n, layers, precision):
+    c = A.GpuCircuit.from_ansatz(workloads.Ansatz(n, layers), precision)
+    c.set_observable(workloads.tfim_terms(n))
+    return c
+
+
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+def test_hea_forward_energy_and_gradient(A, oracle_lib, precision):
+    n, layers = 8, 3
+    rng = np.random.default_rng(10)
+    p = rng.uniform(0, 2 * np.pi, 2 * n * layers)
+    c = _hea(A, n, layers, precision)
+    assert c.num_gates == layers * (3 * n)
+    assert c.num_segments == layers * (n + 1)  # one pass per qubit's RY+RZ, one CZ pass
+    e_ref = orc.energy(oracle_lib, n, orc.hea_ops(n, layers, p), orc.tfim_terms(n))
+    e = c.expectation(p)
+    psi_ref = oracle_lib.statevector(n)
+    for qs, m in orc.hea_ops(n, layers, p):
+        (psi_ref.apply_1q(qs[0], m) if len(qs) == 1 else psi_ref.apply_2q(qs[0], qs[1], m))
+    fid = abs(np.vdot(psi_ref.get_amplitudes(), c.amplitudes())) ** 2
+    e2, g = c.gradient(p)
+    g_ref = orc.grad_parameter_shift(oracle_lib, n, lambda x: orc.hea_ops(n, layers, x), orc.tfim_terms(n), p)
+    if precision == "complex128":
+        assert abs(e - e_ref) < 1e-12 and abs(e2 - e_ref) < 1e-12
+        assert fid > 1 - 1e-12
+        assert np.max(np.abs(g - g_ref)) < 1e-10
+    else:
+        assert abs(e - e_ref) < 1e-5
+        assert fid > 1 - 1e-6
+        assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 1e-4
+
+
+def test_mixed_gates_against_central_differences(A, oracle_lib):
+    """RX / U3 / fixed matrices / CX through the adjoint, vs FP64 central differences."""
+    n = 5
+    rng = np.random.default_rng(11)
+    fixed1 = orc.haar_unitary(rng, 2)
+    fixed2 = orc.haar_unitary(rng, 4)
+    spec = [("rx", [0], [0]), ("u3", [1], [1, 2, 3]), ("cx", [1, 3], []), ("ry", [2], [4]), ("m1", [4], []),
+            ("rz", [3], [5]), ("m2", [4, 0], []), ("u3", [0], [6, 7, 8]), ("cz", [2, 4], []), ("rx", [4], [9]),
+            ("cx", [4, 2], []), ("ry", [1], [10]), ("rz", [1], [11]), ("u3", [3], [12, 13, 14])]
+    c = A.GpuCircuit(n, "complex128")
+    for name, qs, ps in spec:
+        if name == "m1":
+            c.add(A.MATRIX1, qs, matrix=fixed1)
+        elif name == "m2":
+            c.add(A.MATRIX2, qs, matrix=fixed2)
+        else:
+            c.add(name, qs, ps)
+    terms = workloads.tfim_terms(n) + [(0.7, "XY", [0, 3]), (-0.4, "YZ", [2, 4])]
+    c.set_observable(terms)
+
+    def make(p):
+        ops = []
+        for name, qs, ps in spec:
+            if name == "rx":
+                ops.append((qs, gates.rx(p[ps[0]])))
+            elif name == "ry":
+                ops.append((qs, gates.ry(p[ps[0]])))
+            elif name == "rz":
+                ops.append((qs, gates.rz(p[ps[0]])))
+            elif name == "u3":
+                ops.append((qs, gates.u3(p[ps[0]], p[ps[1]], p[ps[2]])))
+            elif name == "cx":
+                ops.append((qs, gates.CX))
+            elif name == "cz":
+                ops.append((qs, gates.CZ))
+            elif name == "m1":
+                ops.append((qs, fixed1))
+            else:
+                ops.append((qs, fixed2))
+        return ops
+
+    p = rng.uniform(0, 2 * np.pi, 15)
+    e, g = c.gradient(p)
+    assert abs(e - orc.energy(oracle_lib, n, make(p), terms)) < 1e-12
+    fd = orc.grad_central_difference(oracle_lib, n, make, terms, p, h=1e-6)
+    assert np.max(np.abs(g - fd)) < 1e-7
+    ps = c.parameter_shift(p)
+    assert np.max(np.abs(ps - orc.grad_parameter_shift(oracle_lib, n, make, terms, p))) < 1e-11
+
+
+def test_shared_parameters_accumulate(A):
+    # the same parameter on two gates: gradient is the sum of both contributions
+    c = A.GpuCircuit(2, "complex128")
+    c.add("ry", [0], [0])
+    c.add("ry", [1], [0])
+    c.set_observable([(1.0, "ZZ", [0, 1])])
+    t = 0.8
+    e, g = c.gradient([t])
+    assert abs(e - math.cos(t) ** 2) < 1e-14
+    assert abs(g[0] + 2 * math.cos(t) * math.sin(t)) < 1e-14
+
+
+def test_torch_autograd_boundary(A):
+    import torch
+    n, layers = 6, 2
+    c = _hea(A, n, layers, "complex64")
+    rng = np.random.default_rng(12)
+    p0 = rng.uniform(0, 2 * np.pi, 2 * n * layers)
+    for device in ("cpu", "cuda"):
+        p = torch.tensor(p0, dtype=torch.float64, device=device, requires_grad=True)
+        e = A.expectation(c, p)
+        (2.0 * e).backward()
+        e_ref, g_ref = c.gradient(p0)
+        assert e.device.type == device
+        assert abs(e.item() - e_ref) < 1e-12
+        assert np.allclose(p.grad.cpu().numpy(), 2.0 * g_ref, atol=1e-12)
+
+
+def test_full_size_24q_adjoint_vs_fp64_parameter_shift(A):
+    """24 qubits, 12 layers (576 parameters, 864 gates): complex64 adjoint against
+    the complex128 parameter-shift on 24 parameters spread over the layers."""
+    n, layers = 24, 12
+    rng = np.random.default_rng(2026)
+    p = rng.uniform(0, 2 * np.pi, 2 * n * layers)
+    fast = _hea(A, n, layers, "complex64")
+    exact = _hea(A, n, layers, "complex128")
+    assert fast.num_gates == 864
+    e64, g64 = fast.gradient(p)
+    e128, g128 = exact.gradient(p)
+    which = list(range(0, 576, 24))
+    ps = exact.parameter_shift(p, which)
+    assert np.max(np.abs(g128[which] - ps)) < 1e-10
+    assert abs(e64 - e128) < 1e-4
+    assert np.linalg.norm(g64 - g128) / np.linalg.norm(g128) < 1e-4
