@@ -49,7 +49,7 @@ def time_fused(binding, torch, n: int, precision: str, steps: int, warmup: int, 
     stream = torch.cuda.ExternalStream(sv.stream)
     mats = [np.asarray(op.matrix) for op in ops]
     qubits = [list(op.qubits) for op in ops]
-    lowest = [min(q) for q in qubits]
+    lowest = [max(q) for q in qubits]  # highest target: < 10 for the qubits-0..9 blocks
     for _ in range(warmup):
         for q, m in zip(qubits, mats):
             sv.apply_fused(q, m)
@@ -67,8 +67,24 @@ def time_fused(binding, torch, n: int, precision: str, steps: int, warmup: int, 
             per_kernel[i].append(a.elapsed_time(b) * 1e-3)
     wall = (time.perf_counter() - t0) / steps
     norm = sv.norm_squared()
+    # Calibration for HBM-counter runs: one in-place scale pass of known size
+    # (reads and writes 2^n amplitudes once), launched after the timed region.
+    sv.normalize()
+    sv.synchronize()
     del sv
     return wall, per_kernel, lowest, norm
+
+
+def measured_traffic(n: int):
+    """HBM bytes per fused-gate launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/fused_gate_30q_traffic.json), or None."""
+    p = ROOT / "profiles" / "fused_gate_30q_traffic.json"
+    if n != 30 or not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_fused_baseline(n: int, blocks_to_time: int):
@@ -155,7 +171,10 @@ def main():
     ap.add_argument("--no-c64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--shift-params", type=int, default=24)
+    ap.add_argument("--only-fused", action="store_true", help="skip the c64, adjoint and CPU legs")
     a = ap.parse_args()
+    if a.only_fused:
+        a.no_c64 = a.no_adjoint = a.no_cpu = True
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU: the statevector is not sharded across GPUs, "
                                    "so only --gpus 1 is measured", "n_gpus": a.gpus}))
@@ -175,8 +194,8 @@ def main():
     flops_block = 8 * 32 * N
     value = bytes_block * a.blocks / wall / 1e12
     mean_k = np.array([np.mean(x) for x in per_kernel])
-    tiled = np.array([lo < 3 for lo in lowest])
-    kinds = {"tiled (lowest target < 3)": tiled, "direct (lowest target >= 3)": ~tiled}
+    low = np.array([hi < 10 for hi in lowest])
+    kinds = {"k_gate_mfma, blocks on qubits 0-9": low, "k_gate_mfma, unrestricted blocks": ~low}
     kstats = {}
     for name, sel in kinds.items():
         if sel.any():
@@ -235,8 +254,9 @@ def main():
         "fp64_TFLOPs": round(flops_block * a.blocks / wall / 1e12, 3),
         "gate_applications_per_s": round(a.blocks / wall, 2),
         "roofline": {"bound": "hbm", "achieved": round(bytes_block / dom_d / 1e9, 1), "peak": HBM_PEAK_TBPS * 1000,
-                     "unit": "GB/s", "frac": round(bytes_block / dom_d / 1e12 / HBM_PEAK_TBPS, 4), "traffic": None,
-                     "kernel": dom},
+                     "unit": "GB/s", "frac": round(bytes_block / dom_d / 1e12 / HBM_PEAK_TBPS, 4),
+                     "traffic": measured_traffic(n), "kernel": dom,
+                     "algorithmic_bytes_per_launch": bytes_block},
         "kernels": kstats,
         "complex64": c64,
         "adjoint": adj,

@@ -103,8 +103,11 @@ int         iq_last_error(void);
 const char* iq_last_error_message(void);
 void        iq_clear_error(void);
 const char* iq_backend_name(void);
-/* Gate kernel selection: 0 automatic, 1 direct (group per lane), 2 LDS-tiled. */
+/* Gate kernel selection: 0 automatic, 1 direct (group per lane), 2 LDS-tiled, 3 matrix-core (5-qubit blocks). */
 void        iq_set_gate_path(int path);
+/* Matrix-core kernel options: groups-of-16 batches per iteration (1, 2, 4; 0 = default)
+ * and matrix fragments in LDS (1) or VGPRs (0; -1 = default). */
+void        iq_set_mfma_options(int batches_per_iteration, int matrix_in_lds);
 
 void*  iq_statevector_create_precision(size_t num_qubits, int precision);
 int    iq_statevector_get_precision(void* sv);

@@ -88,7 +88,7 @@ bool Engine::init(int nqubits, int precision) {
     if (!hip_ok(hipMalloc(&red_part, red), "alloc") || !hip_ok(hipMalloc(&red_out, 128 * sizeof(double)), "alloc") ||
         !hip_ok(hipHostMalloc((void**)&host_out, 128 * sizeof(double)), "pinned alloc"))
         return false;
-    slot_bytes = gate_matrix_bytes(kC128, kMaxK, true);
+    slot_bytes = std::max(gate_matrix_bytes(kC128, kMaxK, true), mfma_fragment_bytes(kC128));
     slots.resize(kSlots);
     for (auto& s : slots) {
         if (!hip_ok(hipHostMalloc(&s.host, slot_bytes), "pinned alloc") || !hip_ok(hipMalloc(&s.dev, slot_bytes), "alloc") ||
@@ -118,12 +118,18 @@ bool Engine::reset_basis(uint64_t index) {
     return hip_ok(launch_init_basis(amp, n, prec, index, stream), "init");
 }
 
-const void* Engine::upload_matrix(int K, bool m3, const double* re, const double* im) {
+const void* Engine::upload_matrix(int K, bool m3, const double* re, const double* im, const uint32_t* mfma_q) {
     Slot& s = slots[next_slot];
     next_slot = (next_slot + 1) % kSlots;
     if (s.pending) hipEventSynchronize(s.ev);
-    pack_gate_matrix(prec, K, m3, re, im, s.host);
-    const size_t bytes = gate_matrix_bytes(prec, K, m3);
+    size_t bytes;
+    if (mfma_q) {
+        pack_mfma_fragments(prec, n, mfma_q, re, im, s.host);
+        bytes = mfma_fragment_bytes(prec);
+    } else {
+        pack_gate_matrix(prec, K, m3, re, im, s.host);
+        bytes = gate_matrix_bytes(prec, K, m3);
+    }
     if (!hip_ok(hipMemcpyAsync(s.dev, s.host, bytes, hipMemcpyHostToDevice, stream), "matrix upload")) return nullptr;
     hip_ok(hipEventRecord(s.ev, stream), "event record");
     s.pending = true;
@@ -169,9 +175,14 @@ bool Engine::apply(int K, const uint32_t* q, const double* re, const double* im,
     g.re = re;
     g.im = im;
     g.path = path;
-    const int chosen = choose_path(n, prec, g);
-    if (K >= 4 || chosen == kPathTiled) {
-        g.dmat = upload_matrix(K, use_three_mult(K), re, im);
+    int chosen = choose_path(n, prec, g);
+    if (chosen == kPathMfma && (K != 5 || n < 9)) chosen = kPathDirect;
+    g.path = chosen;
+    if (chosen == kPathMfma) {
+        g.dmat = upload_matrix(K, false, re, im, g.q);
+        if (!g.dmat) return false;
+    } else if (K >= 4 || chosen == kPathTiled) {
+        g.dmat = upload_matrix(K, use_three_mult(K), re, im, nullptr);
         if (!g.dmat) return false;
     }
     return hip_ok(launch_gate(amp, n, prec, stream, g), "gate launch");
@@ -421,7 +432,10 @@ void iq_clear_error(void) {
     t_msg[0] = '\0';
 }
 const char* iq_backend_name(void) { return "hip-gfx950"; }
-void iq_set_gate_path(int path) { g_gate_path = (path >= 0 && path <= 2) ? path : 0; }
+void iq_set_gate_path(int path) { g_gate_path = (path >= 0 && path <= 3) ? path : 0; }
+void iq_set_mfma_options(int batches_per_iteration, int matrix_in_lds) {
+    set_mfma_options(batches_per_iteration, matrix_in_lds);
+}
 
 // ------------------------------------------------------------- statevector
 

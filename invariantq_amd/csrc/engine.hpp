@@ -70,7 +70,7 @@ struct Engine {
 
     bool init(int nqubits, int precision);
     bool reset_basis(uint64_t index);
-    const void* upload_matrix(int K, bool m3, const double* re, const double* im);
+    const void* upload_matrix(int K, bool m3, const double* re, const double* im, const uint32_t* mfma_q);
     bool apply(int K, const uint32_t* q, const double* re, const double* im, int path);
     bool sync();
     bool ensure_work(size_t doubles);

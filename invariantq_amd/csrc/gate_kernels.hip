@@ -179,6 +179,102 @@ __global__ void __launch_bounds__(64) k_gate_tiled(cv<R>* __restrict__ psi, Tile
     for (int m = 0; m < D; ++m) psi[base + a.off[m]] = tile[pl ^ a.pm[m]];
 }
 
+template <typename R>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// 5-qubit block on the matrix cores, 3-multiply form: P1 = A Xr, P2 = B Xi,
+// P3 = (A+B)(Xr+Xi) as 32x32 by 32x16 real products (2 row tiles x 8 k-steps
+// of 16x16x4 each, 48 MFMAs per 16 groups); Re y = P1 - P2, Im y = P3 - P1 - P2.
+// The matrix fragments (48 per lane) stay in VGPRs for the whole kernel, so
+// the VALU only does addressing and the final combination, and memory traffic
+// of one batch overlaps the matrix-core work of another wave.
+template <typename R, int UB, bool WLDS>
+__global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, MfmaArgs a, const R* __restrict__ frag) {
+    using M = Mfma<R>;
+    using v4 = typename M::v4;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t q = lane >> 4, nn = lane & 15u;
+    // Matrix fragments: 48 per lane, in VGPRs or shared by the block's waves in LDS.
+    __shared__ R wsh[WLDS ? 48 * 64 : 1];
+    R wreg[WLDS ? 1 : 48];
+    if constexpr (WLDS) {
+        for (uint32_t i = threadIdx.x; i < 48u * 64u; i += blockDim.x) wsh[i] = frag[i];
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (int f = 0; f < 48; ++f) wreg[f] = frag[f * 64 + lane];
+    }
+    uint32_t wl = lane;
+    auto W = [&](int f) -> R {
+        if constexpr (WLDS) return wsh[f * 64 + wl];
+        else return wreg[f];
+    };
+    const uint64_t xq = a.xq[q], yq = a.yq[q];
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t nsuper = a.nbatches / UB;
+    for (uint64_t sb = wave; sb < nsuper; sb += nwaves) {
+        // An opaque copy of the lane index per iteration keeps the LDS fragment
+        // reads inside the loop (otherwise they are hoisted into 96 VGPRs).
+        if constexpr (WLDS) asm volatile("" : "+v"(wl));
+        uint64_t base[UB];
+        R xr[UB][8], xi[UB][8];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            base[u] = insert_zero_bits<5>((sb * UB + u) * 16 + nn, a.pos);
+#pragma unroll
+            for (int kb = 0; kb < 8; ++kb) {
+                const cv<R> v = psi[base[u] + xq + a.xk[kb]];
+                xr[u][kb] = v.x;
+                xi[u][kb] = v.y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            v4 p1[2], p2[2], p3[2];
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                p1[mt] = v4{0, 0, 0, 0};
+                p2[mt] = v4{0, 0, 0, 0};
+                p3[mt] = v4{0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int kb = 0; kb < 8; ++kb) {
+                const R xs = xr[u][kb] + xi[u][kb];
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) {
+                    p1[mt] = M::op(W((0 * 2 + mt) * 8 + kb), xr[u][kb], p1[mt]);
+                    p2[mt] = M::op(W((1 * 2 + mt) * 8 + kb), xi[u][kb], p2[mt]);
+                    p3[mt] = M::op(W((2 * 2 + mt) * 8 + kb), xs, p3[mt]);
+                }
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    cv<R> o;
+                    o.x = p1[mt][jj] - p2[mt][jj];
+                    o.y = p3[mt][jj] - p1[mt][jj] - p2[mt][jj];
+                    psi[base[u] + yq + a.yk[mt * 4 + jj]] = o;
+                }
+        }
+    }
+}
+
 template <typename R, int K>
 __global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t count, DirectArgs<K> a,
                                               ValMat<double, 2 * (1 << K)> d) {
@@ -236,9 +332,45 @@ hipError_t run_tiled(void* psi, int n, hipStream_t st, const GateDesc& g) {
     return hipGetLastError();
 }
 
+static int g_mfma_batches = 0;   // 0 = per-precision default
+static int g_mfma_lds = -1;      // -1 = per-precision default
+
+template <typename R, int UB, bool WLDS>
+hipError_t launch_mfma(void* psi, const MfmaArgs& a, hipStream_t st, const void* dmat) {
+    const uint64_t nsuper = a.nbatches / UB;
+    const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nsuper, 256ULL * 16ULL));
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    hipLaunchKernelGGL((k_gate_mfma<R, UB, WLDS>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, a, (const R*)dmat);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t run_mfma(void* psi, int n, hipStream_t st, const GateDesc& g) {
+    MfmaArgs a;
+    uint32_t s_in[32], s_out[32];
+    fill_mfma_args(a, n, g.q, std::is_same<R, double>::value, s_in, s_out);
+    if (a.nbatches == 0 || !g.dmat) return hipErrorInvalidValue;
+    constexpr bool f64 = std::is_same<R, double>::value;
+    // defaults from the 30-qubit sweep: four batches in flight per wave (1 KB
+    // runs per stream), fragments in VGPRs, for both precisions
+    int ub = g_mfma_batches ? g_mfma_batches : 4;
+    const bool lds = g_mfma_lds >= 0 ? g_mfma_lds != 0 : false;
+    (void)f64;
+    while (ub > 1 && a.nbatches % ub) ub >>= 1;
+    if (lds) {
+        if (ub >= 4) return launch_mfma<R, 4, true>(psi, a, st, g.dmat);
+        if (ub == 2) return launch_mfma<R, 2, true>(psi, a, st, g.dmat);
+        return launch_mfma<R, 1, true>(psi, a, st, g.dmat);
+    }
+    if (ub >= 4) return launch_mfma<R, 4, false>(psi, a, st, g.dmat);
+    if (ub == 2) return launch_mfma<R, 2, false>(psi, a, st, g.dmat);
+    return launch_mfma<R, 1, false>(psi, a, st, g.dmat);
+}
+
 template <typename R>
 hipError_t dispatch(void* psi, int n, hipStream_t st, const GateDesc& g, int path) {
     const bool m3 = use_three_mult(g.K);
+    if (path == kPathMfma && g.K == 5) return run_mfma<R>(psi, n, st, g);
     if (path == kPathTiled) {
         switch (g.K) {
             case 3: return m3 ? run_tiled<R, 3, true>(psi, n, st, g) : run_tiled<R, 3, false>(psi, n, st, g);
@@ -277,6 +409,13 @@ hipError_t run_diag(void* psi, int n, hipStream_t st, const uint32_t* q, const d
 
 bool use_three_mult(int K) { return K >= 4; }
 
+// Matrix-core kernel options chosen by the autotuner (or left at the defaults).
+void set_mfma_options(int batches_per_iteration, int matrix_in_lds) {
+    g_mfma_batches = (batches_per_iteration == 1 || batches_per_iteration == 2 || batches_per_iteration == 4)
+                         ? batches_per_iteration : 0;
+    g_mfma_lds = (matrix_in_lds == 0 || matrix_in_lds == 1) ? matrix_in_lds : -1;
+}
+
 size_t gate_matrix_bytes(int prec, int K, bool three_mult) {
     const size_t d = 1ULL << K;
     return (three_mult ? 3 : 2) * d * d * (prec == kC128 ? sizeof(double) : sizeof(float));
@@ -298,8 +437,31 @@ void pack_gate_matrix(int prec, int K, bool three_mult, const double* re, const 
         }
 }
 
+size_t mfma_fragment_bytes(int prec) { return 48 * 64 * (prec == kC128 ? sizeof(double) : sizeof(float)); }
+
+// Matrix-core fragment order for k_gate_mfma: fragment f = (part * 2 + mt) * 8 + kb
+// (part 0 = Re U, 1 = Im U, 2 = Re U + Im U) holds, in lane l, the entry
+// U_part[s_out(16 mt + l % 16)][s_in(4 kb + l / 16)].
+void pack_mfma_fragments(int prec, int n, const uint32_t* q, const double* re, const double* im, void* out) {
+    MfmaArgs a;
+    uint32_t s_in[32], s_out[32];
+    fill_mfma_args(a, n, q, prec == kC128, s_in, s_out);
+    for (int part = 0; part < 3; ++part)
+        for (int mt = 0; mt < 2; ++mt)
+            for (int kb = 0; kb < 8; ++kb)
+                for (int l = 0; l < 64; ++l) {
+                    const uint32_t r = s_out[16 * mt + (l & 15)], c = s_in[4 * kb + (l >> 4)];
+                    const double x = re[r * 32 + c], y = im[r * 32 + c];
+                    const double v = part == 0 ? x : (part == 1 ? y : x + y);
+                    const size_t at = (size_t)((part * 2 + mt) * 8 + kb) * 64 + l;
+                    if (prec == kC128) static_cast<double*>(out)[at] = v;
+                    else static_cast<float*>(out)[at] = (float)v;
+                }
+}
+
 int choose_path(int n, int prec, const GateDesc& g) {
     if (g.path != kPathAuto) return g.path;
+    if (g.K == 5 && n >= 9) return kPathMfma;
     if (g.K <= 2 || n < g.K + 6) return kPathDirect;
     uint32_t lowest = 64;
     for (int m = 0; m < g.K; ++m) lowest = std::min(lowest, g.q[m]);
@@ -310,6 +472,7 @@ int choose_path(int n, int prec, const GateDesc& g) {
 hipError_t launch_gate(void* psi, int n, int prec, hipStream_t st, const GateDesc& g) {
     if (g.K < 1 || g.K > kMaxK || g.K > n) return hipErrorInvalidValue;
     int path = choose_path(n, prec, g);
+    if (path == kPathMfma && (g.K != 5 || n < 9)) path = kPathDirect;
     if (path == kPathTiled && (g.K < 3 || n < g.K + 6)) path = kPathDirect;
     if (g.K >= 4 && !g.dmat) return hipErrorInvalidValue;
     if (path == kPathTiled && !g.dmat) path = kPathDirect;

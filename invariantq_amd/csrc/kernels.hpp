@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "layout.hpp"
 
 namespace iqg {
 
@@ -12,15 +13,21 @@ namespace iqg {
 size_t gate_matrix_bytes(int prec, int K, bool three_mult);
 void pack_gate_matrix(int prec, int K, bool three_mult, const double* re, const double* im, void* out);
 bool use_three_mult(int K);
+// Matrix-core kernel options: batches of 16 groups per loop iteration (1, 2, 4;
+// 0 = default) and whether the matrix fragments live in LDS (1) or VGPRs (0; -1 = default).
+void set_mfma_options(int batches_per_iteration, int matrix_in_lds);
+// Fragment-ordered copy of a 32x32 block for the matrix-core kernel (depends on the targets).
+size_t mfma_fragment_bytes(int prec);
+void pack_mfma_fragments(int prec, int n, const uint32_t* q, const double* re, const double* im, void* out);
 
-enum GatePath : int { kPathAuto = 0, kPathDirect = 1, kPathTiled = 2 };
+enum GatePath : int { kPathAuto = 0, kPathDirect = 1, kPathTiled = 2, kPathMfma = 3 };
 
 struct GateDesc {
     int K = 0;                 // number of target qubits, 1..5
     uint32_t q[kMaxK] = {};    // target list, MSB-first: q[0] is the most significant local bit
     const double* re = nullptr;  // host row-major D x D (kept for by-value launches, K <= 3)
     const double* im = nullptr;
-    const void* dmat = nullptr;  // device copy in pack_gate_matrix layout (required for K >= 4)
+    const void* dmat = nullptr;  // device copy: pack_gate_matrix layout, or pack_mfma_fragments for the MFMA path
     int path = kPathAuto;
 };
 

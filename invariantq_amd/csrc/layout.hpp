@@ -98,6 +98,65 @@ inline void fill_direct_args(DirectArgs<K>& a, const uint32_t* q) {
     }
 }
 
+// MFMA kernel for 5-qubit blocks (16x16x4 matrix-core tiles).  One wave holds
+// 16 groups (lane & 15 = group within the batch); the four lanes of a group
+// (lane >> 4 = q) hold 8 of its 32 amplitudes each.  Column kappa = 4*kb + q of
+// the matrix tiles reads the amplitude at offset xq[q] + xk[kb]: q spans the two
+// lowest target qubits, kb the other three, so global accesses stay in runs.
+// Output value jj of lane (q, n) in row tile mt is the amplitude at offset
+// yq[q] + yk[4*mt + jj] (q -> two lowest targets, jj -> next two, mt -> top).
+// Its tile row is 16*mt + 4*q + jj for the f32 16x16x4 instruction and
+// 16*mt + q + 4*jj for the f64 one (measured lane layouts of the two).
+struct MfmaArgs {
+    uint32_t pos[5];      // targets ascending (zero-bit insertion)
+    uint64_t xq[4], xk[8];
+    uint64_t yq[4], yk[8];
+    uint64_t nbatches;    // 2^(n-5) / 16
+};
+
+// Local (MSB-first) index of tile column kappa / tile row rho, used to permute
+// the matrix into MFMA fragment order on the host.
+IQ_HD uint32_t mfma_row(uint32_t q, uint32_t jj, uint32_t mt, bool f64) {
+    return 16u * mt + (f64 ? q + 4u * jj : 4u * q + jj);
+}
+
+inline void fill_mfma_args(MfmaArgs& a, int n, const uint32_t* q, bool f64, uint32_t* s_in, uint32_t* s_out) {
+    uint32_t t[5];
+    int lbit[5];  // local bit of target t[b]
+    for (int m = 0; m < 5; ++m) t[m] = q[m];
+    std::sort(t, t + 5);
+    for (int b = 0; b < 5; ++b)
+        for (int m = 0; m < 5; ++m)
+            if (q[m] == t[b]) lbit[b] = 4 - m;
+    for (int m = 0; m < 5; ++m) a.pos[m] = t[m];
+    // kappa bit b <-> target t[b]
+    for (int qq = 0; qq < 4; ++qq) a.xq[qq] = ((qq & 1) ? 1ULL << t[0] : 0) | ((qq & 2) ? 1ULL << t[1] : 0);
+    for (int kb = 0; kb < 8; ++kb)
+        a.xk[kb] = ((kb & 1) ? 1ULL << t[2] : 0) | ((kb & 2) ? 1ULL << t[3] : 0) | ((kb & 4) ? 1ULL << t[4] : 0);
+    for (uint32_t kap = 0; kap < 32; ++kap) {
+        uint32_t s = 0;
+        for (int b = 0; b < 5; ++b)
+            if ((kap >> b) & 1u) s |= 1u << lbit[b];
+        s_in[kap] = s;
+    }
+    // f32 rows: bits 0,1 (jj) <-> t[2], t[3]; 2,3 (q) <-> t[0], t[1]; 4 (mt) <-> t[4]
+    // f64 rows: bits 0,1 (q) <-> t[0], t[1]; 2,3 (jj) <-> t[2], t[3]; 4 (mt) <-> t[4]
+    const int rb32[5] = {2, 3, 0, 1, 4}, rb64[5] = {0, 1, 2, 3, 4};
+    const int* rb_of_target = f64 ? rb64 : rb32;
+    for (int qq = 0; qq < 4; ++qq) a.yq[qq] = ((qq & 1) ? 1ULL << t[0] : 0) | ((qq & 2) ? 1ULL << t[1] : 0);
+    for (int r = 0; r < 8; ++r) {  // r = 4*mt + jj
+        const uint32_t jj = r & 3, mt = r >> 2;
+        a.yk[r] = ((jj & 1) ? 1ULL << t[2] : 0) | ((jj & 2) ? 1ULL << t[3] : 0) | (mt ? 1ULL << t[4] : 0);
+    }
+    for (uint32_t rho = 0; rho < 32; ++rho) {
+        uint32_t s = 0;
+        for (int b = 0; b < 5; ++b)
+            if ((rho >> rb_of_target[b]) & 1u) s |= 1u << lbit[b];
+        s_out[rho] = s;
+    }
+    a.nbatches = (n >= 9) ? (1ULL << (n - 5)) / 16 : 0;
+}
+
 // Requires n >= K + 6 and distinct targets < n.
 template <int K>
 inline void fill_tiled_args(TiledArgs<K>& a, int n, const uint32_t* q) {

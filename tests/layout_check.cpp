@@ -8,7 +8,10 @@
 //           the LDS positions of a tile are a permutation of [0, 64*2^K);
 //           the element a lane gathers for local index s is the amplitude whose
 //           target bits spell s and whose other bits are that lane's group;
-//           16 consecutive lanes hit 16 distinct 16-element slots in both passes.
+//           16 consecutive lanes hit 16 distinct 16-element slots in both passes;
+//   mfma:   each lane's loads and stores stay inside its group, read/write every
+//           amplitude exactly once, and land on the local indices the matrix
+//           fragments were permuted to.
 // Prints "ok" or the first violation.
 #include <cstdio>
 #include <cstdlib>
@@ -49,6 +52,38 @@ static int check(int n, const uint32_t* q) {
                 if (local_of(idx) != (uint32_t)s) return fail("direct local index", n, K);
             }
         }
+    }
+    for (int f64 = 0; f64 < 2 && K == 5 && n >= 9; ++f64) {  // matrix-core mapping, both row layouts
+        MfmaArgs ma;
+        uint32_t s_in[32], s_out[32];
+        fill_mfma_args(ma, n, q, f64 == 1, s_in, s_out);
+        std::set<uint32_t> pin(s_in, s_in + 32), pout(s_out, s_out + 32);
+        if (pin.size() != 32 || pout.size() != 32) return fail("mfma local maps not permutations", n, K);
+        std::vector<unsigned char> rd(N, 0), wr(N, 0);
+        for (uint64_t b = 0; b < ma.nbatches; ++b)
+            for (uint32_t lane = 0; lane < 64; ++lane) {
+                const uint32_t qq = lane >> 4, nn = lane & 15;
+                uint32_t pos5[5];
+                for (int m = 0; m < 5; ++m) pos5[m] = ma.pos[m];
+                const uint64_t base = insert_zero_bits<5>(b * 16 + nn, pos5);
+                for (int kb = 0; kb < 8; ++kb) {
+                    const uint64_t idx = base + ma.xq[qq] + ma.xk[kb];
+                    if (idx >= N) return fail("mfma load out of range", n, K);
+                    if (rd[idx]++) return fail("mfma load repeated", n, K);
+                    if (local_of(idx) != s_in[4 * kb + qq]) return fail("mfma load local index", n, K);
+                    if ((idx & ~tmask) != base) return fail("mfma load leaves its group", n, K);
+                }
+                for (int mt = 0; mt < 2; ++mt)
+                    for (int jj = 0; jj < 4; ++jj) {
+                        const uint64_t idx = base + ma.yq[qq] + ma.yk[4 * mt + jj];
+                        if (idx >= N) return fail("mfma store out of range", n, K);
+                        if (wr[idx]++) return fail("mfma store repeated", n, K);
+                        if (local_of(idx) != s_out[mfma_row(qq, jj, mt, f64 == 1)]) return fail("mfma store local index", n, K);
+                        if ((idx & ~tmask) != base) return fail("mfma store leaves its group", n, K);
+                    }
+            }
+        for (uint64_t i = 0; i < N; ++i)
+            if (rd[i] != 1 || wr[i] != 1) return fail("mfma pass missed an amplitude", n, K);
     }
     if (n < K + 6) return 0;
     TiledArgs<K> a;

@@ -108,7 +108,7 @@ TARGETS = [[0], [13], [5], [0, 1], [1, 0], [13, 2], [7, 12], [0, 1, 2], [11, 3, 
            [13, 0, 7, 2, 9], [9, 10, 11, 12, 13], [5, 6, 7, 8, 9], [2, 13, 6, 0, 11]]
 
 
-@pytest.mark.parametrize("path", ["auto", "direct", "tiled"])
+@pytest.mark.parametrize("path", ["auto", "direct", "tiled", "mfma"])
 @pytest.mark.parametrize("precision", ["complex128", "complex64"])
 def test_gates_match_oracle(B, oracle_lib, path, precision):
     n = 14
@@ -192,14 +192,20 @@ def test_random_u3_cx_20q_complex64(B, oracle_lib):
     assert abs(np.vdot(b, a)) ** 2 > 1 - 1e-6
 
 
+@pytest.mark.parametrize("path", ["auto", "direct", "tiled", "mfma"])
 @pytest.mark.parametrize("precision", ["complex128", "complex64"])
-def test_fused_blocks_18q(B, oracle_lib, precision):
+def test_fused_blocks_18q(B, oracle_lib, precision, path):
     """Scaled-down fused-block workload: 40 Haar 32x32 blocks (half on qubits 0-9)."""
     n = 18
     init = workloads.u3_layer(n, seed=3)
     blocks = workloads.fused_blocks(n, 40, seed=4)
     sv, ref = B.NativeStateVector(n, precision=precision), oracle_lib.statevector(n)
-    _run(sv, init + blocks)
+    _run(sv, init)
+    B.set_gate_path(path)
+    try:
+        _run(sv, blocks)
+    finally:
+        B.set_gate_path("auto")
     _run(ref, init + blocks)
     a, b = sv.get_amplitudes(), ref.get_amplitudes()
     tol = TOL128 if precision == "complex128" else TOL64
