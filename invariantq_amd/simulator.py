@@ -1,0 +1,223 @@
+"""Simulator callers of the engine.
+
+Restates the op loop, validation and output formatting of
+invariantq/simulator/statevector.py:21-144 (StateVectorSimulator.run,
+NoisyStatevectorSimulator) and invariantq/simulator/density_matrix.py:22-112
+(DensityMatrixSimulator.run) over a plain op list, so the GPU engine can be
+driven the way the reference drives its native core:
+
+* unitarity check allclose(U^dag U, I, 1e-10) unless non-unitary gates are allowed;
+* dispatch by arity (1 -> apply_1q, 2 -> apply_2q, else apply_fused);
+* counts keyed by MSB-first bit strings f"{index:0{n}b}"; probabilities > 1e-7 kept;
+* samples from the engine's sampler (mt19937_64 per seed), or NumPy's
+  default_rng(seed).choice for non-unitary runs and density matrices, as the reference does.
+
+Op forms accepted: ``workloads.Op`` objects, or tuples
+``("gate", name, qubits, matrix)``, ``("reset", qubit)``,
+``("noise", channel, qubits, params)``.  A gate's matrix may be None when its
+name is a known gate (``gates.gate_matrix``).
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Any, Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import gates
+from .core.binding import HAS_NATIVE_CORE, NativeDensityMatrix, NativeStateVector, native_core_status
+from .workloads import Op
+
+
+@dataclass
+class ExecutionResult:
+    """Field names of invariantq/simulator/base.py:11-22."""
+    counts: Dict[str, int] = field(default_factory=dict)
+    probabilities: Dict[str, float] = field(default_factory=dict)
+    statevector: Optional[np.ndarray] = None
+    density_matrix: Optional[np.ndarray] = None
+    entanglement_entropy: Optional[float] = None
+    execution_time_ms: float = 0.0
+    backend_used: str = ""
+    memory_used_bytes: int = 0
+    shots: int = 1024
+    metadata: Dict[str, Any] = field(default_factory=dict)
+
+
+def checked_power_of_two(qubits: int, bytes_per_amplitude: int, *, copies: int = 1) -> int:
+    """invariantq/simulator/base.py:44-53"""
+    if isinstance(qubits, bool) or not isinstance(qubits, int) or qubits < 0:
+        raise ValueError("qubits must be a non-negative integer")
+    if copies <= 0 or bytes_per_amplitude <= 0:
+        raise ValueError("allocation estimate parameters must be positive")
+    if qubits >= 62:
+        raise MemoryError("requested dense allocation exceeds supported integer range")
+    return (1 << qubits) * bytes_per_amplitude * copies
+
+
+def _normalise(op):
+    if isinstance(op, Op):
+        if op.kind == "kraus":
+            return ("kraus", op.name, list(op.qubits), op.kraus)
+        m = op.matrix if op.matrix is not None else gates.gate_matrix(op.name, op.params)
+        return ("gate", op.name, list(op.qubits), m)
+    kind = op[0]
+    if kind == "gate":
+        _, name, qubits, matrix = op
+        if matrix is None:
+            matrix = gates.gate_matrix(name)
+        return ("gate", name, list(qubits), matrix)
+    if kind == "reset":
+        return ("reset", None, [op[1]], None)
+    if kind == "noise":
+        _, channel, qubits, params = op
+        return ("kraus", channel, list(qubits), gates.build_kraus_channel(channel, params))
+    raise ValueError(f"unknown op {op!r}")
+
+
+def _counts(samples, n):
+    counts: Dict[str, int] = {}
+    for s in samples:
+        key = f"{int(s):0{n}b}"
+        counts[key] = counts.get(key, 0) + 1
+    return counts
+
+
+def _probabilities(probs, n):
+    return {f"{i:0{n}b}": float(p) for i, p in enumerate(probs) if p > 1e-7}
+
+
+class StateVectorSimulator:
+    """GPU counterpart of invariantq/simulator/statevector.py:16-144."""
+
+    def __init__(self, max_memory_bytes: Optional[int] = None, precision: str = "complex128"):
+        self.name = "statevector"
+        self.max_memory_bytes = max_memory_bytes
+        self.precision = precision
+
+    def run(self, num_qubits: int, ops: Sequence, shots: int = 1024, seed: int = 42, *,
+            allow_non_unitary: bool = False, entropy: bool = True) -> ExecutionResult:
+        t0 = time.perf_counter()
+        if isinstance(shots, bool) or not isinstance(shots, (int, np.integer)) or shots <= 0:
+            raise ValueError("shots must be a positive integer")
+        if isinstance(seed, bool) or not isinstance(seed, (int, np.integer)) or seed < 0:
+            raise ValueError("seed must be a non-negative integer")
+        n = num_qubits
+        if n == 0:
+            raise ValueError("Circuit has 0 qubits")
+        mem_bytes = checked_power_of_two(n, 16, copies=2)
+        if self.max_memory_bytes is not None and mem_bytes > self.max_memory_bytes:
+            raise MemoryError(f"statevector requires approximately {mem_bytes} bytes including a working copy, "
+                              f"above configured limit {self.max_memory_bytes}")
+        sv = NativeStateVector(n, precision=self.precision)
+        for raw in ops:
+            kind, name, qubits, matrix = _normalise(raw)
+            if kind == "reset":
+                sv.reset_qubit(qubits[0])
+                continue
+            if kind != "gate":
+                raise ValueError("noise channels need the density-matrix simulator")
+            d = 1 << len(qubits)
+            m = np.asarray(matrix, dtype=np.complex128)
+            ok = (m.shape == (d, d) and np.isfinite(m).all()
+                  and np.allclose(m.conj().T @ m, np.eye(d), atol=1e-10, rtol=1e-10))
+            if not ok and not allow_non_unitary:
+                raise ValueError(f"Gate '{name}' is not a finite unitary matrix")
+            if len(qubits) == 1:
+                sv.apply_1q(qubits[0], m)
+            elif len(qubits) == 2:
+                sv.apply_2q(qubits[0], qubits[1], m)
+            else:
+                sv.apply_fused(qubits, m)
+        amps = sv.get_amplitudes()
+        probs_vec = sv.get_probabilities()
+        ent = 0.0
+        if entropy and n >= 2:
+            ent = sv.von_neumann_entropy(list(range(n // 2))) if n // 2 <= 12 else None
+        if allow_non_unitary:
+            total = float(np.sum(probs_vec))
+            if not np.isfinite(total) or total <= 0.0:
+                raise ValueError("non-unitary trajectory produced an invalid state norm")
+            samples = np.random.default_rng(seed).choice(1 << n, size=shots, p=probs_vec / total)
+        else:
+            samples = sv.sample(shots, seed)
+        return ExecutionResult(
+            counts=_counts(samples, n), probabilities=_probabilities(probs_vec, n), statevector=amps,
+            entanglement_entropy=ent, execution_time_ms=(time.perf_counter() - t0) * 1e3,
+            backend_used="statevector", memory_used_bytes=mem_bytes, shots=shots,
+            metadata={"num_qubits": n, "dimension": 1 << n, "allow_non_unitary": allow_non_unitary,
+                      "state_norm": float(np.sum(probs_vec)), "core": native_core_status(),
+                      "precision": self.precision,
+                      "provenance": {"requested_backend": self.name, "selected_backend": "statevector",
+                                     "implementation": "dense_statevector", "native": HAS_NATIVE_CORE,
+                                     "exact": not allow_non_unitary,
+                                     "resource_limit_bytes": self.max_memory_bytes, "completion": "completed"}})
+
+
+class NoisyStatevectorSimulator(StateVectorSimulator):
+    """invariantq/simulator/statevector.py:147-163: non-unitary gates allowed."""
+
+    def run(self, num_qubits: int, ops: Sequence, shots: int = 1024, seed: int = 42, **kw) -> ExecutionResult:
+        kw["allow_non_unitary"] = True
+        return super().run(num_qubits, ops, shots=shots, seed=seed, **kw)
+
+
+class DensityMatrixSimulator:
+    """GPU counterpart of invariantq/simulator/density_matrix.py:13-112."""
+
+    def __init__(self, noise_model: Optional[Dict[str, List[np.ndarray]]] = None,
+                 max_memory_bytes: Optional[int] = None):
+        self.name = "density_matrix"
+        self.noise_model = noise_model
+        self.max_memory_bytes = max_memory_bytes
+
+    def run(self, num_qubits: int, ops: Sequence, shots: int = 1024, seed: int = 42) -> ExecutionResult:
+        t0 = time.perf_counter()
+        n = num_qubits
+        mem_bytes = checked_power_of_two(n, 16, copies=1 << n)
+        if n > 14 or (self.max_memory_bytes is not None and mem_bytes > self.max_memory_bytes):
+            raise MemoryError(f"Density matrix requires approximately {mem_bytes} bytes. "
+                              f"For n > 14, please use StateVector or TensorNetwork (MPS) backend.")
+        dm = NativeDensityMatrix(n)
+        dim = 1 << n
+        for raw in ops:
+            kind, name, qubits, matrix = _normalise(raw)
+            if kind == "gate":
+                noise = self.noise_model.get(name) if self.noise_model else None
+                if len(qubits) == 1:
+                    dm.apply_1q_unitary(qubits[0], matrix)
+                    if noise:
+                        dm.apply_1q_kraus(qubits[0], noise)
+                elif len(qubits) == 2:
+                    dm.apply_2q_unitary(qubits[0], qubits[1], matrix)
+                    if noise:
+                        if np.asarray(noise[0]).shape == (4, 4):
+                            dm.apply_2q_kraus(qubits[0], qubits[1], noise)
+                        else:
+                            dm.apply_1q_kraus(qubits[0], noise)
+                            dm.apply_1q_kraus(qubits[1], noise)
+                else:
+                    raise ValueError("density-matrix gates act on one or two qubits")
+            elif kind == "reset":
+                dm.reset_qubit(qubits[0])
+            else:
+                for q in qubits:
+                    dm.apply_1q_kraus(q, matrix)
+        mat = dm.get_matrix()
+        trace, purity, ent = dm.trace(), dm.purity(), dm.von_neumann_entropy()
+        diag = np.maximum(0.0, np.real(np.diag(mat)))
+        s = np.sum(diag)
+        if s > 0:
+            diag = diag / s
+        samples = np.random.default_rng(seed).choice(dim, size=shots, p=diag)
+        return ExecutionResult(
+            counts=_counts(samples, n), probabilities=_probabilities(diag, n), density_matrix=mat,
+            entanglement_entropy=ent, execution_time_ms=(time.perf_counter() - t0) * 1e3,
+            backend_used="density_matrix", memory_used_bytes=mem_bytes, shots=shots,
+            metadata={"num_qubits": n, "dimension": dim, "trace": trace, "purity": purity,
+                      "core": native_core_status(),
+                      "provenance": {"requested_backend": self.name, "selected_backend": "density_matrix",
+                                     "implementation": "density_matrix", "native": HAS_NATIVE_CORE, "exact": True,
+                                     "resource_limit_bytes": self.max_memory_bytes, "completion": "completed"}})
