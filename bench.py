@@ -280,6 +280,59 @@ def density_matrix_workload(binding, torch, cpu: bool):
     return out
 
 
+def batch_workload(torch, count: int, cpu: bool):
+    """4096 ansatz circuits (n ~ U{10..20}, layers ~ U{4..16}, complex64, FP64
+    accumulation): energies and full adjoint gradients in one persistent launch,
+    tile widths and workgroup size from the autotuner."""
+    from invariantq_amd import adjoint, batch, workloads
+    specs = workloads.batch_specs(count, seed=3)
+    runner = batch.BatchRunner("complex64")
+    t0 = time.perf_counter()
+    tune = batch.autotune(runner, specs, cache=None, per_n=128)
+    t_tune = time.perf_counter() - t0
+    runner.run(specs[:64], tile_bits=tune["tile_bits"], block_threads=tune["block_threads"])  # warm-up
+    t0 = time.perf_counter()
+    e, g, ms = runner.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"])
+    wall = time.perf_counter() - t0
+    nparams = int(sum(2 * n * l for n, l, _ in specs))
+    gate_pass_bytes = sum(3 * n * l * (2 + 4) * (1 << n) * 8 for n, l, _ in specs)
+    rng = np.random.default_rng(9)
+    check = sorted(rng.choice(count, 12, replace=False).tolist(), key=lambda i: specs[i][0])
+    errs = []
+    for i in check:
+        n, l, p = specs[i]
+        c = adjoint.GpuCircuit.from_ansatz(workloads.Ansatz(n, l), "complex128")
+        c.set_observable(workloads.tfim_terms(n))
+        e_ref, g_ref = c.gradient(p)
+        errs.append(float(np.linalg.norm(g[i] - g_ref) / np.linalg.norm(g_ref)))
+    out = {"workload": f"{count} ansatz circuits, n ~ U{{10..20}}, layers ~ U{{4..16}}, complex64 state, "
+                       "FP64 accumulation, TFIM energy + full adjoint gradient each, one persistent launch",
+           "circuits": count, "parameters": nparams,
+           "kernel_ms": round(ms, 3), "wall_ms": round(wall * 1e3, 3),
+           "circuits_per_s": round(count / (ms * 1e-3), 1),
+           "gradients_per_s": round(nparams / (ms * 1e-3), 1),
+           "gate_pass_equivalent_TBps": round(gate_pass_bytes / (ms * 1e-3) / 1e12, 3),
+           "autotune": {"tile_bits": tune["tile_bits"], "block_threads": tune["block_threads"],
+                        "seconds": round(t_tune, 2)},
+           "max_rel_grad_err_vs_complex128": max(errs), "checked_circuits": [specs[i][0] for i in check]}
+    if cpu:
+        from oracle import oracle as orc
+        core = orc.oracle()
+        sample = [specs[i] for i in range(count) if specs[i][0] in (10, 12)][:4]
+        t0 = time.perf_counter()
+        cnt = 0
+        for n, l, p in sample:
+            orc.grad_parameter_shift(core, n, lambda x, n=n, l=l: orc.hea_ops(n, l, x), orc.tfim_terms(n), p,
+                                     which=list(range(8)))
+            cnt += 8
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cnt / t, 2), "unit": "gradients/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                               "kind": "port", "sample": f"oracle FP64 parameter shift, 8 parameters of each of "
+                                                          f"{len(sample)} circuits with n in (10, 12) ({t:.2f} s)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,7 +346,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--shift-params", type=int, default=24)
     ap.add_argument("--only-fused", action="store_true", help="skip the c64, adjoint and CPU legs")
-    ap.add_argument("--workload", default="fused", choices=["fused", "circuit20", "dm14"],
+    ap.add_argument("--workload", default="fused", choices=["fused", "circuit20", "dm14", "batch4096"],
                     help="fused: the headline line; circuit20 / dm14: the other workloads, printed as one JSON line")
     a = ap.parse_args()
     if a.only_fused:
@@ -310,6 +363,9 @@ def main():
     torch.cuda.init()
     if a.workload == "circuit20":
         print(json.dumps(circuit_workload(binding, torch, a.steps, a.warmup, not a.no_cpu)), flush=True)
+        return
+    if a.workload == "batch4096":
+        print(json.dumps(batch_workload(torch, 4096, not a.no_cpu)), flush=True)
         return
     if a.workload == "dm14":
         print(json.dumps(density_matrix_workload(binding, torch, not a.no_cpu)), flush=True)

@@ -170,6 +170,19 @@ int    iq_circuit_parameter_shift(void* c, const double* params, size_t nparams,
 void   iq_circuit_get_amplitudes(void* c, double* real_out, double* imag_out);
 void*  iq_circuit_stream(void* c);
 
+/* ------------------------------------------- batched sweep, one persistent launch
+ * No reference counterpart.  Circuits are the ansatz above (RY, RZ on every
+ * qubit, CZ ring, per layer) with H = sum Z_q Z_{q+1} + hx sum X_q (open chain). */
+void*  iq_batch_create(int precision);
+void   iq_batch_free(void* b);
+int    iq_batch_max_tile_bits(int precision);
+/* params: 2*nq[c]*layers[c] values per circuit, concatenated; grads_out has the same
+ * layout.  tile_bits[c] (0 = widest) and block_threads (0 = 512) come from the
+ * autotuner; num_blocks 0 = one workgroup per compute unit. */
+int    iq_batch_hea_tfim(void* b, size_t ncirc, const int32_t* nq, const int32_t* layers, const double* params,
+                         double hx, const int32_t* tile_bits, int block_threads, int num_blocks,
+                         double* energies_out, double* grads_out, double* elapsed_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,91 @@
+"""Batched persistent launch against the single-circuit adjoint engine and the
+oracle: both code paths (state resident in LDS, and two tiled HBM passes per
+layer), both precisions, determinism, and the autotuner."""
+
+import numpy as np
+import pytest
+
+from invariantq_amd import workloads
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def M():
+    from invariantq_amd import adjoint, batch
+    return adjoint, batch
+
+
+def _single(adjoint, n, layers, params, precision="complex128"):
+    c = adjoint.GpuCircuit.from_ansatz(workloads.Ansatz(n, layers), precision)
+    c.set_observable(workloads.tfim_terms(n))
+    return c.gradient(params)
+
+
+SHAPES = [(2, 3), (3, 2), (5, 4), (8, 3), (12, 2), (14, 2), (15, 3), (16, 1), (13, 2)]
+TILES = {8: 6, 14: 8, 15: 8, 16: 9, 12: 12, 13: 7}  # forces both paths: n <= width resident, else tiled
+
+
+def _specs(seed):
+    rng = np.random.default_rng(seed)
+    return [(n, l, rng.uniform(0, 2 * np.pi, 2 * n * l)) for n, l in SHAPES]
+
+
+def test_batch_matches_single_circuit_c128(M):
+    adjoint, batch = M
+    specs = _specs(1)
+    r = batch.BatchRunner("complex128")
+    e, g, ms = r.run(specs, tile_bits=TILES, block_threads=256)
+    assert ms > 0
+    for (n, l, p), ei, gi in zip(specs, e, g):
+        e_ref, g_ref = _single(adjoint, n, l, p)
+        assert abs(ei - e_ref) < 1e-10, (n, l)
+        assert np.max(np.abs(gi - g_ref)) < 1e-10, (n, l)
+
+
+def test_batch_small_against_oracle(M, oracle_lib):
+    _, batch = M
+    n, l = 5, 3
+    rng = np.random.default_rng(4)
+    p = rng.uniform(0, 2 * np.pi, 2 * n * l)
+    e, g, _ = batch.BatchRunner("complex128").run([(n, l, p)])
+    terms = orc.tfim_terms(n)
+    assert abs(e[0] - orc.energy(oracle_lib, n, orc.hea_ops(n, l, p), terms)) < 1e-12
+    ps = orc.grad_parameter_shift(oracle_lib, n, lambda x: orc.hea_ops(n, l, x), terms, p)
+    assert np.max(np.abs(g[0] - ps)) < 1e-11
+
+
+def test_batch_complex64_error_and_determinism(M):
+    adjoint, batch = M
+    specs = _specs(2)
+    r = batch.BatchRunner("complex64")
+    e1, g1, _ = r.run(specs, tile_bits=TILES, block_threads=512)
+    e2, g2, _ = r.run(list(reversed(specs)), tile_bits=TILES, block_threads=512)
+    for (n, l, p), a, b, ga, gb in zip(specs, e1, reversed(e2), g1, reversed(g2)):
+        assert a == b and np.array_equal(ga, gb)  # independent of queue order
+        e_ref, g_ref = _single(adjoint, n, l, p)
+        assert abs(a - e_ref) < 1e-4
+        assert np.linalg.norm(ga - g_ref) / np.linalg.norm(g_ref) < 1e-4
+
+
+def test_batch_validation(M):
+    _, batch = M
+    r = batch.BatchRunner("complex64")
+    with pytest.raises(ValueError):
+        r.run([(20, 2, np.zeros(80))], tile_bits={20: 9})  # 2 * width < n
+    with pytest.raises(ValueError):
+        r.run([(4, 2, np.zeros(3))])
+
+
+def test_autotuner_table(M, tmp_path):
+    _, batch = M
+    rng = np.random.default_rng(5)
+    specs = [(n, 2, rng.uniform(0, 6.3, 4 * n)) for n in (10, 14) for _ in range(16)]
+    r = batch.BatchRunner("complex64")
+    t = batch.autotune(r, specs, cache=tmp_path / "tune.json", per_n=16)
+    assert set(t["tile_bits"]) == {10, 14} and t["block_threads"] in (256, 512)
+    again = batch.autotune(r, specs, cache=tmp_path / "tune.json", per_n=16)
+    assert again["cached"] and again["tile_bits"] == t["tile_bits"]
+    e, g, _ = r.run(specs, tile_bits=t["tile_bits"], block_threads=t["block_threads"])
+    assert np.all(np.isfinite(e))
