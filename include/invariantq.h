@@ -105,9 +105,10 @@ void        iq_clear_error(void);
 const char* iq_backend_name(void);
 /* Gate kernel selection: 0 automatic, 1 direct (group per lane), 2 LDS-tiled, 3 matrix-core (5-qubit blocks). */
 void        iq_set_gate_path(int path);
-/* Matrix-core kernel options: groups-of-16 batches per iteration (1, 2, 4; 0 = default)
- * and matrix fragments in LDS (1) or VGPRs (0; -1 = default). */
-void        iq_set_mfma_options(int batches_per_iteration, int matrix_in_lds);
+/* Matrix-core kernel options: groups-of-16 batches per iteration (1, 2, 4; 0 = default),
+ * matrix fragments in LDS (1) or VGPRs (0; -1 = default), non-temporal loads and stores
+ * (1) or cached (0; -1 = default), waves in the grid (0 = default). */
+void        iq_set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontemporal, int grid_waves);
 
 void*  iq_statevector_create_precision(size_t num_qubits, int precision);
 int    iq_statevector_get_precision(void* sv);

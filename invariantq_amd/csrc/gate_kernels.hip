@@ -202,7 +202,32 @@ struct Mfma<float> {
 // The matrix fragments (48 per lane) stay in VGPRs for the whole kernel, so
 // the VALU only does addressing and the final combination, and memory traffic
 // of one batch overlaps the matrix-core work of another wave.
-template <typename R, int UB, bool WLDS>
+// Amplitude load/store with an optional non-temporal (streaming) hint; the
+// builtins need a native vector type, so the pair is viewed as one.
+template <typename R>
+__device__ __forceinline__ cv<R> ld_stream(const cv<R>* p, bool nt) {
+    typedef R nv __attribute__((ext_vector_type(2)));
+    if (!nt) return *p;
+    const nv v = __builtin_nontemporal_load(reinterpret_cast<const nv*>(p));
+    cv<R> o;
+    o.x = v[0];
+    o.y = v[1];
+    return o;
+}
+template <typename R>
+__device__ __forceinline__ void st_stream(cv<R>* p, cv<R> v, bool nt) {
+    typedef R nv __attribute__((ext_vector_type(2)));
+    if (!nt) {
+        *p = v;
+        return;
+    }
+    nv w;
+    w[0] = v.x;
+    w[1] = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<nv*>(p));
+}
+
+template <typename R, int UB, bool WLDS, bool NT>
 __global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, MfmaArgs a, const R* __restrict__ frag) {
     using M = Mfma<R>;
     using v4 = typename M::v4;
@@ -238,7 +263,7 @@ __global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, Mfma
             base[u] = insert_zero_bits<5>((sb * UB + u) * 16 + nn, a.pos);
 #pragma unroll
             for (int kb = 0; kb < 8; ++kb) {
-                const cv<R> v = psi[base[u] + xq + a.xk[kb]];
+                const cv<R> v = ld_stream<R>(psi + base[u] + xq + a.xk[kb], NT);
                 xr[u][kb] = v.x;
                 xi[u][kb] = v.y;
             }
@@ -269,7 +294,7 @@ __global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, Mfma
                     cv<R> o;
                     o.x = p1[mt][jj] - p2[mt][jj];
                     o.y = p3[mt][jj] - p1[mt][jj] - p2[mt][jj];
-                    psi[base[u] + yq + a.yk[mt * 4 + jj]] = o;
+                    st_stream<R>(psi + base[u] + yq + a.yk[mt * 4 + jj], o, NT);
                 }
         }
     }
@@ -334,13 +359,23 @@ hipError_t run_tiled(void* psi, int n, hipStream_t st, const GateDesc& g) {
 
 static int g_mfma_batches = 0;   // 0 = per-precision default
 static int g_mfma_lds = -1;      // -1 = per-precision default
+static int g_mfma_nt = -1;       // non-temporal (streaming) loads and stores (-1 = default: on)
+static int g_mfma_waves = 0;     // waves in the grid (0 = per-precision default)
 
 template <typename R, int UB, bool WLDS>
 hipError_t launch_mfma(void* psi, const MfmaArgs& a, hipStream_t st, const void* dmat) {
     const uint64_t nsuper = a.nbatches / UB;
-    const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nsuper, 256ULL * 16ULL));
+    // defaults from the 30-qubit sweep: 8192 waves (c128), 1024 (c64)
+    const uint64_t target = g_mfma_waves > 0 ? (uint64_t)g_mfma_waves
+                                             : (std::is_same<R, double>::value ? 8192ULL : 1024ULL);
+    const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nsuper, target));
     const unsigned blocks = (unsigned)((waves + 3) / 4);
-    hipLaunchKernelGGL((k_gate_mfma<R, UB, WLDS>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, a, (const R*)dmat);
+    if (g_mfma_nt != 0)
+        hipLaunchKernelGGL((k_gate_mfma<R, UB, WLDS, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, a,
+                           (const R*)dmat);
+    else
+        hipLaunchKernelGGL((k_gate_mfma<R, UB, WLDS, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, a,
+                           (const R*)dmat);
     return hipGetLastError();
 }
 
@@ -410,10 +445,12 @@ hipError_t run_diag(void* psi, int n, hipStream_t st, const uint32_t* q, const d
 bool use_three_mult(int K) { return K >= 4; }
 
 // Matrix-core kernel options chosen by the autotuner (or left at the defaults).
-void set_mfma_options(int batches_per_iteration, int matrix_in_lds) {
+void set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontemporal, int grid_waves) {
     g_mfma_batches = (batches_per_iteration == 1 || batches_per_iteration == 2 || batches_per_iteration == 4)
                          ? batches_per_iteration : 0;
     g_mfma_lds = (matrix_in_lds == 0 || matrix_in_lds == 1) ? matrix_in_lds : -1;
+    g_mfma_nt = (nontemporal == 0 || nontemporal == 1) ? nontemporal : -1;
+    g_mfma_waves = grid_waves > 0 ? grid_waves : 0;
 }
 
 size_t gate_matrix_bytes(int prec, int K, bool three_mult) {
