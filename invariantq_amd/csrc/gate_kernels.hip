@@ -359,23 +359,40 @@ hipError_t run_tiled(void* psi, int n, hipStream_t st, const GateDesc& g) {
 
 static int g_mfma_batches = 0;   // 0 = per-precision default
 static int g_mfma_lds = -1;      // -1 = per-precision default
-static int g_mfma_nt = -1;       // non-temporal (streaming) loads and stores (-1 = default: on)
+static int g_mfma_nt = -1;       // non-temporal loads and stores: 1 on, 0 off, -1 by target set
 static int g_mfma_waves = 0;     // waves in the grid (0 = per-precision default)
+
+// The streaming hint pays only when every load/store instruction covers whole
+// 128-byte lines.  One instruction spans the two lowest targets (lane quad) and
+// the four lowest non-targets (groups); a line is 3 index bits (c128) or 4
+// (c64), so the lines are whole unless three or more targets sit below that.
+// With a split line the two halves arrive from two instructions, and the hint
+// makes L2 drop the line between them, so it is fetched twice.
+inline bool mfma_whole_lines(const MfmaArgs& a, bool f64) {
+    const uint32_t line_bits = f64 ? 3u : 4u;
+    int low = 0;
+    for (int m = 0; m < 5; ++m) low += (a.pos[m] < line_bits);
+    return low <= 2;
+}
 
 template <typename R, int UB, bool WLDS>
 hipError_t launch_mfma(void* psi, const MfmaArgs& a, hipStream_t st, const void* dmat) {
+    constexpr bool f64 = std::is_same<R, double>::value;
     const uint64_t nsuper = a.nbatches / UB;
-    // defaults from the 30-qubit sweep: 8192 waves (c128), 1024 (c64)
-    const uint64_t target = g_mfma_waves > 0 ? (uint64_t)g_mfma_waves
-                                             : (std::is_same<R, double>::value ? 8192ULL : 1024ULL);
+    // Grid from the 30-qubit sweep over 200 random target sets: 131072 waves
+    // (four super-batches each at 30 qubits) for c128; one super-batch per
+    // wave for c64.  Smaller grids leave fewer independent streams in flight.
+    const uint64_t target = g_mfma_waves > 0 ? (uint64_t)g_mfma_waves : (f64 ? 131072ULL : nsuper);
     const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nsuper, target));
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
-    if (g_mfma_nt != 0)
-        hipLaunchKernelGGL((k_gate_mfma<R, UB, WLDS, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, a,
+    const uint64_t blocks = (waves + 3) / 4;
+    if (blocks > 0x7fffffffULL) return hipErrorInvalidValue;
+    const bool nt = g_mfma_nt >= 0 ? g_mfma_nt != 0 : mfma_whole_lines(a, f64);
+    if (nt)
+        hipLaunchKernelGGL((k_gate_mfma<R, UB, WLDS, true>), dim3((unsigned)blocks), dim3(256), 0, st, (cv<R>*)psi, a,
                            (const R*)dmat);
     else
-        hipLaunchKernelGGL((k_gate_mfma<R, UB, WLDS, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, a,
-                           (const R*)dmat);
+        hipLaunchKernelGGL((k_gate_mfma<R, UB, WLDS, false>), dim3((unsigned)blocks), dim3(256), 0, st, (cv<R>*)psi,
+                           a, (const R*)dmat);
     return hipGetLastError();
 }
 
