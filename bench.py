@@ -356,8 +356,10 @@ def main():
                                    "so only --gpus 1 is measured", "n_gpus": a.gpus}))
         sys.exit(2)
 
-    import torch
+    # The binding first: it loads the HIP runtime the library was built with
+    # (ROCm 7.2), and torch then binds to that same copy instead of its own.
     from invariantq_amd.core import binding
+    import torch
     if not binding.HAS_NATIVE_CORE:
         raise SystemExit(f"HIP engine not available: {binding.native_core_status()}")
     torch.cuda.init()

@@ -108,9 +108,11 @@ void        iq_set_gate_path(int path);
 /* Matrix-core kernel options: groups-of-16 batches per iteration (1, 2, 4; 0 = default),
  * matrix fragments in LDS (1) or VGPRs (0; -1 = default), non-temporal loads and stores
  * (1) or cached (0; -1 = by target set: on when every instruction covers whole 128-byte
- * lines), waves in the grid (0 = default: 131072 for complex128, one per 4 batches for
- * complex64). */
-void        iq_set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontemporal, int grid_waves);
+ * lines), waves in the grid (0 = default: 131072 for complex128, 262144 for complex64),
+ * rotation of the super-batch visit order in bits (-1 = default: 2 for complex128 when a
+ * target is at qubit 10 or above, else 0). */
+void        iq_set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontemporal, int grid_waves,
+                                int batch_rotation);
 
 void*  iq_statevector_create_precision(size_t num_qubits, int precision);
 int    iq_statevector_get_precision(void* sv);

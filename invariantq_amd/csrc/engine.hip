@@ -433,8 +433,9 @@ void iq_clear_error(void) {
 }
 const char* iq_backend_name(void) { return "hip-gfx950"; }
 void iq_set_gate_path(int path) { g_gate_path = (path >= 0 && path <= 3) ? path : 0; }
-void iq_set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontemporal, int grid_waves) {
-    set_mfma_options(batches_per_iteration, matrix_in_lds, nontemporal, grid_waves);
+void iq_set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontemporal, int grid_waves,
+                        int batch_rotation) {
+    set_mfma_options(batches_per_iteration, matrix_in_lds, nontemporal, grid_waves, batch_rotation);
 }
 
 // ------------------------------------------------------------- statevector

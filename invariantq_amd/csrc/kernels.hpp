@@ -16,7 +16,8 @@ bool use_three_mult(int K);
 // Matrix-core kernel options: batches of 16 groups per loop iteration (1, 2, 4;
 // 0 = default), matrix fragments in LDS (1) or VGPRs (0; -1 = default),
 // non-temporal loads/stores (1) or cached (0; -1 = default), waves in the grid (0 = default).
-void set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontemporal, int grid_waves);
+void set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontemporal, int grid_waves,
+                      int batch_rotation);
 // Fragment-ordered copy of a 32x32 block for the matrix-core kernel (depends on the targets).
 size_t mfma_fragment_bytes(int prec);
 void pack_mfma_fragments(int prec, int n, const uint32_t* q, const double* re, const double* im, void* out);

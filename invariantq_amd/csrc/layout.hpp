@@ -112,7 +112,16 @@ struct MfmaArgs {
     uint64_t xq[4], xk[8];
     uint64_t yq[4], yk[8];
     uint64_t nbatches;    // 2^(n-5) / 16
+    uint32_t rot, bits;   // visit order of super-batches: rotate_index(i, rot, bits)
 };
+
+// Left rotation of i inside its low `bits` bits: a bijection on [0, 2^bits).
+// With rot = r, waves that run at the same time take super-batches 2^r apart.
+IQ_HD uint64_t rotate_index(uint64_t i, uint32_t rot, uint32_t bits) {
+    if (rot == 0 || rot >= bits) return i;
+    const uint64_t mask = (1ULL << bits) - 1;
+    return ((i << rot) | (i >> (bits - rot))) & mask;
+}
 
 // Local (MSB-first) index of tile column kappa / tile row rho, used to permute
 // the matrix into MFMA fragment order on the host.
@@ -155,6 +164,8 @@ inline void fill_mfma_args(MfmaArgs& a, int n, const uint32_t* q, bool f64, uint
         s_out[rho] = s;
     }
     a.nbatches = (n >= 9) ? (1ULL << (n - 5)) / 16 : 0;
+    a.rot = 0;
+    a.bits = 0;
 }
 
 // Requires n >= K + 6 and distinct targets < n.

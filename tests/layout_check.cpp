@@ -59,8 +59,32 @@ static int check(int n, const uint32_t* q) {
         fill_mfma_args(ma, n, q, f64 == 1, s_in, s_out);
         std::set<uint32_t> pin(s_in, s_in + 32), pout(s_out, s_out + 32);
         if (pin.size() != 32 || pout.size() != 32) return fail("mfma local maps not permutations", n, K);
+        // the kernel's traversal: super-batch sb0 -> rotate_index -> UB batches;
+        // every rotation must visit each batch exactly once
+        std::vector<uint64_t> order;
+        for (uint32_t ub : {1u, 2u, 4u})
+            for (uint32_t rot : {0u, 1u, 2u, 5u}) {
+                if (ma.nbatches % ub) continue;  // the launcher lowers UB until it divides
+                const uint64_t nsuper = ma.nbatches / ub;
+                uint32_t bits = 0;
+                while ((1ULL << bits) < nsuper) ++bits;
+                if ((1ULL << bits) != nsuper) return fail("mfma super-batch count not a power of two", n, K);
+                std::vector<unsigned char> hit(ma.nbatches, 0);
+                std::vector<uint64_t> ord;
+                for (uint64_t sb0 = 0; sb0 < nsuper; ++sb0) {
+                    const uint64_t sb = rotate_index(sb0, rot, bits);
+                    if (sb >= nsuper) return fail("mfma rotated super-batch out of range", n, K);
+                    for (uint32_t u = 0; u < ub; ++u) {
+                        const uint64_t b = sb * ub + u;
+                        if (hit[b]++) return fail("mfma batch visited twice", n, K);
+                        ord.push_back(b);
+                    }
+                }
+                if (ord.size() != ma.nbatches) return fail("mfma batch missed", n, K);
+                if (rot == 2 || order.empty()) order = ord;
+            }
         std::vector<unsigned char> rd(N, 0), wr(N, 0);
-        for (uint64_t b = 0; b < ma.nbatches; ++b)
+        for (uint64_t b : order)
             for (uint32_t lane = 0; lane < 64; ++lane) {
                 const uint32_t qq = lane >> 4, nn = lane & 15;
                 uint32_t pos5[5];

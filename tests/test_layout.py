@@ -40,7 +40,7 @@ def test_layout_exhaustive_small(checker):
 def test_layout_random_wide(checker):
     rng = np.random.default_rng(0)
     cases = []
-    for n in (12, 14, 16):
+    for n in (9, 10, 12, 14, 16):
         for K in range(1, 6):
             for _ in range(30):
                 cases.append((n, [int(v) for v in rng.choice(n, K, replace=False)]))
