@@ -204,6 +204,38 @@ def circuit_workload(binding, torch, steps: int, warmup: int, cpu: bool):
            "circuit_plus_shots_ms": round(t_total * 1e3, 3),
            "effective_TBps_gpu_time": round(len(ops) * 2 * (1 << n) * 16 / t_gpu / 1e12, 3),
            "first_samples": [int(x) for x in samples[:4]]}
+    # the same circuit fused into blocks of up to 5 qubits (fusion.py)
+    from invariantq_amd import fusion
+    t0 = time.perf_counter()
+    fused = fusion.fuse_statevector(ops, 5)
+    t_fuse = time.perf_counter() - t0
+    unfused_amps = sv.get_amplitudes()
+
+    seq = [(f.qubits, f.matrix) for f in fused]
+
+    def run_fused():
+        sv.reset()
+        sv.apply_fused_sequence(seq)
+
+    for _ in range(warmup):
+        run_fused()
+    sv.synchronize()
+    tf, tf_gpu = 0.0, 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        ev[0].record(stream)
+        run_fused()
+        ev[1].record(stream)
+        sv.synchronize()
+        tf += time.perf_counter() - t0
+        tf_gpu += ev[0].elapsed_time(ev[1]) * 1e-3
+    tf, tf_gpu = tf / steps, tf_gpu / steps
+    fs = sv.sample(shots, seed)
+    out["fused"] = {"passes": len(fused), "widths": {str(k): sum(len(f.qubits) == k for f in fused) for k in range(1, 6)},
+                    "fusion_host_ms": round(t_fuse * 1e3, 2), "circuit_ms": round(tf * 1e3, 3),
+                    "circuit_gpu_ms": round(tf_gpu * 1e3, 3), "speedup_gpu_time": round(t_gpu / tf_gpu, 2),
+                    "max_abs_amp_diff_vs_unfused": float(np.max(np.abs(sv.get_amplitudes() - unfused_amps))),
+                    "samples_equal_unfused": bool(np.array_equal(fs, samples))}
     if cpu:
         from oracle import oracle as orc
         core = orc.oracle()
@@ -266,6 +298,28 @@ def density_matrix_workload(binding, torch, cpu: bool):
                       "TBps": round(byts / float(np.mean(v)) / 1e12, 4)}
     out["kraus_apply_TBps"] = out["kraus"]["TBps"]
     out["kraus_apply_pct_hbm_peak"] = round(100 * out["kraus"]["TBps"] / HBM_PEAK_TBPS, 2)
+    # the same circuit as fused superoperators on up to 2 qubits (fusion.py)
+    from invariantq_amd import fusion
+    t0 = time.perf_counter()
+    fused = fusion.fuse_density_matrix(ops, 2)
+    t_fuse = time.perf_counter() - t0
+    dm.reset()
+    dm.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    a.record(stream)
+    for f in fused:
+        dm.apply_superop(f.qubits, f.matrix)
+    b.record(stream)
+    dm.synchronize()
+    fwall = time.perf_counter() - t0
+    fgpu = a.elapsed_time(b) * 1e-3
+    ft, fp, fz = dm.trace(), dm.purity(), dm.expval_z()
+    out["fused"] = {"passes": len(fused), "fusion_host_ms": round(t_fuse * 1e3, 2), "wall_s": round(fwall, 4),
+                    "gpu_s": round(fgpu, 4), "avg_pass_TBps": round(byts * len(fused) / fgpu / 1e12, 4),
+                    "speedup_wall": round(wall / fwall, 2),
+                    "trace_diff": abs(ft - trace), "purity_diff": abs(fp - purity),
+                    "max_expval_z_diff": float(np.max(np.abs(np.asarray(fz) - np.asarray(z))))}
     if cpu:
         from invariantq_amd import gates as G
         from oracle import oracle as orc

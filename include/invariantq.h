@@ -130,6 +130,12 @@ void   iq_statevector_get_cdf(void* sv, double* cdf_out);
 void   iq_statevector_expval_pauli_sum(void* sv, size_t nterms, const double* coeffs, const char* ops,
                                        const size_t* qubits, const size_t* lens, double* out_real,
                                        double* out_imag, double* per_term);
+/* A fused gate sequence in one call (no reference counterpart; the output of a fusion pass,
+ * invariantq_amd/fusion.py): gate i acts on ks[i] qubits (1..5) taken in order from
+ * `qubits`, with 4^ks[i] matrix entries in order from u_real / u_imag, MSB-first as
+ * iq_statevector_apply_fused.  All gates are checked before any runs. */
+void   iq_statevector_apply_fused_sequence(void* sv, size_t count, const size_t* ks, const size_t* qubits,
+                                           const double* u_real, const double* u_imag);
 void   iq_statevector_synchronize(void* sv);
 void*  iq_statevector_device_ptr(void* sv);  /* device address of the amplitudes (re, im interleaved) */
 void*  iq_statevector_stream(void* sv);      /* hipStream_t the handle's work is ordered on */
@@ -139,6 +145,12 @@ void   iq_density_matrix_set_elements(void* dm, const double* real_in, const dou
 void   iq_density_matrix_apply_kraus(void* dm, const size_t* qubits, size_t k, size_t num_kraus,
                                      const double* k_real, const double* k_imag);
 /* <Z_q> for every qubit q from the diagonal (nq doubles). */
+/* Caller-composed superoperator on 1 or 2 qubits (gate + channel fusion; no reference
+ * counterpart: the reference applies each gate and channel separately,
+ * invariantq/simulator/density_matrix.py:35-58).  S is 4^k x 4^k row-major on the local
+ * index (row bits of qubits, then column bits), MSB-first, S = sum K (x) conj(K). */
+void   iq_density_matrix_apply_superop(void* dm, const size_t* qubits, size_t k, const double* s_real,
+                                       const double* s_imag);
 void   iq_density_matrix_expval_z(void* dm, double* out);
 void   iq_density_matrix_synchronize(void* dm);
 void*  iq_density_matrix_stream(void* dm);

@@ -137,11 +137,13 @@ SIGNATURES = {
     "iq_statevector_sample_uniforms": ([_vp, _sz, _d, _u64], None),
     "iq_statevector_get_cdf": ([_vp, _d], None),
     "iq_statevector_expval_pauli_sum": ([_vp, _sz, _d, ctypes.c_char_p, _s, _s, _d, _d, _d], None),
+    "iq_statevector_apply_fused_sequence": ([_vp, _sz, _s, _s, _d, _d], None),
     "iq_statevector_synchronize": ([_vp], None),
     "iq_statevector_device_ptr": ([_vp], _vp),
     "iq_statevector_stream": ([_vp], _vp),
     "iq_density_matrix_set_elements": ([_vp, _d, _d], None),
     "iq_density_matrix_apply_kraus": ([_vp, _s, _sz, _sz, _d, _d], None),
+    "iq_density_matrix_apply_superop": ([_vp, _s, _sz, _d, _d], None),
     "iq_density_matrix_expval_z": ([_vp, _d], None),
     "iq_density_matrix_synchronize": ([_vp], None),
     "iq_density_matrix_stream": ([_vp], _vp),
@@ -262,6 +264,23 @@ class NativeStateVector:
         re, im = _split(matrix, 4 ** len(qubits))
         q = np.ascontiguousarray(qubits, dtype=np.uint64)
         _NATIVE_LIB.iq_statevector_apply_fused(self._ptr, q.ctypes.data_as(_s), len(qubits), _dp(re), _dp(im))
+        check_error()
+
+    def apply_fused_sequence(self, gates_seq) -> None:
+        """Apply [(qubits, matrix), ...] in one call: the gates of a fusion
+        pass (each 1..5 qubits, MSB-first), matrices uploaded in one copy."""
+        gl = [(list(q), np.asarray(m, dtype=np.complex128)) for q, m in gates_seq]
+        if not gl:
+            return
+        for q, m in gl:
+            if m.shape != (1 << len(q), 1 << len(q)):
+                raise ValueError(f"matrix of shape {m.shape} does not fit qubits {q}")
+        ks = np.ascontiguousarray([len(q) for q, _ in gl], dtype=np.uint64)
+        qs = np.ascontiguousarray([x for q, _ in gl for x in q], dtype=np.uint64)
+        flat = np.concatenate([m.reshape(-1) for _, m in gl])
+        re, im = np.ascontiguousarray(flat.real), np.ascontiguousarray(flat.imag)
+        _NATIVE_LIB.iq_statevector_apply_fused_sequence(self._ptr, len(gl), ks.ctypes.data_as(_s), qs.ctypes.data_as(_s),
+                                                        _dp(re), _dp(im))
         check_error()
 
     def apply_diagonal(self, qubits: Sequence[int], diag):
@@ -423,6 +442,16 @@ class NativeDensityMatrix:
         re, im = _split(stacked, 16 * len(kraus_ops))
         q = np.ascontiguousarray([q0, q1], dtype=np.uint64)
         _NATIVE_LIB.iq_density_matrix_apply_kraus(self._ptr, q.ctypes.data_as(_s), 2, len(kraus_ops), _dp(re), _dp(im))
+        check_error()
+
+    def apply_superop(self, qubits, s):
+        """Superoperator on 1 or 2 qubits: 4^k x 4^k, local index (row bits of
+        `qubits`, then column bits), MSB-first; S = sum K (x) conj(K) is
+        rho -> sum K rho K^dagger.  The output of ``fusion.fuse_density_matrix``."""
+        qs = np.ascontiguousarray([int(q) for q in qubits], dtype=np.uint64)
+        k = len(qs)
+        re, im = _split(s, 1 << (4 * k))
+        _NATIVE_LIB.iq_density_matrix_apply_superop(self._ptr, qs.ctypes.data_as(_s), k, _dp(re), _dp(im))
         check_error()
 
     def get_matrix(self) -> np.ndarray:

@@ -91,7 +91,11 @@ bool Engine::init(int nqubits, int precision) {
     slot_bytes = std::max(gate_matrix_bytes(kC128, kMaxK, true), mfma_fragment_bytes(kC128));
     slots.resize(kSlots);
     for (auto& s : slots) {
-        if (!hip_ok(hipHostMalloc(&s.host, slot_bytes), "pinned alloc") || !hip_ok(hipMalloc(&s.dev, slot_bytes), "alloc") ||
+        // coherent: the upload kernel reads the slot uncached, so a reused slot
+        // is never served from a stale GPU cache line
+        if (!hip_ok(hipHostMalloc(&s.host, slot_bytes, hipHostMallocCoherent), "pinned alloc") ||
+            !hip_ok(hipHostGetDevicePointer(&s.host_dev, s.host, 0), "pinned map") ||
+            !hip_ok(hipMalloc(&s.dev, slot_bytes), "alloc") ||
             !hip_ok(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "event"))
             return false;
     }
@@ -100,6 +104,9 @@ bool Engine::init(int nqubits, int precision) {
 
 Engine::~Engine() {
     if (stream) hipStreamSynchronize(stream);
+    if (seq.host) hipHostFree(seq.host);
+    if (seq.dev) hipFree(seq.dev);
+    if (seq.ev) hipEventDestroy(seq.ev);
     for (auto& s : slots) {
         if (s.host) hipHostFree(s.host);
         if (s.dev) hipFree(s.dev);
@@ -130,7 +137,9 @@ const void* Engine::upload_matrix(int K, bool m3, const double* re, const double
         pack_gate_matrix(prec, K, m3, re, im, s.host);
         bytes = gate_matrix_bytes(prec, K, m3);
     }
-    if (!hip_ok(hipMemcpyAsync(s.dev, s.host, bytes, hipMemcpyHostToDevice, stream), "matrix upload")) return nullptr;
+    // a copy kernel on this stream, not hipMemcpyAsync: the copy engine adds
+    // tens of microseconds of cross-queue signalling before the gate kernel
+    if (!hip_ok(launch_upload(s.dev, s.host_dev, (bytes + 15) & ~size_t(15), stream), "matrix upload")) return nullptr;
     hip_ok(hipEventRecord(s.ev, stream), "event record");
     s.pending = true;
     return s.dev;
@@ -144,7 +153,7 @@ static bool is_diagonal(int K, const double* re, const double* im) {
     return true;
 }
 
-bool Engine::apply(int K, const uint32_t* q, const double* re, const double* im, int path) {
+bool Engine::check_gate(int K, const uint32_t* q) const {
     if (K < 1 || K > kMaxK || K > n) {
         set_error(kErrInvalidArgument, "gate on %d qubits is not supported (1..5, <= n)", K);
         return false;
@@ -160,32 +169,120 @@ bool Engine::apply(int K, const uint32_t* q, const double* re, const double* im,
                 return false;
             }
     }
+    return true;
+}
+
+// Kernel choice for one (already checked) gate: diagonal pass, matrix by
+// value (K <= 3 direct), or a device copy of `bytes` (MFMA fragments or the
+// packed row layout).
+Engine::GatePlan Engine::plan_gate(int K, const uint32_t* q, const double* re, const double* im, int path) const {
+    GatePlan p;
+    p.g.K = K;
+    for (int m = 0; m < K; ++m) p.g.q[m] = q[m];
+    p.g.re = re;
+    p.g.im = im;
+    p.g.path = path;
     if (path == kPathAuto && is_diagonal(K, re, im)) {
-        const int d = 1 << K;
+        p.kind = GatePlan::kDiag;
+        return p;
+    }
+    int chosen = choose_path(n, prec, p.g);
+    if (chosen == kPathMfma && (K != 5 || n < 9)) chosen = kPathDirect;
+    p.g.path = chosen;
+    if (chosen == kPathMfma) {
+        p.kind = GatePlan::kDevice;
+        p.mfma = true;
+        p.bytes = mfma_fragment_bytes(prec);
+    } else if (K >= 4 || chosen == kPathTiled) {
+        p.kind = GatePlan::kDevice;
+        p.m3 = use_three_mult(K);
+        p.bytes = gate_matrix_bytes(prec, K, p.m3);
+    } else {
+        p.kind = GatePlan::kValue;
+    }
+    return p;
+}
+
+void Engine::pack_plan(const GatePlan& p, void* host_dst) const {
+    if (p.mfma) pack_mfma_fragments(prec, n, p.g.q, p.g.re, p.g.im, host_dst);
+    else pack_gate_matrix(prec, p.g.K, p.m3, p.g.re, p.g.im, host_dst);
+}
+
+bool Engine::launch_plan(GatePlan& p, const void* dmat) {
+    if (p.kind == GatePlan::kDiag) {
+        const int d = 1 << p.g.K;
         double dre[32], dimv[32];
         for (int l = 0; l < d; ++l) {
-            dre[l] = re[l * d + l];
-            dimv[l] = im[l * d + l];
+            dre[l] = p.g.re[l * d + l];
+            dimv[l] = p.g.im[l * d + l];
         }
-        return hip_ok(launch_diag(amp, n, prec, stream, K, q, dre, dimv), "diagonal gate launch");
+        return hip_ok(launch_diag(amp, n, prec, stream, p.g.K, p.g.q, dre, dimv), "diagonal gate launch");
     }
-    GateDesc g;
-    g.K = K;
-    for (int m = 0; m < K; ++m) g.q[m] = q[m];
-    g.re = re;
-    g.im = im;
-    g.path = path;
-    int chosen = choose_path(n, prec, g);
-    if (chosen == kPathMfma && (K != 5 || n < 9)) chosen = kPathDirect;
-    g.path = chosen;
-    if (chosen == kPathMfma) {
-        g.dmat = upload_matrix(K, false, re, im, g.q);
-        if (!g.dmat) return false;
-    } else if (K >= 4 || chosen == kPathTiled) {
-        g.dmat = upload_matrix(K, use_three_mult(K), re, im, nullptr);
-        if (!g.dmat) return false;
+    p.g.dmat = dmat;
+    return hip_ok(launch_gate(amp, n, prec, stream, p.g), "gate launch");
+}
+
+bool Engine::apply(int K, const uint32_t* q, const double* re, const double* im, int path) {
+    if (!check_gate(K, q)) return false;
+    GatePlan p = plan_gate(K, q, re, im, path);
+    const void* dmat = nullptr;
+    if (p.kind == GatePlan::kDevice) {
+        dmat = upload_matrix(K, p.m3, re, im, p.mfma ? p.g.q : nullptr);
+        if (!dmat) return false;
     }
-    return hip_ok(launch_gate(amp, n, prec, stream, g), "gate launch");
+    return launch_plan(p, dmat);
+}
+
+// A whole gate sequence: every gate checked first (an invalid one leaves the
+// state untouched), every device matrix packed into one pinned buffer and
+// uploaded once, then the gate kernels back to back on the stream.
+bool Engine::apply_sequence(size_t count, const int* ks, const uint32_t* qubits, const double* re, const double* im,
+                            int path) {
+    std::vector<GatePlan> plans(count);
+    std::vector<size_t> offs(count, 0);
+    size_t qo = 0, mo = 0, bytes = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const int K = ks[i];
+        if (K < 1 || K > kMaxK) {
+            set_error(kErrInvalidArgument, "gate %zu: %d qubits is not supported (1..5)", i, K);
+            return false;
+        }
+        if (!check_gate(K, qubits + qo)) return false;
+        plans[i] = plan_gate(K, qubits + qo, re + mo, im + mo, path);
+        if (plans[i].kind == GatePlan::kDevice) {
+            offs[i] = bytes;
+            bytes += (plans[i].bytes + 255) & ~size_t(255);
+        }
+        qo += (size_t)K;
+        mo += (size_t)1 << (2 * K);
+    }
+    if (bytes) {
+        if (seq.pending) hip_ok(hipEventSynchronize(seq.ev), "sequence wait");
+        seq.pending = false;
+        if (bytes > seq.cap) {
+            if (seq.host) hipHostFree(seq.host);
+            if (seq.dev) hipFree(seq.dev);
+            seq.host = seq.host_dev = seq.dev = nullptr;
+            seq.cap = 0;
+            if (!hip_ok(hipHostMalloc(&seq.host, bytes, hipHostMallocCoherent), "pinned alloc") ||
+                !hip_ok(hipHostGetDevicePointer(&seq.host_dev, seq.host, 0), "pinned map") ||
+                !hip_ok(hipMalloc(&seq.dev, bytes), "alloc"))
+                return false;
+            seq.cap = bytes;
+        }
+        if (!seq.ev && !hip_ok(hipEventCreateWithFlags(&seq.ev, hipEventDisableTiming), "event")) return false;
+        for (size_t i = 0; i < count; ++i)
+            if (plans[i].kind == GatePlan::kDevice) pack_plan(plans[i], (char*)seq.host + offs[i]);
+        if (!hip_ok(launch_upload(seq.dev, seq.host_dev, bytes, stream), "sequence upload")) return false;
+    }
+    for (size_t i = 0; i < count; ++i)
+        if (!launch_plan(plans[i], plans[i].kind == GatePlan::kDevice ? (const char*)seq.dev + offs[i] : nullptr))
+            return false;
+    if (bytes) {
+        if (!hip_ok(hipEventRecord(seq.ev, stream), "event record")) return false;
+        seq.pending = true;
+    }
+    return true;
 }
 
 bool Engine::sync() { return hip_ok(hipStreamSynchronize(stream), "stream synchronize"); }
@@ -496,6 +593,30 @@ void iq_statevector_apply_fused(void* sv, const size_t* qubits, size_t k, const 
     uint32_t q[kMaxK];
     to_u32(qubits, k, q);
     sv_of(sv)->apply((int)k, q, u_real, u_imag, gate_path_override());
+}
+
+// A sequence of fused gates in one call (the output of a fusion pass): gate i
+// acts on ks[i] qubits (1..5) taken in order from `qubits`, its 4^ks[i]
+// matrix entries in order from u_real / u_imag, MSB-first as apply_fused.
+// All gates are checked before any runs; their matrices go up in one copy.
+void iq_statevector_apply_fused_sequence(void* sv, size_t count, const size_t* ks, const size_t* qubits,
+                                         const double* u_real, const double* u_imag) {
+    if (!sv || !ks || !qubits || !u_real || !u_imag || count == 0) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    std::vector<int> k32(count);
+    size_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if (ks[i] < 1 || ks[i] > (size_t)kMaxK) {
+            set_error(kErrInvalidArgument, "fused gate %zu on %zu qubits is not supported (1..5)", i, ks[i]);
+            return;
+        }
+        k32[i] = (int)ks[i];
+        total += ks[i];
+    }
+    std::vector<uint32_t> q(total);
+    to_u32(qubits, total, q.data());
+    e->apply_sequence(count, k32.data(), q.data(), u_real, u_imag, gate_path_override());
 }
 
 void iq_statevector_apply_diagonal(void* sv, const size_t* qubits, size_t k, const double* d_real, const double* d_imag) {
@@ -844,6 +965,34 @@ void iq_density_matrix_apply_kraus(void* dm, const size_t* qubits, size_t k, siz
             return;
         }
     dm_apply_superop(h, q, (int)k, num_kraus, k_real, k_imag);
+}
+
+// Superoperator composed by the caller (gate and channel fusion) on 1 or 2
+// qubits: S is 4^k x 4^k, row-major, on the local index (row bits of
+// qubits[0..k-1], then their column bits), MSB-first; rho -> sum K rho K^dagger
+// is S = sum K (x) conj(K).  One pass over rho, like a single Kraus channel.
+void iq_density_matrix_apply_superop(void* dm, const size_t* qubits, size_t k, const double* s_real,
+                                     const double* s_imag) {
+    if (!dm || !qubits || !s_real || !s_imag) return;
+    DMHandle* h = dm_of(dm);
+    DeviceGuard g(h->eng.dev);
+    if (k < 1 || k > 2) {
+        set_error(kErrInvalidArgument, "superoperators act on 1 or 2 qubits");
+        return;
+    }
+    uint32_t q[2];
+    to_u32(qubits, k, q);
+    for (size_t i = 0; i < k; ++i)
+        if ((int)q[i] >= h->nq || (i == 1 && q[0] == q[1])) {
+            set_error(kErrInvalidArgument, "invalid superoperator qubits");
+            return;
+        }
+    uint32_t bits[4];
+    for (size_t i = 0; i < k; ++i) {
+        bits[i] = q[i] + (uint32_t)h->nq;  // row qubits
+        bits[k + i] = q[i];                 // column qubits
+    }
+    h->eng.apply(2 * (int)k, bits, s_real, s_imag, gate_path_override());
 }
 
 void iq_density_matrix_get_elements(void* dm, double* real_out, double* imag_out) {

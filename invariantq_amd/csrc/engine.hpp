@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "kernels.hpp"
 
 namespace iqg {
 
@@ -41,6 +42,7 @@ struct Engine {
 
     struct Slot {
         void* host = nullptr;
+        void* host_dev = nullptr;  // device view of the pinned slot
         void* dev = nullptr;
         hipEvent_t ev = nullptr;
         bool pending = false;
@@ -63,6 +65,25 @@ struct Engine {
     size_t work_doubles = 0;
     double* stage = nullptr;
 
+    // matrices of one apply_sequence call: pinned (coherent) + device copy
+    struct SeqBuf {
+        void* host = nullptr;
+        void* host_dev = nullptr;
+        void* dev = nullptr;
+        size_t cap = 0;
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+    } seq;
+
+    struct GatePlan {
+        enum Kind { kDiag, kValue, kDevice };
+        Kind kind = kValue;
+        GateDesc g;
+        size_t bytes = 0;
+        bool mfma = false;
+        bool m3 = false;
+    };
+
     Engine() = default;
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
@@ -71,7 +92,13 @@ struct Engine {
     bool init(int nqubits, int precision);
     bool reset_basis(uint64_t index);
     const void* upload_matrix(int K, bool m3, const double* re, const double* im, const uint32_t* mfma_q);
+    bool check_gate(int K, const uint32_t* q) const;
+    GatePlan plan_gate(int K, const uint32_t* q, const double* re, const double* im, int path) const;
+    void pack_plan(const GatePlan& p, void* host_dst) const;
+    bool launch_plan(GatePlan& p, const void* dmat);
     bool apply(int K, const uint32_t* q, const double* re, const double* im, int path);
+    bool apply_sequence(size_t count, const int* ks, const uint32_t* qubits, const double* re, const double* im,
+                        int path);
     bool sync();
     bool ensure_work(size_t doubles);
     bool ensure_stage();

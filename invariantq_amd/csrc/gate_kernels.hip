@@ -425,6 +425,11 @@ hipError_t run_mfma(void* psi, int n, hipStream_t st, const GateDesc& g) {
     const bool lds = g_mfma_lds >= 0 ? g_mfma_lds != 0 : false;
     (void)f64;
     while (ub > 1 && a.nbatches % ub) ub >>= 1;
+    // small and mid-size states: more waves before more batches per wave,
+    // keeping at least 32768 waves in the grid (measured on fused 24-qubit
+    // circuits: one batch per wave 12.6 ms, four 14.1 ms for the same blocks)
+    if (!g_mfma_batches)
+        while (ub > 1 && a.nbatches / ub < 32768) ub >>= 1;
     if (lds) {
         if (ub >= 4) return launch_mfma<R, 4, true>(psi, a, st, g.dmat);
         if (ub == 2) return launch_mfma<R, 2, true>(psi, a, st, g.dmat);

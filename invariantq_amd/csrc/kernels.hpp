@@ -42,6 +42,8 @@ int choose_path(int n, int prec, const GateDesc& g);
 
 // State initialisation and element-wise passes.
 hipError_t launch_init_basis(void* psi, int n, int prec, uint64_t index, hipStream_t st);
+// Copies `bytes` (a multiple of 16) from device-accessible pinned host memory.
+hipError_t launch_upload(void* dst, const void* src_host, size_t bytes, hipStream_t st);
 hipError_t launch_collapse(void* psi, int n, int prec, int target, int outcome, double factor, hipStream_t st);
 hipError_t launch_scale(void* psi, uint64_t count, int prec, double factor, hipStream_t st);
 hipError_t launch_split(const void* psi, uint64_t first, uint64_t count, int prec, double* re, double* im, hipStream_t st);

@@ -344,6 +344,15 @@ __global__ void __launch_bounds__(256) k_partial_trace(const cv<R>* __restrict__
     out[id] = make_double2(sr, si);
 }
 
+// Gate-matrix upload on the compute stream: reads pinned (coherent, uncached)
+// host memory over PCIe and writes device memory, so the gate kernel that
+// follows needs no copy-engine round trip.  One 16-byte element per thread
+// per round trip: the grid covers the buffer in as few round trips as it can.
+__global__ void __launch_bounds__(256) k_upload(const double2* __restrict__ src, double2* __restrict__ dst, uint32_t n2) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
 
 hipError_t launch_init_basis(void* psi, int n, int prec, uint64_t index, hipStream_t st) {
@@ -351,6 +360,14 @@ hipError_t launch_init_basis(void* psi, int n, int prec, uint64_t index, hipStre
     const unsigned b = blocks_for(count, 256, 65536);
     if (prec == kC128) hipLaunchKernelGGL(k_init_basis<double>, dim3(b), dim3(256), 0, st, (double2*)psi, count, index);
     else hipLaunchKernelGGL(k_init_basis<float>, dim3(b), dim3(256), 0, st, (float2*)psi, count, index);
+    return hipGetLastError();
+}
+
+hipError_t launch_upload(void* dst, const void* src_host, size_t bytes, hipStream_t st) {
+    if (bytes % 16 || bytes / 16 > 0xffffffffULL) return hipErrorInvalidValue;
+    const uint32_t n2 = (uint32_t)(bytes / 16);
+    const unsigned blocks = std::max(1u, std::min(1024u, (n2 + 255) / 256));
+    hipLaunchKernelGGL(k_upload, dim3(blocks), dim3(256), 0, st, (const double2*)src_host, (double2*)dst, n2);
     return hipGetLastError();
 }
 

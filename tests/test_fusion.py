@@ -1,0 +1,181 @@
+"""Gate and channel fusion (invariantq_amd/fusion.py) on the CPU.
+
+Pinned by code written independently of the fusion module: the statevector
+programs by products of the oracle's brute-force 2^n x 2^n operators
+(`full_operator`, MSB-first), the density-matrix programs by the compiled
+oracle's op-by-op restatement of density_matrix.cpp (unitaries, Kraus
+channels, reset), with the fused superoperators applied by the oracle's
+tensor-axis routine on the vectorised rho (row qubit q = bit n+q, column
+qubit q = bit q).
+"""
+
+import numpy as np
+import pytest
+
+from invariantq_amd import fusion, gates, workloads
+from oracle.oracle import apply_tensor, full_operator, haar_unitary
+
+
+def random_gates(rng, n, count, kmax):
+    ops = []
+    for _ in range(count):
+        k = int(rng.integers(1, kmax + 1))
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        ops.append(("gate", f"u{k}", qs, haar_unitary(rng, 1 << k)))
+    return ops
+
+
+def product(n, items):
+    t = np.eye(1 << n, dtype=np.complex128)
+    for qs, m in items:
+        t = full_operator(n, qs, m) @ t
+    return t
+
+
+@pytest.mark.parametrize("width", [1, 2, 3, 4, 5])
+def test_statevector_fusion_equals_program(width):
+    rng = np.random.default_rng(10 + width)
+    n = 7
+    ops = random_gates(rng, n, 80, min(width, 3))
+    fused = fusion.fuse_statevector(ops, width)
+    assert all(len(f.qubits) <= width for f in fused)
+    assert sum(f.count for f in fused) == len(ops)
+    if width >= 2:
+        assert len(fused) < len(ops)
+    if width >= 4:
+        assert len(fused) < len(ops) // 2
+    a = product(n, [(o[2], o[3]) for o in ops])
+    b = product(n, [(list(f.qubits), f.matrix) for f in fused])
+    assert np.max(np.abs(a - b)) < 1e-12
+
+
+def test_statevector_fusion_u3_cx_brick_pass_count():
+    ops = workloads.random_u3_cx(10, 10, seed=4)
+    fused = fusion.fuse_statevector(ops, 5)
+    assert all(len(f.qubits) <= 5 for f in fused)
+    assert len(fused) * 4 <= len(ops)  # at least 4 gates per pass on this brick circuit
+    a = product(10, [(op.qubits, op.matrix) for op in ops])
+    b = product(10, [(list(f.qubits), f.matrix) for f in fused])
+    assert np.max(np.abs(a - b)) < 1e-11
+
+
+def test_embed_matches_brute_force():
+    rng = np.random.default_rng(3)
+    m = haar_unitary(rng, 4)
+    # m on wires (5, 2), embedded into (2, 7, 5): as full operators on 8 qubits they agree
+    e = fusion.embed(m, [5, 2], [2, 7, 5])
+    assert np.max(np.abs(full_operator(8, [5, 2], m) - full_operator(8, [2, 7, 5], e))) < 1e-14
+
+
+def test_fuser_rejects_bad_ops():
+    with pytest.raises(ValueError):
+        fusion.fuse_statevector([("gate", "u", [1, 1], np.eye(4))], 5)
+    with pytest.raises(ValueError):
+        fusion.fuse_statevector([("gate", "u", [0, 1, 2], np.eye(8))], 2)
+    with pytest.raises(ValueError):
+        fusion.Fuser(0, list)
+
+
+def random_rho(rng, n, rank=3):
+    vs = rng.normal(size=(rank, 1 << n)) + 1j * rng.normal(size=(rank, 1 << n))
+    w = rng.random(rank)
+    rho = sum(wi * np.outer(v, v.conj()) / np.vdot(v, v).real for wi, v in zip(w, vs))
+    return rho / np.trace(rho).real
+
+
+def apply_fused_dm(rho, n, fused):
+    vec = rho.reshape(-1)  # index r * 2^n + c
+    for f in fused:
+        bits = [n + q for q in f.qubits] + list(f.qubits)
+        vec = apply_tensor(vec, 2 * n, bits, f.matrix)
+    return vec.reshape(1 << n, 1 << n)
+
+
+def dm_program(rng, n, count):
+    ops = []
+    dep, amp = gates.depolarizing(0.03), gates.amplitude_damping(0.1)
+    for _ in range(count):
+        r = rng.random()
+        if r < 0.35:
+            q = int(rng.integers(n))
+            ops.append(("gate", "u3", [q], gates.u3(*rng.uniform(0, 2 * np.pi, 3))))
+        elif r < 0.6:
+            qs = [int(x) for x in rng.choice(n, 2, replace=False)]
+            ops.append(("gate", "u2", qs, haar_unitary(rng, 4)))
+        elif r < 0.8:
+            q = int(rng.integers(n))
+            ops.append(("noise", "depolarizing", [q], (0.03,)))
+        elif r < 0.93:
+            q = int(rng.integers(n))
+            ops.append(("noise", "amplitude_damping", [q], (0.1,)))
+        else:
+            ops.append(("reset", int(rng.integers(n))))
+    return ops, dep, amp
+
+
+def oracle_run(oracle_lib, n, rho0, ops, noise_model=None):
+    dm = oracle_lib.density_matrix(n)
+    dm.set_matrix(rho0)
+    for op in ops:
+        if op[0] == "gate":
+            qs, m = op[2], op[3]
+            if len(qs) == 1:
+                dm.apply_1q_unitary(qs[0], m)
+            else:
+                dm.apply_2q_unitary(qs[0], qs[1], m)
+            noise = noise_model.get(op[1]) if noise_model else None
+            if noise:
+                for q in qs:
+                    dm.apply_1q_kraus(q, noise)
+        elif op[0] == "noise":
+            ks = gates.build_kraus_channel(op[1], op[3])
+            for q in op[2]:
+                dm.apply_1q_kraus(q, ks)
+        else:
+            dm.reset_qubit(op[1])
+    return dm.get_matrix()
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_density_matrix_fusion_equals_oracle(oracle_lib, seed):
+    rng = np.random.default_rng(seed)
+    n = 4
+    rho0 = random_rho(rng, n)
+    ops, _, _ = dm_program(rng, n, 60)
+    fused = fusion.fuse_density_matrix(ops, 2)
+    assert all(len(f.qubits) <= 2 for f in fused)
+    assert sum(f.count for f in fused) == len(ops)
+    assert len(fused) < len(ops) // 2
+    want = oracle_run(oracle_lib, n, rho0, ops)
+    got = apply_fused_dm(rho0, n, fused)
+    assert np.max(np.abs(got - want)) < 1e-12
+    assert abs(np.trace(got).real - 1.0) < 1e-12
+
+
+def test_density_matrix_fusion_noise_model(oracle_lib):
+    rng = np.random.default_rng(9)
+    n = 4
+    rho0 = random_rho(rng, n)
+    ops = [("gate", "u3", [int(rng.integers(n))], gates.u3(*rng.uniform(0, 2 * np.pi, 3))) for _ in range(10)]
+    ops += [("gate", "cx", [0, 2], gates.CX), ("gate", "cx", [3, 1], gates.CX)]
+    model = {"u3": gates.depolarizing(0.02), "cx": gates.phase_damping(0.05)}
+    fused = fusion.fuse_density_matrix(ops, 2, noise_model=model)
+    assert sum(f.count for f in fused) == 10 * 2 + 2 * 3
+    want = oracle_run(oracle_lib, n, rho0, ops, model)
+    assert np.max(np.abs(apply_fused_dm(rho0, n, fused) - want)) < 1e-12
+
+
+def test_density_matrix_two_qubit_channel_and_reset():
+    """A 2-qubit Kraus set stays one channel; reset is |0><0|, |0><1|
+    (density_matrix.cpp:258-270): from |1><1| it gives |0><0|."""
+    rng = np.random.default_rng(2)
+    n = 3
+    ks = [np.sqrt(0.7) * np.eye(4), np.sqrt(0.3) * haar_unitary(rng, 4)]
+    items = fusion._kraus_items([0, 2], ks)
+    assert len(items) == 1 and items[0][0] == [0, 2]
+    rho = np.zeros((8, 8), dtype=np.complex128)
+    rho[2, 2] = 1.0  # qubit 1 set
+    out = apply_fused_dm(rho, n, fusion.fuse_density_matrix([("reset", 1)], 2))
+    want = np.zeros_like(rho)
+    want[0, 0] = 1.0
+    assert np.max(np.abs(out - want)) < 1e-15
