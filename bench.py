@@ -236,6 +236,24 @@ def circuit_workload(binding, torch, steps: int, warmup: int, cpu: bool):
                     "circuit_gpu_ms": round(tf_gpu * 1e3, 3), "speedup_gpu_time": round(t_gpu / tf_gpu, 2),
                     "max_abs_amp_diff_vs_unfused": float(np.max(np.abs(sv.get_amplitudes() - unfused_amps))),
                     "samples_equal_unfused": bool(np.array_equal(fs, samples))}
+    # seed-tree tasks (4 workers, job "cfg0", master 42): one call on one state,
+    # against the reference's way, a re-simulation plus sample() per task
+    from invariantq_amd import tasks as T
+    plan = T.plan_tasks("cfg0", shots, 42, 4)
+    run()
+    sv.synchronize()
+    t0 = time.perf_counter()
+    per = sv.sample_tasks([t.seed for t in plan], [t.shots for t in plan])
+    t_one = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    resim = []
+    for t in plan:
+        run()
+        resim.append(sv.sample(t.shots, t.seed))
+    t_re = time.perf_counter() - t0
+    out["tasks"] = {"tasks": len(plan), "shots": shots, "one_call_ms": round(t_one * 1e3, 3),
+                    "resimulate_per_task_ms": round(t_re * 1e3, 3), "speedup": round(t_re / t_one, 1),
+                    "indices_equal": bool(all(np.array_equal(a, b) for a, b in zip(per, resim)))}
     if cpu:
         from oracle import oracle as orc
         core = orc.oracle()

@@ -136,6 +136,12 @@ void   iq_statevector_expval_pauli_sum(void* sv, size_t nterms, const double* co
  * iq_statevector_apply_fused.  All gates are checked before any runs. */
 void   iq_statevector_apply_fused_sequence(void* sv, size_t count, const size_t* ks, const size_t* qubits,
                                            const double* u_real, const double* u_imag);
+/* Sampling tasks of a distributed job in one call (no reference counterpart; the reference
+ * re-simulates per task, invariantq/distributed/worker.py:52-95, with the seeds of
+ * invariantq/distributed/planner.py:110-115): task t draws shots[t] indices exactly as
+ * iq_statevector_sample(sv, shots[t], seeds[t], ...); results concatenated in task order. */
+void   iq_statevector_sample_tasks(void* sv, size_t ntasks, const uint64_t* seeds, const uint64_t* shots,
+                                   uint64_t* results_out);
 void   iq_statevector_synchronize(void* sv);
 void*  iq_statevector_device_ptr(void* sv);  /* device address of the amplitudes (re, im interleaved) */
 void*  iq_statevector_stream(void* sv);      /* hipStream_t the handle's work is ordered on */

@@ -138,6 +138,7 @@ SIGNATURES = {
     "iq_statevector_get_cdf": ([_vp, _d], None),
     "iq_statevector_expval_pauli_sum": ([_vp, _sz, _d, ctypes.c_char_p, _s, _s, _d, _d, _d], None),
     "iq_statevector_apply_fused_sequence": ([_vp, _sz, _s, _s, _d, _d], None),
+    "iq_statevector_sample_tasks": ([_vp, _sz, _u64, _u64, _u64], None),
     "iq_statevector_synchronize": ([_vp], None),
     "iq_statevector_device_ptr": ([_vp], _vp),
     "iq_statevector_stream": ([_vp], _vp),
@@ -335,6 +336,19 @@ class NativeStateVector:
         _NATIVE_LIB.iq_statevector_sample(self._ptr, num_shots, seed, out.ctypes.data_as(_u64))
         check_error()
         return out
+
+    def sample_tasks(self, seeds, shots) -> list:
+        """Shots of several tasks in one call: task t is sample(shots[t], seeds[t]);
+        returns one index array per task."""
+        sd = np.ascontiguousarray([int(x) for x in seeds], dtype=np.uint64)
+        sh = np.ascontiguousarray([int(x) for x in shots], dtype=np.uint64)
+        if sd.size != sh.size or sd.size == 0:
+            raise ValueError("seeds and shots must be non-empty and of equal length")
+        out = np.empty(int(sh.sum()), dtype=np.uint64)
+        _NATIVE_LIB.iq_statevector_sample_tasks(self._ptr, sd.size, sd.ctypes.data_as(_u64), sh.ctypes.data_as(_u64),
+                                                out.ctypes.data_as(_u64))
+        check_error()
+        return np.split(out, np.cumsum(sh)[:-1].astype(np.int64))
 
     def sample_uniforms(self, uniforms) -> np.ndarray:
         u = np.ascontiguousarray(uniforms, dtype=np.float64)

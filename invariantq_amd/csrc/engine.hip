@@ -32,6 +32,7 @@
 #include <new>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -754,6 +755,63 @@ void iq_statevector_sample(void* sv, size_t num_shots, uint64_t seed, uint64_t* 
     if (!hip_ok(hipMemcpyAsync(du, u.data(), num_shots * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
         !hip_ok(launch_search(cdf, e->dim, du, num_shots, dout, e->stream), "search") ||
         !hip_ok(hipMemcpyAsync(results_out, dout, num_shots * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
+        return;
+    e->sync();
+}
+
+// Several sampling tasks on one state in one call (the tasks of a distributed
+// job, invariantq/distributed/planner.py:110-163, each of which the reference
+// serves by re-simulating the circuit, invariantq/distributed/worker.py:52-95):
+// task t draws shots[t] indices with std::mt19937_64(seeds[t]) exactly as
+// iq_statevector_sample(sv, shots[t], seeds[t], ...) does; results are
+// concatenated in task order.  The CDF is built once and one search launch
+// serves every shot; the tasks' uniforms are drawn on host threads.
+void iq_statevector_sample_tasks(void* sv, size_t ntasks, const uint64_t* seeds, const uint64_t* shots,
+                                 uint64_t* results_out) {
+    if (!sv || !results_out || !seeds || !shots || ntasks == 0) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    std::vector<size_t> off(ntasks + 1, 0);
+    for (size_t t = 0; t < ntasks; ++t) {
+        if (shots[t] > (SIZE_MAX / 16) - off[t]) {
+            set_error(kErrInvalidArgument, "total shot count overflows");
+            return;
+        }
+        off[t + 1] = off[t] + shots[t];
+    }
+    const size_t total = off[ntasks];
+    const size_t nchunks = scan_scratch_doubles(e->dim);
+    if (!e->ensure_work(e->dim + nchunks + 2 * std::max<size_t>(total, 1))) return;
+    double* cdf = e->work;
+    double* scratch = cdf + e->dim;
+    double* du = scratch + nchunks;
+    uint64_t* dout = reinterpret_cast<uint64_t*>(du + std::max<size_t>(total, 1));
+    if (!hip_ok(launch_probs(e->amp, e->dim, e->prec, cdf, e->stream), "probs") ||
+        !hip_ok(launch_scan(cdf, cdf, e->dim, scratch, e->stream), "scan"))
+        return;
+    std::vector<double> u(total);
+    auto draw = [&](size_t t0, size_t t1) {
+        for (size_t t = t0; t < t1; ++t) {
+            std::mt19937_64 gen(seeds[t]);
+            std::uniform_real_distribution<double> uni(0.0, 1.0);
+            for (size_t s = off[t]; s < off[t + 1]; ++s) u[s] = uni(gen);
+        }
+    };
+    const size_t nthreads = std::min<size_t>({ntasks, 16, std::max<size_t>(1, total / 65536)});
+    if (nthreads <= 1) {
+        draw(0, ntasks);
+    } else {
+        std::vector<std::thread> pool;
+        for (size_t i = 0; i < nthreads; ++i) pool.emplace_back(draw, ntasks * i / nthreads, ntasks * (i + 1) / nthreads);
+        for (auto& th : pool) th.join();
+    }
+    if (total == 0) {
+        e->sync();
+        return;
+    }
+    if (!hip_ok(hipMemcpyAsync(du, u.data(), total * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
+        !hip_ok(launch_search(cdf, e->dim, du, total, dout, e->stream), "search") ||
+        !hip_ok(hipMemcpyAsync(results_out, dout, total * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
         return;
     e->sync();
 }

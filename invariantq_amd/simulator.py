@@ -108,20 +108,10 @@ class StateVectorSimulator:
             raise ValueError("fusion_width must be between 1 and 5")
         self.fusion_width = fusion_width
 
-    def run(self, num_qubits: int, ops: Sequence, shots: int = 1024, seed: int = 42, *,
-            allow_non_unitary: bool = False, entropy: bool = True) -> ExecutionResult:
-        t0 = time.perf_counter()
-        if isinstance(shots, bool) or not isinstance(shots, (int, np.integer)) or shots <= 0:
-            raise ValueError("shots must be a positive integer")
-        if isinstance(seed, bool) or not isinstance(seed, (int, np.integer)) or seed < 0:
-            raise ValueError("seed must be a non-negative integer")
+    def evolve(self, num_qubits: int, ops: Sequence, *, allow_non_unitary: bool = False):
+        """The op loop of run(): a fresh engine state with every gate and reset
+        applied (fused when fusion_width is set).  Returns (state, fused op count)."""
         n = num_qubits
-        if n == 0:
-            raise ValueError("Circuit has 0 qubits")
-        mem_bytes = checked_power_of_two(n, 16, copies=2)
-        if self.max_memory_bytes is not None and mem_bytes > self.max_memory_bytes:
-            raise MemoryError(f"statevector requires approximately {mem_bytes} bytes including a working copy, "
-                              f"above configured limit {self.max_memory_bytes}")
         sv = NativeStateVector(n, precision=self.precision)
         pending = []  # gates awaiting fusion (fusion_width set)
         fused_ops = 0
@@ -162,6 +152,23 @@ class StateVectorSimulator:
             else:
                 apply(qubits, m)
         drain()
+        return sv, (fused_ops if self.fusion_width else None)
+
+    def run(self, num_qubits: int, ops: Sequence, shots: int = 1024, seed: int = 42, *,
+            allow_non_unitary: bool = False, entropy: bool = True) -> ExecutionResult:
+        t0 = time.perf_counter()
+        if isinstance(shots, bool) or not isinstance(shots, (int, np.integer)) or shots <= 0:
+            raise ValueError("shots must be a positive integer")
+        if isinstance(seed, bool) or not isinstance(seed, (int, np.integer)) or seed < 0:
+            raise ValueError("seed must be a non-negative integer")
+        n = num_qubits
+        if n == 0:
+            raise ValueError("Circuit has 0 qubits")
+        mem_bytes = checked_power_of_two(n, 16, copies=2)
+        if self.max_memory_bytes is not None and mem_bytes > self.max_memory_bytes:
+            raise MemoryError(f"statevector requires approximately {mem_bytes} bytes including a working copy, "
+                              f"above configured limit {self.max_memory_bytes}")
+        sv, fused_ops = self.evolve(n, ops, allow_non_unitary=allow_non_unitary)
         amps = sv.get_amplitudes()
         probs_vec = sv.get_probabilities()
         ent = 0.0
@@ -181,7 +188,7 @@ class StateVectorSimulator:
             metadata={"num_qubits": n, "dimension": 1 << n, "allow_non_unitary": allow_non_unitary,
                       "state_norm": float(np.sum(probs_vec)), "core": native_core_status(),
                       "precision": self.precision, "fusion_width": self.fusion_width,
-                      "fused_ops": fused_ops if self.fusion_width else None,
+                      "fused_ops": fused_ops,
                       "provenance": {"requested_backend": self.name, "selected_backend": "statevector",
                                      "implementation": "dense_statevector", "native": HAS_NATIVE_CORE,
                                      "exact": not allow_non_unitary,
