@@ -254,6 +254,17 @@ def circuit_workload(binding, torch, steps: int, warmup: int, cpu: bool):
     out["tasks"] = {"tasks": len(plan), "shots": shots, "one_call_ms": round(t_one * 1e3, 3),
                     "resimulate_per_task_ms": round(t_re * 1e3, 3), "speedup": round(t_re / t_one, 1),
                     "indices_equal": bool(all(np.array_equal(a, b) for a, b in zip(per, resim)))}
+    # entanglement entropy: the half cut StateVectorSimulator.run computes (k = 10),
+    # and k = 8, which the CPU oracle finishes in seconds
+    sv.synchronize()
+    t0 = time.perf_counter()
+    h_half = sv.von_neumann_entropy(list(range(n // 2)))
+    t_half = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    h8 = sv.von_neumann_entropy(list(range(8)))
+    t8 = time.perf_counter() - t0
+    out["entropy"] = {"half_cut_qubits": n // 2, "half_cut_bits": h_half, "half_cut_ms": round(t_half * 1e3, 3),
+                      "k8_bits": h8, "k8_ms": round(t8 * 1e3, 3)}
     if cpu:
         from oracle import oracle as orc
         core = orc.oracle()
@@ -267,6 +278,10 @@ def circuit_workload(binding, torch, steps: int, warmup: int, cpu: bool):
         t2 = time.perf_counter()
         a, b = sv.get_amplitudes(), ref.get_amplitudes()
         out["max_abs_amp_err_vs_oracle"] = float(np.max(np.abs(a - b)))
+        te = time.perf_counter()
+        rh8 = ref.von_neumann_entropy(list(range(8)))
+        out["entropy"]["k8_cpu_oracle_ms"] = round((time.perf_counter() - te) * 1e3, 1)
+        out["entropy"]["k8_abs_diff_vs_oracle"] = abs(rh8 - h8)
         out["samples_bit_exact_vs_oracle"] = bool(np.array_equal(samples, rs))
         out["cpu_baseline"] = {"value": round(len(ops) / (t1 - t0), 1), "unit": "gate-applications/s",
                                "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",

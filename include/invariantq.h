@@ -142,6 +142,11 @@ void   iq_statevector_apply_fused_sequence(void* sv, size_t count, const size_t*
  * iq_statevector_sample(sv, shots[t], seeds[t], ...); results concatenated in task order. */
 void   iq_statevector_sample_tasks(void* sv, size_t ntasks, const uint64_t* seeds, const uint64_t* shots,
                                    uint64_t* results_out);
+/* Reduced density matrix of `qubits` (k = 1..12; bit b of a row index = qubits[b]), 2^k x 2^k
+ * row-major (the reference's StateVector::partial_trace, core/src/statevector.cpp:293-330, a
+ * class method with no C export). */
+void   iq_statevector_reduced_density_matrix(void* sv, const size_t* qubits, size_t k, double* re_out,
+                                             double* im_out);
 void   iq_statevector_synchronize(void* sv);
 void*  iq_statevector_device_ptr(void* sv);  /* device address of the amplitudes (re, im interleaved) */
 void*  iq_statevector_stream(void* sv);      /* hipStream_t the handle's work is ordered on */

@@ -139,6 +139,7 @@ SIGNATURES = {
     "iq_statevector_expval_pauli_sum": ([_vp, _sz, _d, ctypes.c_char_p, _s, _s, _d, _d, _d], None),
     "iq_statevector_apply_fused_sequence": ([_vp, _sz, _s, _s, _d, _d], None),
     "iq_statevector_sample_tasks": ([_vp, _sz, _u64, _u64, _u64], None),
+    "iq_statevector_reduced_density_matrix": ([_vp, _s, _sz, _d, _d], None),
     "iq_statevector_synchronize": ([_vp], None),
     "iq_statevector_device_ptr": ([_vp], _vp),
     "iq_statevector_stream": ([_vp], _vp),
@@ -349,6 +350,15 @@ class NativeStateVector:
                                                 out.ctypes.data_as(_u64))
         check_error()
         return np.split(out, np.cumsum(sh)[:-1].astype(np.int64))
+
+    def reduced_density_matrix(self, qubits) -> np.ndarray:
+        """Partial trace onto `qubits` (bit b of a row index = qubits[b])."""
+        q = np.ascontiguousarray([int(x) for x in qubits], dtype=np.uint64)
+        d = 1 << q.size
+        re, im = np.empty(d * d), np.empty(d * d)
+        _NATIVE_LIB.iq_statevector_reduced_density_matrix(self._ptr, q.ctypes.data_as(_s), q.size, _dp(re), _dp(im))
+        check_error()
+        return (re + 1j * im).reshape(d, d)
 
     def sample_uniforms(self, uniforms) -> np.ndarray:
         u = np.ascontiguousarray(uniforms, dtype=np.float64)

@@ -416,56 +416,6 @@ bool parse_pauli(const char* ops, const size_t* qubits, size_t len, int n, Pauli
     return true;
 }
 
-// Largest-pivot complex Jacobi with an iteration cap and an off-diagonal
-// threshold: the eigenvalue method of reference core/src/statevector.cpp:333-393
-// and core/src/density_matrix.cpp:322-370, kept so entropies agree with it.
-std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> A, size_t d, int max_iter, double tol) {
-    using cplx = std::complex<double>;
-    auto at = [&](size_t i, size_t j) -> cplx& { return A[i * d + j]; };
-    for (int it = 0; it < max_iter; ++it) {
-        double best = 0.0;
-        size_t p = 0, q = 1;
-        for (size_t i = 0; i < d; ++i)
-            for (size_t j = i + 1; j < d; ++j) {
-                const double v = std::abs(at(i, j));
-                if (v > best) {
-                    best = v;
-                    p = i;
-                    q = j;
-                }
-            }
-        if (best < tol) break;
-        const double app = at(p, p).real(), aqq = at(q, q).real();
-        const cplx apq = at(p, q);
-        const double th = 0.5 * std::atan2(2.0 * std::abs(apq), app - aqq);
-        const cplx ph = apq / std::abs(apq);
-        const double c = std::cos(th);
-        const cplx sn = std::sin(th) * ph;
-        for (size_t i = 0; i < d; ++i) {
-            if (i == p || i == q) continue;
-            const cplx aip = at(i, p), aiq = at(i, q);
-            at(i, p) = c * aip + std::conj(sn) * aiq;
-            at(p, i) = std::conj(at(i, p));
-            at(i, q) = -sn * aip + c * aiq;
-            at(q, i) = std::conj(at(i, q));
-        }
-        const double cross = 2.0 * (c * std::conj(sn) * apq).real();
-        at(p, p) = c * c * app + std::norm(sn) * aqq + cross;
-        at(q, q) = std::norm(sn) * app + c * c * aqq - cross;
-        at(p, q) = 0.0;
-        at(q, p) = 0.0;
-    }
-    std::vector<double> ev(d);
-    for (size_t i = 0; i < d; ++i) ev[i] = at(i, i).real();
-    return ev;
-}
-
-double entropy_bits(const std::vector<double>& ev) {
-    double h = 0.0;
-    for (double l : ev)
-        if (l > 1e-14) h -= l * std::log2(l);
-    return std::max(0.0, h);
-}
 
 }  // namespace iqg
 
@@ -856,37 +806,63 @@ void iq_statevector_get_cdf(void* sv, double* cdf_out) {
 
 // reference core/src/statevector.cpp:293-330 (partial trace, LSB-first local
 // index over `qubits`) and :395-410 (entropy from capped Jacobi eigenvalues).
-double iq_statevector_entropy(void* sv, const size_t* qubits, size_t num_subsys_qubits) {
-    if (!sv || !qubits) return 0.0;
-    Engine* e = sv_of(sv);
-    DeviceGuard g(e->dev);
-    const size_t k = num_subsys_qubits;
-    if (k == 0 || k == (size_t)e->n) return 0.0;
-    if (k > 12) {
-        set_error(kErrUnsupported, "entropy of more than 12 subsystem qubits is not supported");
-        return 0.0;
+// Reduced density matrix of `qubits` (bit b of a row index = qubits[b]) into the
+// work buffer: the partial trace of reference core/src/statevector.cpp:293-330,
+// on the matrix cores for 4 or more kept and 2 or more traced qubits.
+static bool reduced_matrix(Engine* e, const size_t* qubits, size_t k, std::vector<std::complex<double>>& red) {
+    if (k < 1 || k > 12 || k > (size_t)e->n) {
+        set_error(k > 12 ? kErrUnsupported : kErrInvalidArgument,
+                  "reduced matrices of 1..12 qubits (at most the state's) are supported");
+        return false;
     }
     uint32_t q[16];
     to_u32(qubits, k, q);
     for (size_t i = 0; i < k; ++i) {
         if ((int)q[i] >= e->n) {
             set_error(kErrInvalidArgument, "subsystem qubit out of range");
-            return 0.0;
+            return false;
         }
         for (size_t j = 0; j < i; ++j)
             if (q[i] == q[j]) {
                 set_error(kErrInvalidArgument, "repeated subsystem qubit");
-                return 0.0;
+                return false;
             }
     }
     const size_t kd = (size_t)1 << k;
-    if (!e->ensure_work(2 * kd * kd)) return 0.0;
-    std::vector<std::complex<double>> red(kd * kd);
-    if (!hip_ok(launch_partial_trace(e->amp, e->n, e->prec, q, (int)k, reinterpret_cast<double2*>(e->work), e->stream),
-                "partial trace") ||
-        !hip_ok(hipMemcpyAsync(red.data(), e->work, kd * kd * 16, hipMemcpyDeviceToHost, e->stream), "D2H") || !e->sync())
-        return 0.0;
-    return entropy_bits(jacobi_eigenvalues(std::move(red), kd, 100, 1e-12));
+    // reduced matrix, then (matrix-core path) the state reordered to (environment, kept)
+    if (!e->ensure_work(2 * kd * kd + 2 * e->dim)) return false;
+    red.resize(kd * kd);
+    double2* out = reinterpret_cast<double2*>(e->work);
+    return hip_ok(launch_partial_trace(e->amp, e->n, e->prec, q, (int)k, out, out + kd * kd, e->stream),
+                  "partial trace") &&
+           hip_ok(hipMemcpyAsync(red.data(), e->work, kd * kd * 16, hipMemcpyDeviceToHost, e->stream), "D2H") &&
+           e->sync();
+}
+
+double iq_statevector_entropy(void* sv, const size_t* qubits, size_t num_subsys_qubits) {
+    if (!sv || !qubits) return 0.0;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    const size_t k = num_subsys_qubits;
+    if (k == 0 || k == (size_t)e->n) return 0.0;
+    std::vector<std::complex<double>> red;
+    if (!reduced_matrix(e, qubits, k, red)) return 0.0;
+    return entropy_bits(jacobi_eigenvalues(std::move(red), (size_t)1 << k, 100, 1e-12));
+}
+
+// Reduced density matrix export (no C export in the reference: partial_trace
+// is a class method, core/src/statevector.cpp:293-330), 2^k x 2^k row-major.
+void iq_statevector_reduced_density_matrix(void* sv, const size_t* qubits, size_t k, double* re_out,
+                                           double* im_out) {
+    if (!sv || !qubits || !re_out || !im_out) return;
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    std::vector<std::complex<double>> red;
+    if (!reduced_matrix(e, qubits, k, red)) return;
+    for (size_t i = 0; i < red.size(); ++i) {
+        re_out[i] = red[i].real();
+        im_out[i] = red[i].imag();
+    }
 }
 
 void iq_statevector_expval_pauli(void* sv, const char* pauli_str, const size_t* qubits, size_t len, double* val_real,

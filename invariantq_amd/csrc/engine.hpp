@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "jacobi.hpp"
 #include "kernels.hpp"
 
 namespace iqg {
@@ -33,8 +34,6 @@ struct PauliTerm {
 };
 
 bool parse_pauli(const char* ops, const size_t* qubits, size_t len, int n, PauliTerm& t);
-std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> A, size_t d, int max_iter, double tol);
-double entropy_bits(const std::vector<double>& ev);
 
 struct Engine {
     static constexpr int kSlots = 16;

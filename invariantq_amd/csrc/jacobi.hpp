@@ -1,0 +1,114 @@
+// Eigenvalues of a Hermitian matrix by the capped largest-pivot complex Jacobi
+// method of reference core/src/statevector.cpp:333-393 (and
+// core/src/density_matrix.cpp:322-370), kept so entropies agree with it:
+// each rotation zeroes the off-diagonal entry of largest modulus (the first
+// in row-major order among equals), at most `max_iter` rotations, stopping
+// once that modulus is below `tol`.
+//
+// The reference rescans the whole upper triangle for every pivot, O(d^2) per
+// rotation (seconds at d = 4096).  Here each row keeps the largest modulus
+// right of its diagonal and the column of its first occurrence.  A rotation at
+// (p, q) changes rows p and q and the entries (i, p), (i, q) of the rows
+// above them, so only those are updated, and a row is rescanned only when its
+// recorded maximum sat in a changed column.  The pivots, and therefore the
+// arithmetic and the result, are those of the full scan, bit for bit.
+#pragma once
+
+#include <cmath>
+#include <complex>
+#include <cstddef>
+#include <vector>
+
+namespace iqg {
+
+inline std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> A, size_t d, int max_iter,
+                                              double tol) {
+    using cplx = std::complex<double>;
+    auto at = [&](size_t i, size_t j) -> cplx& { return A[i * d + j]; };
+    std::vector<double> rmax(d, 0.0);  // largest |A(i, j)|, j > i
+    std::vector<size_t> rarg(d, 0);    // first j attaining it
+    auto rescan = [&](size_t i) {
+        double best = 0.0;
+        size_t arg = i + 1;
+        for (size_t j = i + 1; j < d; ++j) {
+            const double v = std::abs(at(i, j));
+            if (v > best) {
+                best = v;
+                arg = j;
+            }
+        }
+        rmax[i] = best;
+        rarg[i] = arg;
+    };
+    // row i's entries in columns j1 (and j2) changed: rescan when its recorded
+    // maximum sat in one of them, else offer the new values (ties go to the
+    // smaller column, as in a row-major scan)
+    auto update = [&](size_t i, size_t j1, size_t j2, bool two) {
+        if (rarg[i] == j1 || (two && rarg[i] == j2)) {
+            rescan(i);
+            return;
+        }
+        auto offer = [&](size_t j) {
+            const double v = std::abs(at(i, j));
+            if (v > rmax[i] || (v == rmax[i] && v > 0.0 && j < rarg[i])) {
+                rmax[i] = v;
+                rarg[i] = j;
+            }
+        };
+        offer(j1);
+        if (two) offer(j2);
+    };
+    for (size_t i = 0; i + 1 < d; ++i) rescan(i);
+    for (int it = 0; it < max_iter; ++it) {
+        double best = 0.0;
+        size_t p = 0, q = 1;
+        for (size_t i = 0; i + 1 < d; ++i)
+            if (rmax[i] > best) {
+                best = rmax[i];
+                p = i;
+                q = rarg[i];
+            }
+        if (best < tol) break;
+        const double app = at(p, p).real(), aqq = at(q, q).real();
+        const cplx apq = at(p, q);
+        const double th = 0.5 * std::atan2(2.0 * std::abs(apq), app - aqq);
+        const cplx ph = apq / std::abs(apq);
+        const double c = std::cos(th);
+        const cplx sn = std::sin(th) * ph;
+        for (size_t i = 0; i < d; ++i) {
+            if (i == p || i == q) continue;
+            const cplx aip = at(i, p), aiq = at(i, q);
+            at(i, p) = c * aip + std::conj(sn) * aiq;
+            at(p, i) = std::conj(at(i, p));
+            at(i, q) = -sn * aip + c * aiq;
+            at(q, i) = std::conj(at(i, q));
+        }
+        const double cross = 2.0 * (c * std::conj(sn) * apq).real();
+        const double npp = c * c * app + std::norm(sn) * aqq + cross;
+        const double nqq = std::norm(sn) * app + c * c * aqq - cross;
+        at(p, p) = npp;
+        at(q, q) = nqq;
+        at(p, q) = 0.0;
+        at(q, p) = 0.0;
+        // rows above p and q: their entries in columns p and q changed
+        for (size_t i = 0; i < q; ++i) {
+            if (i == p) continue;
+            if (i < p) update(i, p, q, true);
+            else update(i, q, q, false);
+        }
+        rescan(p);
+        if (q + 1 < d) rescan(q);
+    }
+    std::vector<double> ev(d);
+    for (size_t i = 0; i < d; ++i) ev[i] = at(i, i).real();
+    return ev;
+}
+
+inline double entropy_bits(const std::vector<double>& ev) {
+    double h = 0.0;
+    for (double l : ev)
+        if (l > 1e-14) h -= l * std::log2(l);
+    return h > 0.0 ? h : 0.0;
+}
+
+}  // namespace iqg

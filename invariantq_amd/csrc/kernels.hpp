@@ -72,6 +72,11 @@ hipError_t launch_scan(const double* probs, double* cdf, uint64_t count, double*
 hipError_t launch_search(const double* cdf, uint64_t count, const double* u, uint64_t shots, uint64_t* out, hipStream_t st);
 
 // Reduced density matrix of `keep` (LSB-first local index), complex128 out [kd * kd].
-hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t* keep, int k, double2* out, hipStream_t st);
+// Reduced density matrix of the kept qubits (bit b of a row index = qubit
+// keep[b]), d x d row-major.  With k >= 4 kept and >= 2 traced qubits it runs
+// on the matrix cores and needs `scratch` (2^n double2) unless keep is 0..k-1
+// in order on a complex128 state; otherwise one thread per element.
+hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t* keep, int k, double2* out,
+                                double2* scratch, hipStream_t st);
 
 }  // namespace iqg

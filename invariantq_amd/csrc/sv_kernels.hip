@@ -344,6 +344,55 @@ __global__ void __launch_bounds__(256) k_partial_trace(const cv<R>* __restrict__
     out[id] = make_double2(sr, si);
 }
 
+// Amplitudes in (environment, kept) order, FP64: phi[e * 2^k + r] = psi[i]
+// where bit b of r is qubit keep[b] of i and the bits of e are the traced
+// qubits of i in ascending order (the index split of
+// core/src/statevector.cpp:306-323).
+template <typename R>
+__global__ void __launch_bounds__(256) k_gather_keep(const cv<R>* __restrict__ psi, KeepList keep, int k,
+                                                     uint64_t envmask, uint64_t total, double2* __restrict__ phi) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += stride) {
+        const uint64_t r = o & ((1ULL << k) - 1), e = o >> k;
+        uint64_t idx = deposit_bits(e, envmask);
+        for (int b = 0; b < k; ++b) idx |= ((r >> b) & 1ULL) << keep.q[b];
+        const cv<R> a = psi[idx];
+        phi[o] = make_double2((double)a.x, (double)a.y);
+    }
+}
+
+// rho_A = X^T conj(X) for X = phi viewed as m x d (row e, column r), on the
+// FP64 matrix cores (v_mfma_f64_16x16x4): one wave per 16x16 tile of the upper
+// triangle (tile row <= tile column), four e per step:
+//   Re += Xr^T Xr + Xi^T Xi,  Im += Xi^T Xr - Xr^T Xi.
+// Operand lanes: A holds row (lane % 16), k (lane / 16); B holds k (lane / 16),
+// column (lane % 16); result value jj of a lane is row lane/16 + 4 jj, column
+// lane % 16.  Off-diagonal tiles also write their conjugate transpose.
+__global__ void __launch_bounds__(64) k_herk_mfma(const double2* __restrict__ phi, uint64_t m, uint32_t d,
+                                                  double2* __restrict__ out) {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    const uint32_t tr = blockIdx.y, tc = blockIdx.x;
+    if (tc < tr) return;
+    const uint32_t lane = threadIdx.x & 63u, kk = lane >> 4, j = lane & 15u;
+    const uint64_t r0 = (uint64_t)tr * 16 + j, c0 = (uint64_t)tc * 16 + j;
+    v4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
+#pragma unroll 4
+    for (uint64_t e0 = 0; e0 < m; e0 += 4) {
+        const double2 a = phi[(e0 + kk) * d + r0];
+        const double2 b = phi[(e0 + kk) * d + c0];
+        re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.x, b.y, im, 0, 0, 0);
+    }
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        const uint64_t row = (uint64_t)tr * 16 + kk + 4 * jj, col = (uint64_t)tc * 16 + j;
+        out[row * d + col] = make_double2(re[jj], im[jj]);
+        if (tr != tc) out[col * d + row] = make_double2(re[jj], -im[jj]);
+    }
+}
+
 // Gate-matrix upload on the compute stream: reads pinned (coherent, uncached)
 // host memory over PCIe and writes device memory, so the gate kernel that
 // follows needs no copy-engine round trip.  One 16-byte element per thread
@@ -486,16 +535,39 @@ hipError_t launch_search(const double* cdf, uint64_t count, const double* u, uin
     return hipGetLastError();
 }
 
-hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t* keep, int k, double2* out, hipStream_t st) {
-    if (k < 1 || k > 16) return hipErrorInvalidValue;
+hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t* keep, int k, double2* out,
+                                double2* scratch, hipStream_t st) {
+    if (k < 1 || k > 16 || k > n) return hipErrorInvalidValue;
     KeepList kl{};
     uint64_t kmask = 0;
+    bool in_order = true;
     for (int b = 0; b < k; ++b) {
         kl.q[b] = keep[b];
         kmask |= 1ULL << keep[b];
+        in_order &= keep[b] == (uint32_t)b;
     }
     const uint64_t envmask = ((1ULL << n) - 1) & ~kmask;
-    const uint64_t total = (1ULL << k) * (1ULL << k);
+    const uint32_t d = 1u << k;
+    const uint64_t m = 1ULL << (n - k);
+    if (k >= 4 && n - k >= 2 && (scratch || (in_order && prec == kC128))) {
+        // matrix-core path: reorder to (environment, kept) unless already so
+        const double2* phi = (const double2*)psi;
+        if (!(in_order && prec == kC128)) {
+            const uint64_t total = 1ULL << n;
+            const unsigned b = blocks_for(total, 256, 65536);
+            if (prec == kC128)
+                hipLaunchKernelGGL(k_gather_keep<double>, dim3(b), dim3(256), 0, st, (const double2*)psi, kl, k, envmask,
+                                   total, scratch);
+            else
+                hipLaunchKernelGGL(k_gather_keep<float>, dim3(b), dim3(256), 0, st, (const float2*)psi, kl, k, envmask,
+                                   total, scratch);
+            phi = scratch;
+        }
+        const uint32_t nt = d / 16;
+        hipLaunchKernelGGL(k_herk_mfma, dim3(nt, nt), dim3(64), 0, st, phi, m, d, out);
+        return hipGetLastError();
+    }
+    const uint64_t total = (uint64_t)d * d;
     const unsigned b = blocks_for(total, 256, 1ULL << 30);
     if (prec == kC128)
         hipLaunchKernelGGL(k_partial_trace<double>, dim3(b), dim3(256), 0, st, (const double2*)psi, kl, k, envmask, out);

@@ -457,6 +457,17 @@ extern "C" {
 
 int iqo_last_error(void) { return g_last_error; }
 
+// The capped largest-pivot Jacobi above on a caller's Hermitian matrix
+// (d x d row-major, split re/im): the reference's eigenvalue method, exposed
+// so faster restatements can be checked against it bit for bit.
+void iqo_hermitian_eigenvalues(const double* re, const double* im, size_t d, int max_iter, double tol,
+                               double* out) {
+    std::vector<cplx> m(d * d);
+    for (size_t i = 0; i < d * d; ++i) m[i] = cplx(re[i], im[i]);
+    const auto ev = jacobi_eigenvalues(m, d, max_iter, tol);
+    for (size_t i = 0; i < d; ++i) out[i] = ev[i];
+}
+
 void* iq_statevector_create(size_t n) {
     if (n > 40) return nullptr;
     try { return new SV(n); } catch (...) { return nullptr; }
@@ -544,6 +555,16 @@ void iq_statevector_expval_pauli(void* p, const char* ops, const size_t* q, size
     const cplx v = sv_expval_pauli(*static_cast<SV*>(p), ops, q, len);
     *vr = v.real();
     *vi = v.imag();
+}
+// Reduced density matrix of the kept qubits (reference StateVector::partial_trace,
+// core/src/statevector.cpp:293-330, a class method with no C export).
+void iqo_statevector_partial_trace(void* p, const size_t* q, size_t k, double* re, double* im) {
+    if (!p || !q || !re || !im) return;
+    const auto red = sv_partial_trace(*static_cast<SV*>(p), std::vector<size_t>(q, q + k));
+    for (size_t i = 0; i < red.size(); ++i) {
+        re[i] = red[i].real();
+        im[i] = red[i].imag();
+    }
 }
 // Oracle-only helpers (no reference ABI counterpart): load a state, read the norm.
 void iqo_statevector_set_amplitudes(void* p, const double* re, const double* im) {

@@ -112,6 +112,10 @@ class CoreLib:
             L.iqo_mt19937_64_raw.restype = None
             L.iqo_uniforms.argtypes = [ctypes.c_uint64, sz, _dp]
             L.iqo_uniforms.restype = None
+            L.iqo_statevector_partial_trace.argtypes = [vp, _sp, sz, _dp, _dp]
+            L.iqo_statevector_partial_trace.restype = None
+            L.iqo_hermitian_eigenvalues.argtypes = [_dp, _dp, sz, ctypes.c_int, ctypes.c_double, _dp]
+            L.iqo_hermitian_eigenvalues.restype = None
 
     def mt19937_64_raw(self, seed: int, count: int) -> np.ndarray:
         out = np.empty(count, dtype=np.uint64)
@@ -189,6 +193,14 @@ class CoreStateVector:
         out = np.empty(shots, dtype=np.uint64)
         self.L.iq_statevector_sample(self._ptr, shots, seed, out.ctypes.data_as(_u64p))
         return out
+
+    def partial_trace(self, qubits: Sequence[int]) -> np.ndarray:
+        """Reduced density matrix (statevector.cpp:293-330); oracle library only."""
+        q = np.ascontiguousarray(qubits, dtype=np.uint64)
+        d = 1 << q.size
+        re, im = np.empty(d * d), np.empty(d * d)
+        self.L.iqo_statevector_partial_trace(self._ptr, q.ctypes.data_as(_sp), q.size, _dptr(re), _dptr(im))
+        return (re + 1j * im).reshape(d, d)
 
     def von_neumann_entropy(self, qubits: Sequence[int]) -> float:
         q = np.ascontiguousarray(qubits, dtype=np.uint64)
