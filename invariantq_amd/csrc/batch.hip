@@ -28,6 +28,7 @@
 #include <cstring>
 #include <new>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "engine.hpp"
@@ -46,7 +47,7 @@ struct BatchCircuit {
     int64_t out_index;  // energy slot
 };
 
-constexpr int kMaxSlots = 64;  // gradient slots per layer (2 * n <= 40)
+constexpr int kMaxSlots = 96;  // gradient slots per layer: 3 per qubit (n <= 30)
 
 template <typename R>
 struct Rot {
@@ -62,24 +63,21 @@ __device__ __forceinline__ void mul2(const R* u, R& x0r, R& x0i, R& x1r, R& x1i)
     x0r = y0r; x0i = y0i; x1r = y1r; x1i = y1i;
 }
 
-// RZ(phi) RY(theta) and the two daggers, from FP64 angles (24 values).
+// RZ(phi) RY(theta) and its inverse (RZ RY)^dagger, from FP64 angles.
 template <typename R>
-__device__ __forceinline__ void make_rots(double th, double ph, R* fwd, R* rzd, R* ryd) {
+__device__ __forceinline__ void make_rots(double th, double ph, R* fwd, R* inv) {
     const double c = cos(0.5 * th), s = sin(0.5 * th);
     const double cp = cos(0.5 * ph), sp = sin(0.5 * ph);
     // RZ RY = [[e^{-i ph/2} c, -e^{-i ph/2} s], [e^{i ph/2} s, e^{i ph/2} c]]
     const double f[8] = {cp * c, -sp * c, -cp * s, sp * s, cp * s, sp * s, cp * c, sp * c};
     for (int i = 0; i < 8; ++i) fwd[i] = (R)f[i];
-    // RZ^dag = diag(e^{i ph/2}, e^{-i ph/2})
-    const double z[8] = {cp, sp, 0, 0, 0, 0, cp, -sp};
-    for (int i = 0; i < 8; ++i) rzd[i] = (R)z[i];
-    // RY^dag = [[c, s], [-s, c]]
-    const double y[8] = {c, 0, s, 0, -s, 0, c, 0};
-    for (int i = 0; i < 8; ++i) ryd[i] = (R)y[i];
+    // conjugate transpose: entry (r, c) = conj(f(c, r))
+    const double g[8] = {f[0], -f[1], f[4], -f[5], f[2], -f[3], f[6], -f[7]};
+    for (int i = 0; i < 8; ++i) inv[i] = (R)g[i];
 }
 
-// One block per circuit: rots[(param_off / 2 + n*l + q) * 24 + (0..7 | 8..15 | 16..23)]
-// = (RZ RY, RZ^dag, RY^dag) of qubit q in layer l.
+// One block per circuit: rots[(param_off / 2 + n*l + q) * 24 + (0..7 | 8..15)]
+// = (RZ RY, (RZ RY)^dagger) of qubit q in layer l (16..23 unused).
 template <typename R>
 __global__ void k_prepare_rots(const BatchCircuit* __restrict__ circs, const double* __restrict__ params,
                                R* __restrict__ rots) {
@@ -89,7 +87,7 @@ __global__ void k_prepare_rots(const BatchCircuit* __restrict__ circs, const dou
         const int l = k / n, q = k % n;
         const double* P = params + C.param_off + 2 * n * l;
         R* o = rots + (C.param_off / 2 + k) * 24;
-        make_rots<R>(P[q], P[n + q], o, o + 8, o + 16);
+        make_rots<R>(P[q], P[n + q], o, o + 8);
     }
 }
 
@@ -118,7 +116,8 @@ __device__ __forceinline__ bool cz_ring_sign(uint64_t g, int n) {
 
 template <typename R>
 __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restrict__ circs, int ncirc,
-                                                   const R* __restrict__ rots, double* __restrict__ grads,
+                                                   const R* __restrict__ rots, const double* __restrict__ params,
+                                                   double* __restrict__ grads,
                                                    double* __restrict__ energies, cv<R>* __restrict__ scratch,
                                                    uint64_t scratch_elems, int* __restrict__ queue, double hx, int bmax) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -147,74 +146,224 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         const uint64_t ntA = resident ? 1 : (N >> b);
         const int h = n - b;
 
-        // --- tile transfer helpers
-        auto load = [&](int kind, uint64_t t, cv<R>* dst, const cv<R>* src) {
-            for (uint32_t j = tid; j < TS; j += T) dst[j] = src[tile_global(kind, t, j, b, n)];
+        // LDS position of tile element j: the low four index bits XOR the next
+        // four, so sixteen lanes reading sixteen elements of a register group
+        // hit distinct 8-byte slots whichever tile bits the group holds
+        auto P = [](uint32_t j) __attribute__((always_inline)) -> uint32_t { return j ^ ((j >> 4) & 15u); };
+        // qubits per register group: forward sweeps 4 (16 elements per
+        // sub-cube) for complex64, 3 for complex128; reverse sweeps hold psi and
+        // lambda next to the prefetched tile, so one fewer
+        constexpr int GM = sizeof(R) == 4 ? 4 : 3;
+        constexpr int GA = GM;
+        // --- tile transfer helpers: eight elements per thread in flight before
+        // any is written on, so a tile costs about one memory latency, not
+        // one per element
+        // psi (and lambda) tile in one round of loads: up to 16 elements per
+        // thread per array in flight (64 KiB per CU for a 2^13 complex64 tile)
+        auto load2 = [&](int kind, uint64_t t, bool two) __attribute__((always_inline)) {
+            for (uint32_t j0 = tid; j0 < TS; j0 += 16 * T) {
+                cv<R> v[16], w[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {  // every element assigned: the arrays stay in registers
+                    const uint64_t gi = tile_global(kind, t, min(j0 + u * T, TS - 1), b, n);
+                    v[u] = gpsi[gi];
+                    w[u] = two ? glam[gi] : v[u];
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const uint32_t j = j0 + u * T;
+                    if (j < TS) {
+                        tp[P(j)] = v[u];
+                        if (two) tl[P(j)] = w[u];
+                    }
+                }
+            }
         };
-        auto store = [&](int kind, uint64_t t, const cv<R>* src, cv<R>* dst) {
-            for (uint32_t j = tid; j < TS; j += T) dst[tile_global(kind, t, j, b, n)] = src[j];
+        auto load = [&](int kind, uint64_t t, cv<R>* dst, const cv<R>* src) __attribute__((always_inline)) {
+            for (uint32_t j0 = tid; j0 < TS; j0 += 8 * T) {
+                cv<R> v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {  // every v[u] assigned: the array stays in registers
+                    const uint32_t j = min(j0 + u * T, TS - 1);
+                    v[u] = src[tile_global(kind, t, j, b, n)];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const uint32_t j = j0 + u * T;
+                    if (j < TS) dst[P(j)] = v[u];
+                }
+            }
         };
-        // rotation of tile bit tb by a 2x2 matrix on one tile
-        auto rot = [&](cv<R>* tile, int tb, const R* u) {
-            const uint32_t bit = 1u << tb, lo = bit - 1;
-            for (uint32_t p = tid; p < (TS >> 1); p += T) {
-                const uint32_t i0 = ((p & ~lo) << 1) | (p & lo), i1 = i0 | bit;
-                cv<R> a = tile[i0], c = tile[i1];
-                mul2<R>(u, a.x, a.y, c.x, c.y);
-                tile[i0] = a;
-                tile[i1] = c;
+        auto store = [&](int kind, uint64_t t, const cv<R>* src, cv<R>* dst) __attribute__((always_inline)) {
+            for (uint32_t j0 = tid; j0 < TS; j0 += 8 * T) {
+                cv<R> v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const uint32_t j = min(j0 + u * T, TS - 1);
+                    v[u] = src[P(j)];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const uint32_t j = j0 + u * T;
+                    if (j < TS) dst[tile_global(kind, t, j, b, n)] = v[u];
+                }
+            }
+        };
+        // Rotations of tile bits [t0, t0 + G) (qubit = tile bit + qoff): one LDS
+        // round trip per group, not per qubit.  Each thread takes sub-cubes of
+        // 2^G elements that differ only in those bits into registers and
+        // applies the G rotations there.  G is a compile-time constant so the
+        // sub-cube stays in registers.
+        auto rot_g = [&](auto Gc, cv<R>* tile, int t0, int qoff, int l) __attribute__((always_inline)) {
+            constexpr int G = decltype(Gc)::value;
+            constexpr uint32_t D = 1u << G;
+            const uint32_t cubes = TS >> G, lo = (1u << t0) - 1;
+            for (uint32_t s = tid; s < cubes; s += T) {
+                const uint32_t base = ((s & ~lo) << G) | (s & lo);
+                cv<R> x[D];
+#pragma unroll
+                for (uint32_t m = 0; m < D; ++m) x[m] = tile[P(base | (m << t0))];
+#pragma unroll
+                for (int jb = 0; jb < G; ++jb) {
+                    const R* u = RT + (n * l + (t0 + jb + qoff)) * 24;
+#pragma unroll
+                    for (uint32_t m = 0; m < D; ++m)
+                        if (!(m & (1u << jb))) mul2<R>(u, x[m].x, x[m].y, x[m | (1u << jb)].x, x[m | (1u << jb)].y);
+                }
+#pragma unroll
+                for (uint32_t m = 0; m < D; ++m) tile[P(base | (m << t0))] = x[m];
             }
             __syncthreads();
         };
-        // reverse step on tile bit tb: grad phi (Z) then un-RZ, grad theta (Y) then un-RY
-        auto adj = [&](int tb, const R* rzd, const R* ryd, int slot) {
-            const uint32_t bit = 1u << tb, lo = bit - 1;
-            double gz = 0.0, gy = 0.0;
-            for (uint32_t p = tid; p < (TS >> 1); p += T) {
-                const uint32_t i0 = ((p & ~lo) << 1) | (p & lo), i1 = i0 | bit;
-                cv<R> a0 = tp[i0], a1 = tp[i1], b0 = tl[i0], b1 = tl[i1];
-                gz += ((double)b0.x * a0.y - (double)b0.y * a0.x) - ((double)b1.x * a1.y - (double)b1.y * a1.x);
-                mul2<R>(rzd, a0.x, a0.y, a1.x, a1.y);
-                mul2<R>(rzd, b0.x, b0.y, b1.x, b1.y);
-                gy += (-(double)b0.x * a1.x - (double)b0.y * a1.y) + ((double)b1.x * a0.x + (double)b1.y * a0.y);
-                mul2<R>(ryd, a0.x, a0.y, a1.x, a1.y);
-                mul2<R>(ryd, b0.x, b0.y, b1.x, b1.y);
-                tp[i0] = a0; tp[i1] = a1; tl[i0] = b0; tl[i1] = b1;
+        // dispatch on the group width; callers pass 1 <= g <= GM
+        auto rot_group = [&](cv<R>* tile, int t0, int g, int qoff, int l) __attribute__((always_inline)) {
+            static_assert(GM >= 3 && GM <= 4, "group width");
+            switch (g) {
+                case 1: rot_g(std::integral_constant<int, 1>{}, tile, t0, qoff, l); break;
+                case 2: rot_g(std::integral_constant<int, 2>{}, tile, t0, qoff, l); break;
+                case 3: rot_g(std::integral_constant<int, 3>{}, tile, t0, qoff, l); break;
+                case 4: rot_g(std::integral_constant<int, GM>{}, tile, t0, qoff, l); break;
+                default: break;
             }
-            gz = wave_sum(gz);
-            gy = wave_sum(gy);
-            if (lane == 0) {
-                wred[wave * kMaxSlots + 2 * slot + 1] += gz;  // RZ parameter slot n + q
-                wred[wave * kMaxSlots + 2 * slot] += gy;      // RY parameter slot q
+        };
+        // Reverse steps of tile bits [t0, t0 + G) on psi and lambda.  For qubit
+        // q with U = RZ(phi) RY(theta), both gradients are read before undoing U:
+        //   dE/dphi   = Im<lambda|Z psi>  = sum Im(l0* p0) - Im(l1* p1)
+        //   dE/dtheta = Im<lambda|RZ Y RZ^dag psi>,  RZ Y RZ^dag = [[0, -i e^{-i phi}], [i e^{i phi}, 0]]
+        //             = cos(phi) A - sin(phi) B,  A = sum Re(l1* p0) - Re(l0* p1),
+        //                                         B = sum Im(l1* p0) + Im(l0* p1)
+        // then one (RZ RY)^dagger on psi and lambda.  Products are summed in the
+        // state's precision over a sub-cube, then in FP64; cos and sin are applied
+        // in FP64 when the layer's gradients are written.  Rotations on different
+        // qubits commute, so the qubits of a group may go in any order.
+        auto adj_g = [&](auto Gc, int t0, int qoff, int l) __attribute__((always_inline)) {
+            constexpr int G = decltype(Gc)::value;
+            constexpr uint32_t D = 1u << G;
+            const uint32_t cubes = TS >> G, lo = (1u << t0) - 1;
+            double az[G], aa[G], ab[G];
+#pragma unroll
+            for (int jb = 0; jb < G; ++jb) az[jb] = aa[jb] = ab[jb] = 0.0;
+            for (uint32_t s = tid; s < cubes; s += T) {
+                const uint32_t base = ((s & ~lo) << G) | (s & lo);
+                cv<R> x[D], y[D];
+#pragma unroll
+                for (uint32_t m = 0; m < D; ++m) {
+                    x[m] = tp[P(base | (m << t0))];
+                    y[m] = tl[P(base | (m << t0))];
+                }
+#pragma unroll
+                for (int jb = G - 1; jb >= 0; --jb) {
+                    const R* inv = RT + (n * l + (t0 + jb + qoff)) * 24 + 8;
+                    const uint32_t bit = 1u << jb;
+                    R pz = 0, pa = 0, pb = 0;
+#pragma unroll
+                    for (uint32_t m = 0; m < D; ++m) {
+                        if (m & bit) continue;
+                        cv<R>&a0 = x[m], &a1 = x[m | bit], &b0 = y[m], &b1 = y[m | bit];
+                        pz += (b0.x * a0.y - b0.y * a0.x) - (b1.x * a1.y - b1.y * a1.x);
+                        pa += (b1.x * a0.x + b1.y * a0.y) - (b0.x * a1.x + b0.y * a1.y);
+                        pb += (b1.x * a0.y - b1.y * a0.x) + (b0.x * a1.y - b0.y * a1.x);
+                        mul2<R>(inv, a0.x, a0.y, a1.x, a1.y);
+                        mul2<R>(inv, b0.x, b0.y, b1.x, b1.y);
+                    }
+                    az[jb] += (double)pz;
+                    aa[jb] += (double)pa;
+                    ab[jb] += (double)pb;
+                }
+#pragma unroll
+                for (uint32_t m = 0; m < D; ++m) {
+                    tp[P(base | (m << t0))] = x[m];
+                    tl[P(base | (m << t0))] = y[m];
+                }
+            }
+#pragma unroll
+            for (int jb = 0; jb < G; ++jb) {
+                const double z = wave_sum(az[jb]), va = wave_sum(aa[jb]), vb = wave_sum(ab[jb]);
+                if (lane == 0) {
+                    double* w = wred + wave * kMaxSlots + 3 * (t0 + jb + qoff);
+                    w[0] += z;
+                    w[1] += va;
+                    w[2] += vb;
+                }
             }
             __syncthreads();
         };
-        auto cz = [&](int kind, uint64_t t, bool both) {
+        // dispatch on the group width; callers pass 1 <= g <= GA
+        auto adj_group = [&](int t0, int g, int qoff, int l) __attribute__((always_inline)) {
+            static_assert(GA >= 3 && GA <= 4, "group width");
+            switch (g) {
+                case 1: adj_g(std::integral_constant<int, 1>{}, t0, qoff, l); break;
+                case 2: adj_g(std::integral_constant<int, 2>{}, t0, qoff, l); break;
+                case 3: adj_g(std::integral_constant<int, 3>{}, t0, qoff, l); break;
+                case 4: adj_g(std::integral_constant<int, GA>{}, t0, qoff, l); break;
+                default: break;
+            }
+        };
+        // One pass over the tiles of a decomposition (kind 1: the b low qubits,
+        // 2: the high qubits): each tile goes global -> LDS, through `body`,
+        // and back.  (Prefetching the next tile into registers across `body`
+        // needs 64 more VGPRs per lane and spilled at 512 threads.)
+        auto tile_pass = [&](int kind, bool two, auto&& body) __attribute__((always_inline)) {
+            for (uint64_t t = 0; t < ntA; ++t) {
+                load2(kind, t, two);
+                __syncthreads();
+                body(t);
+                store(kind, t, tp, gpsi);
+                if (two) store(kind, t, tl, glam);
+                __syncthreads();
+            }
+        };
+        auto cz = [&](int kind, uint64_t t, bool both) __attribute__((always_inline)) {
             for (uint32_t j = tid; j < TS; j += T) {
                 if (cz_ring_sign(tile_global(kind, t, j, b, n), n)) {
-                    cv<R> v = tp[j];
+                    cv<R> v = tp[P(j)];
                     v.x = -v.x; v.y = -v.y;
-                    tp[j] = v;
+                    tp[P(j)] = v;
                     if (both) {
-                        cv<R> w = tl[j];
+                        cv<R> w = tl[P(j)];
                         w.x = -w.x; w.y = -w.y;
-                        tl[j] = w;
+                        tl[P(j)] = w;
                     }
                 }
             }
             __syncthreads();
         };
-        auto clear_slots = [&]() {
+        auto clear_slots = [&]() __attribute__((always_inline)) {
             for (int i = tid; i < nw * kMaxSlots; i += T) wred[i] = 0.0;
             __syncthreads();
         };
-        // layer gradients: slot 2q = theta_q, 2q+1 = phi_q
-        auto flush_slots = [&](int l) {
-            for (int s = tid; s < 2 * n; s += T) {
-                double v = 0.0;
-                for (int w = 0; w < nw; ++w) v += wred[w * kMaxSlots + s];
-                const int q = s >> 1;
-                G[2 * n * l + ((s & 1) ? n + q : q)] = v;
+        // layer gradients from the slots (3q: dE/dphi_q, 3q+1 / 3q+2: A, B)
+        auto flush_slots = [&](int l) __attribute__((always_inline)) {
+            for (int q = tid; q < n; q += T) {
+                double z = 0.0, va = 0.0, vb = 0.0;
+                for (int w = 0; w < nw; ++w) {
+                    z += wred[w * kMaxSlots + 3 * q];
+                    va += wred[w * kMaxSlots + 3 * q + 1];
+                    vb += wred[w * kMaxSlots + 3 * q + 2];
+                }
+                const double ph = params[C.param_off + 2 * n * l + n + q];
+                G[2 * n * l + n + q] = z;
+                G[2 * n * l + q] = cos(ph) * va - sin(ph) * vb;
             }
             __syncthreads();
         };
@@ -225,7 +374,7 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                 cv<R> v;
                 v.x = (j == 0) ? (R)1 : (R)0;
                 v.y = 0;
-                tp[j] = v;
+                tp[P(j)] = v;
             }
             __syncthreads();
         } else {
@@ -239,46 +388,38 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         }
         for (int l = 0; l < L; ++l) {
             if (resident) {
-                for (int q = 0; q < n; ++q) rot(tp, q, RT + (n * l + q) * 24);
+                for (int t0 = 0; t0 < n; t0 += GM) rot_group(tp, t0, min(GM, n - t0), 0, l);
                 cz(0, 0, false);
             } else {
-                for (uint64_t t = 0; t < ntA; ++t) {  // pass A: qubits 0..b-1
-                    load(1, t, tp, gpsi);
-                    __syncthreads();
-                    for (int q = 0; q < b; ++q) rot(tp, q, RT + (n * l + q) * 24);
-                    store(1, t, tp, gpsi);
-                    __syncthreads();
-                }
-                for (uint64_t t = 0; t < ntA; ++t) {  // pass B: qubits b..n-1, then the CZ ring
-                    load(2, t, tp, gpsi);
-                    __syncthreads();
-                    for (int q = b; q < n; ++q) rot(tp, q - b + (b - h), RT + (n * l + q) * 24);
+                tile_pass(1, false, [&](uint64_t) {  // pass A: qubits 0..b-1
+                    for (int t0 = 0; t0 < b; t0 += GM) rot_group(tp, t0, min(GM, b - t0), 0, l);
+                });
+                tile_pass(2, false, [&](uint64_t t) {  // pass B: qubits b..n-1, then the CZ ring
+                    for (int t0 = b - h; t0 < b; t0 += GM) rot_group(tp, t0, min(GM, b - t0), h, l);
                     cz(2, t, false);
-                    store(2, t, tp, gpsi);
-                    __syncthreads();
-                }
+                });
             }
         }
 
         // --- lambda = H psi, energy = Re <psi|lambda>
         double e = 0.0;
-        auto hdiag = [&](uint64_t g) {
+        auto hdiag = [&](uint64_t g) __attribute__((always_inline)) {
             const uint64_t diff = (g ^ (g >> 1)) & ((1ULL << (n - 1)) - 1);
             return (double)(n - 1) - 2.0 * (double)__popcll(diff);
         };
         if (resident) {
             for (uint32_t j = tid; j < TS; j += T) {
-                const cv<R> v = tp[j];
+                const cv<R> v = tp[P(j)];
                 double sr = hdiag(j) * (double)v.x, si = hdiag(j) * (double)v.y;
                 for (int q = 0; q < n; ++q) {
-                    const cv<R> w = tp[j ^ (1u << q)];
+                    const cv<R> w = tp[P(j ^ (1u << q))];
                     sr += hx * (double)w.x;
                     si += hx * (double)w.y;
                 }
                 cv<R> o;
                 o.x = (R)sr;
                 o.y = (R)si;
-                tl[j] = o;
+                tl[P(j)] = o;
                 e += (double)v.x * sr + (double)v.y * si;
             }
             __syncthreads();
@@ -288,10 +429,10 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                 __syncthreads();
                 for (uint32_t j = tid; j < TS; j += T) {
                     const uint64_t g = tile_global(1, t, j, b, n);
-                    const cv<R> v = tp[j];
+                    const cv<R> v = tp[P(j)];
                     double sr = hdiag(g) * (double)v.x, si = hdiag(g) * (double)v.y;
                     for (int q = 0; q < b; ++q) {
-                        const cv<R> w = tp[j ^ (1u << q)];
+                        const cv<R> w = tp[P(j ^ (1u << q))];
                         sr += hx * (double)w.x;
                         si += hx * (double)w.y;
                     }
@@ -307,17 +448,17 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                 load(2, t, tl, glam);
                 __syncthreads();
                 for (uint32_t j = tid; j < TS; j += T) {
-                    const cv<R> v = tp[j];
-                    cv<R> o = tl[j];
+                    const cv<R> v = tp[P(j)];
+                    cv<R> o = tl[P(j)];
                     double sr = (double)o.x, si = (double)o.y;
                     for (int q = b; q < n; ++q) {
-                        const cv<R> w = tp[j ^ (1u << (q - b + (b - h)))];
+                        const cv<R> w = tp[P(j ^ (1u << (q - b + (b - h))))];
                         sr += hx * (double)w.x;
                         si += hx * (double)w.y;
                     }
                     o.x = (R)sr;
                     o.y = (R)si;
-                    tl[j] = o;
+                    tl[P(j)] = o;
                     e += (double)v.x * sr + (double)v.y * si;
                 }
                 __syncthreads();
@@ -340,28 +481,15 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             clear_slots();
             if (resident) {
                 cz(0, 0, true);
-                for (int q = n - 1; q >= 0; --q) adj(q, RT + (n * l + q) * 24 + 8, RT + (n * l + q) * 24 + 16, q);
+                for (int t0 = 0; t0 < n; t0 += GA) adj_group(t0, min(GA, n - t0), 0, l);
             } else {
-                for (uint64_t t = 0; t < ntA; ++t) {  // pass B': CZ ring, qubits n-1..b
-                    load(2, t, tp, gpsi);
-                    load(2, t, tl, glam);
-                    __syncthreads();
+                tile_pass(2, true, [&](uint64_t t) {  // pass B': CZ ring, qubits n-1..b
                     cz(2, t, true);
-                    for (int q = n - 1; q >= b; --q)
-                        adj(q - b + (b - h), RT + (n * l + q) * 24 + 8, RT + (n * l + q) * 24 + 16, q);
-                    store(2, t, tp, gpsi);
-                    store(2, t, tl, glam);
-                    __syncthreads();
-                }
-                for (uint64_t t = 0; t < ntA; ++t) {  // pass A': qubits b-1..0
-                    load(1, t, tp, gpsi);
-                    load(1, t, tl, glam);
-                    __syncthreads();
-                    for (int q = b - 1; q >= 0; --q) adj(q, RT + (n * l + q) * 24 + 8, RT + (n * l + q) * 24 + 16, q);
-                    store(1, t, tp, gpsi);
-                    store(1, t, tl, glam);
-                    __syncthreads();
-                }
+                    for (int t0 = b - h; t0 < b; t0 += GA) adj_group(t0, min(GA, b - t0), h, l);
+                });
+                tile_pass(1, true, [&](uint64_t) {  // pass A': qubits b-1..0
+                    for (int t0 = 0; t0 < b; t0 += GA) adj_group(t0, min(GA, b - t0), 0, l);
+                });
             }
             flush_slots(l);
         }
@@ -470,7 +598,7 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
     int64_t total = 0;
     int nmax = 0;
     for (size_t c = 0; c < ncirc; ++c) {
-        if (nq[c] < 2 || nq[c] > 30 || layers[c] < 1 || 2 * nq[c] > kMaxSlots) {
+        if (nq[c] < 2 || nq[c] > 30 || layers[c] < 1 || 3 * nq[c] > kMaxSlots) {
             set_error(kErrInvalidArgument, "circuit %zu: unsupported shape (n=%d, layers=%d)", c, nq[c], layers[c]);
             return -1;
         }
@@ -528,12 +656,12 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
     hipEventRecord(e0, st);
     if (B.prec == kC128) {
         hipFuncSetAttribute((const void*)k_batch_hea<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_batch_hea<double>, dim3(nblk), dim3(T), lds, st, B.circs, (int)ncirc, (const double*)B.rots, B.grads,
-                           B.energies, (double2*)B.scratch, scratch_elems, B.queue, hx, bmax);
+        hipLaunchKernelGGL(k_batch_hea<double>, dim3(nblk), dim3(T), lds, st, B.circs, (int)ncirc, (const double*)B.rots, B.params,
+                           B.grads, B.energies, (double2*)B.scratch, scratch_elems, B.queue, hx, bmax);
     } else {
         hipFuncSetAttribute((const void*)k_batch_hea<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_batch_hea<float>, dim3(nblk), dim3(T), lds, st, B.circs, (int)ncirc, (const float*)B.rots, B.grads,
-                           B.energies, (float2*)B.scratch, scratch_elems, B.queue, hx, bmax);
+        hipLaunchKernelGGL(k_batch_hea<float>, dim3(nblk), dim3(T), lds, st, B.circs, (int)ncirc, (const float*)B.rots, B.params,
+                           B.grads, B.energies, (float2*)B.scratch, scratch_elems, B.queue, hx, bmax);
     }
     if (!hip_ok(hipGetLastError(), "batch launch")) return -1;
     hipEventRecord(e1, st);
