@@ -402,6 +402,29 @@ def batch_workload(torch, count: int, cpu: bool):
            "autotune": {"tile_bits": tune["tile_bits"], "block_threads": tune["block_threads"],
                         "seconds": round(t_tune, 2)},
            "max_rel_grad_err_vs_complex128": max(errs), "checked_circuits": [specs[i][0] for i in check]}
+    # per size class: each class launched alone.  Circuits that fit the tile
+    # width live in LDS (bound: LDS, 128 B/clk/CU); wider ones stream psi and
+    # lambda through HBM, two tiled passes per layer each way (bound: HBM).
+    lds_peak = 256 * 128 * 2.4e9
+    gm, elem = 4, 8  # complex64: 4 qubits per register group, 8-byte amplitudes
+    classes = {}
+    for n in sorted({s[0] for s in specs}):
+        cls = [s for s in specs if s[0] == n]
+        _, _, cms = runner.run(cls, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"])
+        b, N = tune["tile_bits"][n], 1 << n
+        L = sum(s[1] for s in cls)
+        if n <= b:
+            sweeps_f, sweeps_r = -(-n // gm) + 1, -(-n // gm) + 1
+            byts = L * (sweeps_f * 2 + sweeps_r * 4) * N * elem
+            bound, peak = "lds", lds_peak
+        else:
+            byts = (96 * L + 48 * len(cls)) * N  # per layer 96 N B, per circuit 48 N B (init, H)
+            bound, peak = "hbm", HBM_PEAK_TBPS * 1e12
+        ach = byts / (cms * 1e-3)
+        classes[str(n)] = {"circuits": len(cls), "layers": L, "ms": round(cms, 3), "tile_bits": b,
+                           "circuits_per_s": round(len(cls) / (cms * 1e-3), 1), "bound": bound,
+                           "achieved_TBps": round(ach / 1e12, 3), "frac": round(ach / peak, 3)}
+    out["size_classes"] = classes
     if cpu:
         from oracle import oracle as orc
         core = orc.oracle()
