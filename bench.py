@@ -456,8 +456,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--shift-params", type=int, default=24)
     ap.add_argument("--only-fused", action="store_true", help="skip the c64, adjoint and CPU legs")
-    ap.add_argument("--workload", default="fused", choices=["fused", "circuit20", "dm14", "batch4096"],
-                    help="fused: the headline line; circuit20 / dm14: the other workloads, printed as one JSON line")
+    ap.add_argument("--workload", default="fused", choices=["fused", "circuit20", "dm14", "batch4096", "adjoint24"],
+                    help="fused: the headline line; circuit20 / dm14 / batch4096 / adjoint24: the other "
+                         "workloads, each printed as one JSON line")
     a = ap.parse_args()
     if a.only_fused:
         a.no_c64 = a.no_adjoint = a.no_cpu = True
@@ -481,6 +482,9 @@ def main():
         return
     if a.workload == "dm14":
         print(json.dumps(density_matrix_workload(binding, torch, not a.no_cpu)), flush=True)
+        return
+    if a.workload == "adjoint24":
+        print(json.dumps(adjoint_vs_shift(torch, 24, 12, max(1, a.steps), a.shift_params)), flush=True)
         return
     n = a.qubits
     N = 1 << n

@@ -197,6 +197,10 @@ int    iq_circuit_parameter_shift(void* c, const double* params, size_t nparams,
                                   size_t nwhich, double* grad_out);
 void   iq_circuit_get_amplitudes(void* c, double* real_out, double* imag_out);
 void*  iq_circuit_stream(void* c);
+/* Device memory a gradient may hold for the forward sweep's pass outputs
+ * (default 16 GiB).  While they fit, the reverse sweep reads psi back from them
+ * instead of taking it back gate by gate; 0 turns them off.  -1 for a null handle. */
+int    iq_circuit_set_stored_budget(void* c, size_t bytes);
 
 /* ------------------------------------------- batched sweep, one persistent launch
  * No reference counterpart.  Circuits are the ansatz above (RY, RZ on every

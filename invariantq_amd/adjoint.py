@@ -44,14 +44,15 @@ class GpuCircuit:
         self.precision = precision
         self.num_params = 0
         L.iq_clear_error()
+        self._free = L.iq_circuit_free  # held here: module globals may be gone when __del__ runs at exit
         self._ptr = L.iq_circuit_create(num_qubits, binding.PRECISIONS[precision])
         if not self._ptr:
             binding.check_error()
             raise MemoryError("could not allocate the circuit states on the GPU")
 
     def __del__(self):
-        if getattr(self, "_ptr", None) and binding.HAS_NATIVE_CORE:
-            binding._NATIVE_LIB.iq_circuit_free(self._ptr)
+        if getattr(self, "_ptr", None):
+            self._free(self._ptr)
             self._ptr = None
 
     # ---------------------------------------------------------------- tape
@@ -112,6 +113,15 @@ class GpuCircuit:
     @property
     def stream(self) -> int:
         return binding._NATIVE_LIB.iq_circuit_stream(self._ptr) or 0
+
+    def set_stored_budget(self, nbytes: int) -> None:
+        """Device memory a gradient may keep for the forward sweep's pass
+        outputs (default 16 GiB); within it the reverse sweep reads psi back
+        instead of recomputing it.  0 turns the stored states off."""
+        if isinstance(nbytes, bool) or not isinstance(nbytes, int) or nbytes < 0:
+            raise ValueError("nbytes must be a non-negative integer")
+        if binding._NATIVE_LIB.iq_circuit_set_stored_budget(self._ptr, nbytes) != 0:
+            binding.check_error()
 
     # ------------------------------------------------------------ evaluate
     def _params(self, params) -> np.ndarray:

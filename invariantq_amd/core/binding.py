@@ -160,6 +160,7 @@ SIGNATURES = {
     "iq_circuit_parameter_shift": ([_vp, _d, _sz, ctypes.POINTER(ctypes.c_int64), _sz, _d], ctypes.c_int),
     "iq_circuit_get_amplitudes": ([_vp, _d, _d], None),
     "iq_circuit_stream": ([_vp], _vp),
+    "iq_circuit_set_stored_budget": ([_vp, _sz], ctypes.c_int),
     "iq_batch_create": ([ctypes.c_int], _vp),
     "iq_batch_free": ([_vp], None),
     "iq_batch_max_tile_bits": ([ctypes.c_int], ctypes.c_int),

@@ -177,6 +177,401 @@ __global__ void __launch_bounds__(256) k_apply_h(const cv<R>* __restrict__ psi, 
     block_sum_store<1>(e, 1, part);
 }
 
+// ------------------------------------------------------------ tiled stages
+//
+// A stage (runs of one-qubit gates on any qubits, then at most one CZ set) is
+// applied in a few passes over the state instead of one pass per run.  Pass 0
+// tiles qubits [0, b): tile index bits = global bits 0..b-1.  Pass k >= 1
+// tiles r run bits (global 0..r-1, so every access is a run of 2^r
+// amplitudes) plus the group [g0, g0 + m) of high qubits.  Every tile is
+// staged in LDS by one workgroup; the runs of the pass's qubits are applied
+// there, one LDS sweep per run, and the CZ set (diagonal, from the global
+// index) last.  Runs on different qubits commute, so splitting a stage's runs
+// by qubit over the passes keeps the stage's product.
+constexpr int kPassSegMax = 32;
+
+struct PassSeg {
+    double u[kSubMax][8];  // forward: u[0] = the run's product; reverse: U^dagger of each sub-step
+    int gen[kSubMax];
+    int count;
+    int tbit;  // tile bit of the run's qubit
+    int slot;  // gradient slot (reverse)
+    int pad;
+};
+
+constexpr int kGroupMax = 16;
+
+// Runs of a pass sit in the descriptor array sorted by tile bit and cut into
+// groups of up to 4 consecutive tile bits, one run per bit: a group is applied
+// in registers, one LDS round trip for all its runs.
+constexpr int kShiftMax = 16;
+
+struct PassDesc {
+    int r, m, g0, ts_bits;
+    uint64_t ntiles;
+    int nseg, first, ncz, ngroups;
+    uint8_t gt0[kGroupMax], gsize[kGroupMax], gfirst[kGroupMax];  // first tile bit, runs, first run (in the pass)
+    // Bit g set when an earlier group of the pass shares a qubit with group g.
+    // With none set and psi stored, the reverse sweep takes every group's
+    // overlaps at the pass's end and then moves lambda alone.
+    uint32_t gundo;
+    // The pass's CZ set (ncz edges), split by where the two qubits sit.  Both
+    // in the tile: for each distance s between their tile bits, qmask holds
+    // the lower bit of every such edge.  One in the tile: (tile bit, global
+    // bit) in cza/czb[0, ncross).  Neither: (global, global) in
+    // cza/czb[ncross, ncross + nfixed).
+    int ncross, nfixed, nshift;
+    uint8_t cza[kEdgeMax], czb[kEdgeMax];
+    uint8_t qshift[kShiftMax];
+    uint32_t qmask[kShiftMax];
+};
+
+__device__ __forceinline__ uint64_t pass_global(const PassDesc& d, uint64_t t, uint32_t j) {
+    const uint64_t low = j & ((1u << d.r) - 1);
+    const uint64_t mid = ((uint64_t)(j >> d.r) & ((1ULL << d.m) - 1)) << d.g0;
+    const int gap = d.g0 - d.r;  // global bits [r, g0) come from the tile number's low part
+    const uint64_t tlo = t & ((1ULL << gap) - 1), thi = t >> gap;
+    return low | mid | (tlo << d.r) | (thi << (d.g0 + d.m));
+}
+
+// LDS position of tile element j (low four index bits XOR the next four):
+// sixteen lanes reading a register group's sub-cube elements hit distinct
+// 8-byte slots whichever tile bits the group holds.
+__device__ __forceinline__ uint32_t lds_pos(uint32_t j) { return j ^ ((j >> 4) & 15u); }
+
+// tile in: 16 elements per thread in flight, every register assigned
+template <typename R>
+__device__ __forceinline__ void tile_in(const PassDesc& d, uint64_t t, const cv<R>* __restrict__ src, cv<R>* dst,
+                                        uint32_t TS) {
+    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    for (uint32_t j0 = tid; j0 < TS; j0 += 16 * T) {
+        cv<R> v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = src[pass_global(d, t, min(j0 + u * T, TS - 1))];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (j0 + u * T < TS) dst[lds_pos(j0 + u * T)] = v[u];
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void tile_out(const PassDesc& d, uint64_t t, const cv<R>* src, cv<R>* __restrict__ dst,
+                                         uint32_t TS) {
+    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    for (uint32_t j = tid; j < TS; j += T) dst[pass_global(d, t, j)] = src[lds_pos(j)];
+}
+
+// CZ set on tile t: the phase of element j is (-1)^(c ^ <j & M> ^ sum_s
+// <j & (j >> s) & qmask_s>), where c (edges between two fixed qubits) and M
+// (tile bits paired with a fixed qubit that is set) depend only on the tile.
+template <typename R>
+__device__ __forceinline__ void cz_tile(const PassDesc& d, uint64_t t, cv<R>* tp, cv<R>* tl, uint32_t TS) {
+    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    const uint64_t base = pass_global(d, t, 0);
+    uint32_t M = 0, c = 0;
+    for (int k = 0; k < d.ncross; ++k) M ^= (uint32_t)((base >> d.czb[k]) & 1u) << d.cza[k];
+    for (int k = d.ncross; k < d.ncross + d.nfixed; ++k) c ^= (uint32_t)((base >> d.cza[k]) & (base >> d.czb[k]) & 1u);
+    M = __builtin_amdgcn_readfirstlane(M);
+    c = __builtin_amdgcn_readfirstlane(c);
+    for (uint32_t j = tid; j < TS; j += T) {
+        uint32_t par = c ^ __popc(j & M);
+        for (int k = 0; k < d.nshift; ++k) par ^= __popc(j & (j >> d.qshift[k]) & d.qmask[k]);
+        if (par & 1u) {
+            const uint32_t at = lds_pos(j);
+            cv<R> v = tp[at];
+            v.x = -v.x;
+            v.y = -v.y;
+            tp[at] = v;
+            if (tl) {
+                cv<R> w = tl[at];
+                w.x = -w.x;
+                w.y = -w.y;
+                tl[at] = w;
+            }
+        }
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void mat2(const R* u, cv<R>& a, cv<R>& c) {
+    const R y0r = u[0] * a.x - u[1] * a.y + u[2] * c.x - u[3] * c.y;
+    const R y0i = u[0] * a.y + u[1] * a.x + u[2] * c.y + u[3] * c.x;
+    const R y1r = u[4] * a.x - u[5] * a.y + u[6] * c.x - u[7] * c.y;
+    const R y1i = u[4] * a.y + u[5] * a.x + u[6] * c.y + u[7] * c.x;
+    a.x = y0r; a.y = y0i; c.x = y1r; c.y = y1i;
+}
+
+// Forward: the G runs of a group (tile bits t0 .. t0+G-1), each its product
+// matrix, applied to sub-cubes of 2^G elements held in registers.
+template <typename R, int G>
+__device__ __forceinline__ void fwd_group(cv<R>* tile, uint32_t TS, int t0, const PassSeg* __restrict__ sg) {
+    constexpr uint32_t D = 1u << G;
+    const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
+    R u[G][8];
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) u[i][e] = (R)sg[i].u[0][e];
+    for (uint32_t s = tid; s < cubes; s += T) {
+        const uint32_t base = ((s & ~lo) << G) | (s & lo);
+        cv<R> x[D];
+#pragma unroll
+        for (uint32_t m = 0; m < D; ++m) x[m] = tile[lds_pos(base | (m << t0))];
+#pragma unroll
+        for (int i = 0; i < G; ++i)
+#pragma unroll
+            for (uint32_t m = 0; m < D; ++m)
+                if (!(m & (1u << i))) mat2<R>(u[i], x[m], x[m | (1u << i)]);
+#pragma unroll
+        for (uint32_t m = 0; m < D; ++m) tile[lds_pos(base | (m << t0))] = x[m];
+    }
+}
+
+// Reverse: the runs of a group sit on distinct qubits and commute, so every
+// run's overlaps T_s = Im<lambda| s psi> (s = X, Y, Z on the run's qubit) are
+// taken from the same register contents, before any inverse; then each run's
+// combined inverse U^dagger is applied to psi and lambda.  A sub-step k of
+// the run has gradient Im<lambda| G'_k psi> with G'_k = V_k G_k V_k^dagger
+// (V_k the run's later sub-steps), a real combination of the three T_s formed
+// on the host.  Products are summed over a sub-cube in the state's precision,
+// then into FP64 per-thread sums.
+template <typename R, int G>
+__device__ __forceinline__ void sub_overlaps(const cv<R> (&x)[1 << G], const cv<R> (&y)[1 << G], double (&acc)[4][3]) {
+    constexpr uint32_t D = 1u << G;
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const uint32_t bit = 1u << i;
+        R tx = 0, ty = 0, tz = 0;
+#pragma unroll
+        for (uint32_t m = 0; m < D; ++m) {
+            if (m & bit) continue;
+            const cv<R>&a0 = x[m], &a1 = x[m | bit], &b0 = y[m], &b1 = y[m | bit];
+            tx += (b0.x * a1.y - b0.y * a1.x) + (b1.x * a0.y - b1.y * a0.x);
+            ty += (b1.x * a0.x + b1.y * a0.y) - (b0.x * a1.x + b0.y * a1.y);
+            tz += (b0.x * a0.y - b0.y * a0.x) - (b1.x * a1.y - b1.y * a1.x);
+        }
+        acc[i][0] += (double)tx;
+        acc[i][1] += (double)ty;
+        acc[i][2] += (double)tz;
+    }
+}
+
+template <typename R, int G>
+__device__ __forceinline__ void load_runs(const PassSeg* __restrict__ sg, R (&u)[G][8]) {
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) u[i][e] = (R)sg[i].u[0][e];
+}
+
+template <typename R, int G>
+__device__ __forceinline__ void rev_group(cv<R>* tp, cv<R>* tl, uint32_t TS, int t0, const PassSeg* __restrict__ sg,
+                                          double (&acc)[4][3]) {
+    constexpr uint32_t D = 1u << G;
+    const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
+    R u[G][8];
+    load_runs<R, G>(sg, u);
+    for (uint32_t s = tid; s < cubes; s += T) {
+        const uint32_t base = ((s & ~lo) << G) | (s & lo);
+        cv<R> x[D], y[D];
+#pragma unroll
+        for (uint32_t m = 0; m < D; ++m) {
+            x[m] = tp[lds_pos(base | (m << t0))];
+            y[m] = tl[lds_pos(base | (m << t0))];
+        }
+        sub_overlaps<R, G>(x, y, acc);
+#pragma unroll
+        for (int i = 0; i < G; ++i)
+#pragma unroll
+            for (uint32_t m = 0; m < D; ++m) {
+                if (m & (1u << i)) continue;
+                mat2<R>(u[i], x[m], x[m | (1u << i)]);
+                mat2<R>(u[i], y[m], y[m | (1u << i)]);
+            }
+#pragma unroll
+        for (uint32_t m = 0; m < D; ++m) {
+            tp[lds_pos(base | (m << t0))] = x[m];
+            tl[lds_pos(base | (m << t0))] = y[m];
+        }
+    }
+}
+
+// Overlaps of a group's runs only: psi and lambda are read, not changed.
+template <typename R, int G>
+__device__ __forceinline__ void rev_overlaps(const cv<R>* tp, const cv<R>* tl, uint32_t TS, int t0,
+                                             double (&acc)[4][3]) {
+    constexpr uint32_t D = 1u << G;
+    const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
+    for (uint32_t s = tid; s < cubes; s += T) {
+        const uint32_t base = ((s & ~lo) << G) | (s & lo);
+        cv<R> x[D], y[D];
+#pragma unroll
+        for (uint32_t m = 0; m < D; ++m) {
+            x[m] = tp[lds_pos(base | (m << t0))];
+            y[m] = tl[lds_pos(base | (m << t0))];
+        }
+        sub_overlaps<R, G>(x, y, acc);
+    }
+}
+
+// A group's inverses on lambda only.
+template <typename R, int G>
+__device__ __forceinline__ void lam_back(cv<R>* tl, uint32_t TS, int t0, const PassSeg* __restrict__ sg) {
+    constexpr uint32_t D = 1u << G;
+    const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
+    R u[G][8];
+    load_runs<R, G>(sg, u);
+    for (uint32_t s = tid; s < cubes; s += T) {
+        const uint32_t base = ((s & ~lo) << G) | (s & lo);
+        cv<R> y[D];
+#pragma unroll
+        for (uint32_t m = 0; m < D; ++m) y[m] = tl[lds_pos(base | (m << t0))];
+#pragma unroll
+        for (int i = 0; i < G; ++i)
+#pragma unroll
+            for (uint32_t m = 0; m < D; ++m)
+                if (!(m & (1u << i))) mat2<R>(u[i], y[m], y[m | (1u << i)]);
+#pragma unroll
+        for (uint32_t m = 0; m < D; ++m) tl[lds_pos(base | (m << t0))] = y[m];
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(512) k_tile_fwd(const cv<R>* __restrict__ src, cv<R>* __restrict__ dst, PassDesc d,
+                                                  const PassSeg* __restrict__ segs) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cv<R>* tile = reinterpret_cast<cv<R>*>(smem);
+    const uint32_t TS = 1u << d.ts_bits;
+    const PassSeg* ps = segs + d.first;
+    for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        tile_in<R>(d, t, src, tile, TS);
+        __syncthreads();
+        for (int gi = 0; gi < d.ngroups; ++gi) {
+            // a wave-uniform index: the descriptor reads below become scalar loads
+            const int g = __builtin_amdgcn_readfirstlane(gi);
+            const int t0 = d.gt0[g];
+            const PassSeg* sg = ps + d.gfirst[g];
+            switch (d.gsize[g]) {
+                case 1: fwd_group<R, 1>(tile, TS, t0, sg); break;
+                case 2: fwd_group<R, 2>(tile, TS, t0, sg); break;
+                case 3: fwd_group<R, 3>(tile, TS, t0, sg); break;
+                default: if (sizeof(R) == 4) fwd_group<R, sizeof(R) == 4 ? 4 : 3>(tile, TS, t0, sg); break;
+            }
+            __syncthreads();
+        }
+        if (d.ncz) {
+            cz_tile<R>(d, t, tile, (cv<R>*)nullptr, TS);
+            __syncthreads();
+        }
+        tile_out<R>(d, t, tile, dst, TS);
+        __syncthreads();
+    }
+}
+
+// Group width dispatch (register groups hold up to 4 complex64 or 3
+// complex128 qubits).
+#define IQ_GROUP_SWITCH(gs, CALL)                               \
+    switch (gs) {                                               \
+        case 1: CALL(1); break;                                 \
+        case 2: CALL(2); break;                                 \
+        case 3: CALL(3); break;                                 \
+        default: if (sizeof(R) == 4) CALL(sizeof(R) == 4 ? 4 : 3); break; \
+    }
+
+template <typename R>
+__device__ __forceinline__ void flush_overlaps(const double (&acc)[4][3], int gs, double* wred, int wave, int lane,
+                                               int first) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= gs) continue;
+#pragma unroll
+        for (int st = 0; st < 3; ++st) {
+            double x = acc[i][st];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+            if (lane == 0) wred[(wave * kPassSegMax + first + i) * kSubMax + st] += x;
+        }
+    }
+}
+
+// Reverse of a forward pass on psi and lambda: the CZ set first, then the
+// groups last to first.  STORED: psi is the pass's stored output, only read
+// (when a qubit repeats in the pass, its tile copy is taken back alongside
+// lambda); otherwise psi is taken back through the pass in place.  Each run's overlaps (T_X, T_Y, T_Z in entries 0-2 of
+// its slot, entry 3 zero) go per wave into LDS, per block (fixed order over
+// waves) into the partial layout k_finalize_slots reduces; the grid is
+// exactly `nblk` blocks, each writing every run's partials, zeros included.
+template <typename R, bool STORED>
+__global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>* __restrict__ lam, PassDesc d,
+                                                  const PassSeg* __restrict__ segs, double* __restrict__ gpart,
+                                                  int nblk) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t T = blockDim.x, tid = threadIdx.x, TS = 1u << d.ts_bits;
+    const int lane = tid & 63, wave = tid >> 6, nw = T >> 6;
+    cv<R>* tp = reinterpret_cast<cv<R>*>(smem);
+    cv<R>* tl = tp + TS;
+    double* wred = reinterpret_cast<double*>(tl + TS);  // [nw][kPassSegMax][kSubMax]
+    for (int i = tid; i < nw * kPassSegMax * kSubMax; i += T) wred[i] = 0.0;
+    __syncthreads();
+    const PassSeg* ps = segs + d.first;
+    for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        tile_in<R>(d, t, psi, tp, TS);
+        tile_in<R>(d, t, lam, tl, TS);
+        __syncthreads();
+        if (d.ncz) {
+            cz_tile<R>(d, t, tp, tl, TS);
+            __syncthreads();
+        }
+        if (STORED && d.gundo == 0) {
+            // no qubit repeats in the pass: every group's overlaps at the
+            // pass's end (psi as stored), then lambda back through the groups
+            for (int gi = d.ngroups - 1; gi >= 0; --gi) {
+                // a wave-uniform index: the descriptor reads below become scalar loads
+                const int g = __builtin_amdgcn_readfirstlane(gi);
+                const int t0 = d.gt0[g], gs = d.gsize[g];
+                double acc[4][3] = {};
+#define IQ_CALL(G) rev_overlaps<R, G>(tp, tl, TS, t0, acc)
+                IQ_GROUP_SWITCH(gs, IQ_CALL)
+#undef IQ_CALL
+                flush_overlaps<R>(acc, gs, wred, wave, lane, d.gfirst[g]);
+            }
+            __syncthreads();
+            for (int gi = d.ngroups - 1; gi >= 0; --gi) {
+                const int g = __builtin_amdgcn_readfirstlane(gi);
+                const int t0 = d.gt0[g], gs = d.gsize[g];
+                const PassSeg* sg = ps + d.gfirst[g];
+#define IQ_CALL(G) lam_back<R, G>(tl, TS, t0, sg)
+                IQ_GROUP_SWITCH(gs, IQ_CALL)
+#undef IQ_CALL
+                __syncthreads();
+            }
+        } else {
+            // psi and lambda back together, group by group (a repeated qubit
+            // needs the cut between its runs)
+            for (int gi = d.ngroups - 1; gi >= 0; --gi) {
+                const int g = __builtin_amdgcn_readfirstlane(gi);
+                const int t0 = d.gt0[g], gs = d.gsize[g];
+                const PassSeg* sg = ps + d.gfirst[g];
+                double acc[4][3] = {};
+#define IQ_CALL(G) rev_group<R, G>(tp, tl, TS, t0, sg, acc)
+                IQ_GROUP_SWITCH(gs, IQ_CALL)
+#undef IQ_CALL
+                flush_overlaps<R>(acc, gs, wred, wave, lane, d.gfirst[g]);
+                __syncthreads();
+            }
+        }
+        if (!STORED) tile_out<R>(d, t, tp, psi, TS);
+        tile_out<R>(d, t, tl, lam, TS);
+        __syncthreads();
+    }
+    for (int i = tid; i < d.nseg * kSubMax; i += T) {
+        const int k = i / kSubMax, st = i % kSubMax;
+        double v = 0.0;
+        for (int w = 0; w < nw; ++w) v += wred[(w * kPassSegMax + k) * kSubMax + st];
+        gpart[((uint64_t)ps[k].slot * nblk + blockIdx.x) * kSubMax + st] = v;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_finalize_slots(const double* __restrict__ part, int nblocks, int nslots,
                                                         double* __restrict__ out) {
     // one block per output: slot = blockIdx.x / kSubMax, sub-step = blockIdx.x % kSubMax
@@ -210,6 +605,14 @@ M2 mul(const M2& a, const M2& b) {  // a * b
              a.m[2] * b.m[0] + a.m[3] * b.m[2], a.m[2] * b.m[1] + a.m[3] * b.m[3]}};
 }
 M2 dag(const M2& a) { return {{std::conj(a.m[0]), std::conj(a.m[2]), std::conj(a.m[1]), std::conj(a.m[3])}}; }
+M2 pauli(int gen) {
+    switch (gen) {
+        case kGenX: return {{0.0, 1.0, 1.0, 0.0}};
+        case kGenY: return {{0.0, cplx(0, -1), cplx(0, 1), 0.0}};
+        case kGenZ: return {{1.0, 0.0, 0.0, -1.0}};
+        default: return {{0.0, 0.0, 0.0, 0.0}};
+    }
+}
 
 }  // namespace
 
@@ -254,15 +657,40 @@ struct Circuit {
     ObsTable obs{};
     double* evals = nullptr;  // device energies for parameter shift
     size_t evals_cap = 0;
+    // tiled stages: pass descriptors go up through a small ring of pinned
+    // (coherent) + device buffers, so consecutive evaluations need no sync
+    struct DescSlot {
+        void* host = nullptr;
+        void* host_dev = nullptr;
+        void* dev = nullptr;
+        size_t cap = 0;
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+    };
+    DescSlot dring[4];
+    int dnext = 0;
+    // Stored states of a tiled forward sweep: pass i writes stored[i] (the
+    // last pass writes the engine state), and the reverse sweep reads them
+    // instead of taking psi back.  Used while they fit stored_budget bytes.
+    void* stored = nullptr;
+    size_t stored_cap = 0;   // bytes
+    size_t stored_budget = size_t(16) << 30;
+    bool stored_valid = false;  // the last forward sweep filled them
 
     ~Circuit() {
         if (eng.stream) hipStreamSynchronize(eng.stream);
         if (lam) hipFree(lam);
+        if (stored) hipFree(stored);
         if (gpart) hipFree(gpart);
         if (gout) hipFree(gout);
         if (hgout) hipHostFree(hgout);
         if (obs_dev) hipFree(obs_dev);
         if (evals) hipFree(evals);
+        for (auto& d : dring) {
+            if (d.host) hipHostFree(d.host);
+            if (d.dev) hipFree(d.dev);
+            if (d.ev) hipEventDestroy(d.ev);
+        }
     }
 };
 
@@ -427,7 +855,7 @@ bool check_params(const Circuit& c, size_t nparams) {
 }
 
 template <typename R>
-bool run_forward(Circuit& c, const double* params) {
+bool run_forward_segments(Circuit& c, const double* params) {
     Engine& e = c.eng;
     if (!e.reset_basis(0)) return false;
     for (const auto& s : c.segs) {
@@ -493,7 +921,7 @@ bool apply_observable(Circuit& c, bool write, double* energy_dev) {
 }
 
 template <typename R>
-bool run_reverse(Circuit& c, const double* params) {
+bool run_reverse_segments(Circuit& c, const double* params) {
     Engine& e = c.eng;
     const uint64_t npairs = e.dim >> 1;
     for (int si = (int)c.segs.size() - 1; si >= 0; --si) {
@@ -554,6 +982,406 @@ bool run_reverse(Circuit& c, const double* params) {
         if (!hip_ok(hipGetLastError(), "gradient finalize")) return false;
     }
     return true;
+}
+
+// ----------------------------------------------------------- tiled stages
+// Stage plan: consecutive one-qubit runs (any qubits, order kept per qubit)
+// closed by a CZ set; a two-qubit gate stands alone between stages.
+struct TileGeom {
+    int b, r;  // pass-0 tile bits; run bits of later passes
+};
+
+// 64 KiB tiles: with 512 threads and register groups of up to 4 (complex64)
+// or 3 (complex128) qubits, every thread owns one sub-cube per group.  Runs
+// of 2^r amplitudes are 128-byte lines.
+TileGeom tile_geom(const Circuit& c) { return c.prec == kC64 ? TileGeom{13, 4} : TileGeom{12, 3}; }
+
+// tiled stages pay off once the state no longer fits a few tiles
+bool use_tiles(const Circuit& c) { return c.n >= 16 && c.n <= 34; }
+
+// Qubits [b, n) are split evenly over the passes after pass 0, at most b - r
+// each; a pass's run bits fill the rest of its 2^b tile (>= r of them), so no
+// pass degenerates into many tiny tiles.
+int high_passes(const TileGeom& g, int n) { return n <= g.b ? 0 : (n - g.b + (g.b - g.r) - 1) / (g.b - g.r); }
+
+PassDesc pass_desc(const TileGeom& g, int n, int k) {
+    PassDesc d{};
+    if (k == 0) {
+        d.r = std::min(g.b, n);
+        d.m = 0;
+        d.g0 = d.r;
+    } else {
+        const int P = high_passes(g, n), total = n - g.b, base = total / P, extra = total % P;
+        d.m = base + (k - 1 < extra ? 1 : 0);
+        d.g0 = g.b + (k - 1) * base + std::min(k - 1, extra);
+        d.r = g.b - d.m;
+    }
+    d.ts_bits = d.r + d.m;
+    d.ntiles = 1ULL << (n - d.ts_bits);
+    return d;
+}
+
+int pass_of(const TileGeom& g, int n, int q) {
+    if (q < g.b) return 0;
+    for (int k = 1;; ++k) {
+        const PassDesc d = pass_desc(g, n, k);
+        if (q < d.g0 + d.m) return k;
+    }
+}
+
+int tile_bit(const TileGeom&, const PassDesc& d, int k, int q) { return k == 0 ? q : d.r + (q - d.g0); }
+
+// Tile bit of global qubit q in pass d, or -1 when q is fixed by the tile index.
+int local_bit(const PassDesc& d, int q) {
+    if (q < d.r) return q;
+    if (d.m > 0 && q >= d.g0 && q < d.g0 + d.m) return d.r + (q - d.g0);
+    return -1;
+}
+
+// Files a CZ set into the pass descriptor by where each edge's qubits sit
+// (the split cz_tile evaluates).  A repeated edge cancels: masks combine by XOR.
+void set_cz(PassDesc& d, const std::vector<std::pair<int, int>>& edges) {
+    d.ncz = (int)edges.size();
+    d.ncross = d.nfixed = d.nshift = 0;
+    std::vector<std::pair<int, int>> fixed;
+    for (const auto& e : edges) {
+        int la = local_bit(d, e.first), lb = local_bit(d, e.second);
+        if (la >= 0 && lb >= 0) {
+            if (la > lb) std::swap(la, lb);
+            const int sh = lb - la;
+            int k = 0;
+            while (k < d.nshift && d.qshift[k] != sh) ++k;
+            if (k == d.nshift) {
+                d.qshift[k] = (uint8_t)sh;
+                d.qmask[k] = 0;
+                ++d.nshift;
+            }
+            d.qmask[k] ^= 1u << la;
+        } else if (la >= 0 || lb >= 0) {
+            d.cza[d.ncross] = (uint8_t)(la >= 0 ? la : lb);
+            d.czb[d.ncross] = (uint8_t)(la >= 0 ? e.second : e.first);
+            ++d.ncross;
+        } else {
+            fixed.push_back(e);
+        }
+    }
+    for (const auto& e : fixed) {
+        d.cza[d.ncross + d.nfixed] = (uint8_t)e.first;
+        d.czb[d.ncross + d.nfixed] = (uint8_t)e.second;
+        ++d.nfixed;
+    }
+}
+
+struct Launch {
+    bool tile = true;
+    PassDesc d{};
+    int seg2 = -1;  // a two-qubit segment
+};
+
+// Passes of the whole tape in forward order (reverse = false) or the reverse
+// sweep's order, with their run descriptors.
+void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vector<PassSeg>& segv,
+                std::vector<Launch>& launches) {
+    const TileGeom g = tile_geom(c);
+    struct Stage {
+        std::vector<int> runs;
+        int cz = -1, seg2 = -1;
+    };
+    std::vector<Stage> stages;
+    Stage cur;
+    for (int i = 0; i < (int)c.segs.size(); ++i) {
+        const Segment& sg = c.segs[i];
+        if (sg.kind == 0) {
+            cur.runs.push_back(i);
+        } else if (sg.kind == 1) {
+            cur.cz = i;
+            stages.push_back(cur);
+            cur = Stage{};
+        } else {
+            if (!cur.runs.empty()) stages.push_back(cur);
+            cur = Stage{};
+            Stage two;
+            two.seg2 = i;
+            stages.push_back(two);
+        }
+    }
+    if (!cur.runs.empty() || cur.cz >= 0) stages.push_back(cur);
+    const int npass = pass_of(g, c.n, c.n - 1) + 1;
+    auto fill = [&](const Segment& sg, const PassDesc& d, int k) {
+        PassSeg ps{};
+        ps.tbit = tile_bit(g, d, k, sg.q);
+        ps.slot = sg.slot;
+        // the run's product (forward) or its inverse (reverse sweep)
+        M2 m = {{1.0, 0.0, 0.0, 1.0}};
+        for (const auto& sub : sg.subs) m = mul(sub_matrix(c.gates[sub.gate], sub.which, params), m);
+        if (reverse) m = dag(m);
+        for (int i = 0; i < 4; ++i) {
+            ps.u[0][2 * i] = m.m[i].real();
+            ps.u[0][2 * i + 1] = m.m[i].imag();
+        }
+        ps.count = 1;
+        segv.push_back(ps);
+    };
+    std::vector<Launch> fwd;  // in forward order; reversed at the end for the sweep
+    std::vector<std::vector<int>> run_chunks;  // per forward launch: the runs, in forward order
+    for (const Stage& st : stages) {
+        if (st.seg2 >= 0) {
+            Launch l;
+            l.tile = false;
+            l.seg2 = st.seg2;
+            fwd.push_back(l);
+            run_chunks.push_back({});
+            continue;
+        }
+        int last = -1;  // index in fwd of the stage's last pass
+        const int gmax = c.prec == kC64 ? 4 : 3;
+        for (int k = 0; k < npass; ++k) {
+            const PassDesc dk = pass_desc(g, c.n, k);
+            std::vector<int> mine;
+            for (int ri : st.runs)
+                if (pass_of(g, c.n, c.segs[ri].q) == k) mine.push_back(ri);
+            if (mine.empty()) continue;
+            // runs on different qubits commute: sort by tile bit (stable, so a
+            // qubit's own runs keep their order) and cut into groups of up to
+            // gmax consecutive tile bits
+            std::stable_sort(mine.begin(), mine.end(), [&](int a, int b2) {
+                return tile_bit(g, dk, k, c.segs[a].q) < tile_bit(g, dk, k, c.segs[b2].q);
+            });
+            Launch l;
+            l.d = dk;
+            std::vector<int> chunk;
+            int last_tb = -2;
+            auto emit = [&]() {
+                if (chunk.empty()) return;
+                l.d.nseg = (int)chunk.size();
+                fwd.push_back(l);
+                run_chunks.push_back(chunk);
+                last = (int)fwd.size() - 1;
+                l = Launch{};
+                l.d = dk;
+                chunk.clear();
+                last_tb = -2;
+            };
+            for (int ri : mine) {
+                const int tb = tile_bit(g, dk, k, c.segs[ri].q);
+                const bool extend = l.d.ngroups > 0 && l.d.gsize[l.d.ngroups - 1] < gmax && tb == last_tb + 1;
+                if (!extend) {
+                    if (l.d.ngroups == kGroupMax || (int)chunk.size() == kPassSegMax) emit();
+                    l.d.gt0[l.d.ngroups] = (uint8_t)tb;
+                    l.d.gsize[l.d.ngroups] = 0;
+                    l.d.gfirst[l.d.ngroups] = (uint8_t)chunk.size();
+                    ++l.d.ngroups;
+                } else if ((int)chunk.size() == kPassSegMax) {
+                    emit();
+                    l.d.gt0[0] = (uint8_t)tb;
+                    l.d.gsize[0] = 0;
+                    l.d.gfirst[0] = 0;
+                    l.d.ngroups = 1;
+                }
+                chunk.push_back(ri);
+                ++l.d.gsize[l.d.ngroups - 1];
+                last_tb = tb;
+            }
+            emit();
+        }
+        if (st.cz >= 0) {
+            if (last < 0) {  // a CZ set with no runs in its stage
+                Launch l;
+                l.d = pass_desc(g, c.n, 0);
+                fwd.push_back(l);
+                run_chunks.push_back({});
+                last = (int)fwd.size() - 1;
+            }
+            set_cz(fwd[last].d, c.segs[st.cz].edges);
+        }
+    }
+    std::vector<int> order(fwd.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = reverse ? (int)(order.size() - 1 - i) : (int)i;
+    for (int i : order) {
+        Launch l = fwd[i];
+        if (l.tile) {
+            l.d.first = (int)segv.size();
+            l.d.nseg = (int)run_chunks[i].size();
+            // groups are tile-bit intervals; one overlapping an earlier group
+            // shares a qubit with it
+            l.d.gundo = 0;
+            for (int g2 = 0; g2 < l.d.ngroups; ++g2)
+                for (int h = 0; h < g2; ++h)
+                    if (l.d.gt0[h] < l.d.gt0[g2] + l.d.gsize[g2] && l.d.gt0[g2] < l.d.gt0[h] + l.d.gsize[h])
+                        l.d.gundo |= 1u << g2;
+            const int k = l.d.m == 0 ? 0 : pass_of(g, c.n, l.d.g0);
+            for (int ri : run_chunks[i]) fill(c.segs[ri], l.d, k);
+        }
+        launches.push_back(l);
+    }
+}
+
+Circuit::DescSlot* upload_descs(Circuit& c, const std::vector<PassSeg>& segv) {
+    Circuit::DescSlot& d = c.dring[c.dnext];
+    c.dnext = (c.dnext + 1) % 4;
+    if (d.pending) hip_ok(hipEventSynchronize(d.ev), "descriptor wait");
+    d.pending = false;
+    const size_t bytes = std::max<size_t>(16, (segv.size() * sizeof(PassSeg) + 15) & ~size_t(15));
+    if (bytes > d.cap) {
+        if (d.host) hipHostFree(d.host);
+        if (d.dev) hipFree(d.dev);
+        d.host = d.host_dev = d.dev = nullptr;
+        d.cap = 0;
+        if (!hip_ok(hipHostMalloc(&d.host, bytes, hipHostMallocCoherent), "pinned alloc") ||
+            !hip_ok(hipHostGetDevicePointer(&d.host_dev, d.host, 0), "pinned map") ||
+            !hip_ok(hipMalloc(&d.dev, bytes), "alloc"))
+            return nullptr;
+        d.cap = bytes;
+    }
+    if (!d.ev && !hip_ok(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming), "event")) return nullptr;
+    std::memcpy(d.host, segv.data(), segv.size() * sizeof(PassSeg));
+    if (!hip_ok(launch_upload(d.dev, d.host_dev, bytes, c.eng.stream), "descriptor upload")) return nullptr;
+    return &d;
+}
+
+// A two-qubit gate of the tape, U (forward) or U^dagger (reverse sweep; CX is
+// its own inverse), on each of the states a and b that is not null.
+bool launch_two_qubit(Circuit& c, const Segment& s, bool dagger, void* a, void* b) {
+    Engine& e = c.eng;
+    const TapeGate& tg = c.gates[s.gate];
+    double re[16], im[16];
+    if (tg.kind == kCX) {
+        const double cx[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0};
+        std::memcpy(re, cx, sizeof(re));
+        std::memset(im, 0, sizeof(im));
+    } else if (!dagger) {
+        std::memcpy(re, tg.mre, sizeof(re));
+        std::memcpy(im, tg.mim, sizeof(im));
+    } else {
+        for (int r = 0; r < 4; ++r)
+            for (int col = 0; col < 4; ++col) {
+                re[r * 4 + col] = tg.mre[col * 4 + r];
+                im[r * 4 + col] = -tg.mim[col * 4 + r];
+            }
+    }
+    GateDesc g;
+    g.K = 2;
+    g.q[0] = (uint32_t)tg.q0;
+    g.q[1] = (uint32_t)tg.q1;
+    g.re = re;
+    g.im = im;
+    if (a && !hip_ok(launch_gate(a, c.n, c.prec, e.stream, g), "two-qubit gate")) return false;
+    return !b || hip_ok(launch_gate(b, c.n, c.prec, e.stream, g), "two-qubit gate on lambda");
+}
+
+void release_stored(Circuit& c) {
+    if (!c.stored) return;
+    hipStreamSynchronize(c.eng.stream);
+    hipFree(c.stored);
+    c.stored = nullptr;
+    c.stored_cap = 0;
+    c.stored_valid = false;
+}
+
+// Room for the outputs of a tiled forward sweep of `nl` passes (the last one
+// goes to the engine state): false when they exceed the circuit's budget or
+// cannot be allocated, and the reverse sweep then takes psi back instead.
+bool ensure_stored(Circuit& c, size_t nl) {
+    const size_t need = (nl > 1 ? nl - 1 : 0) * (c.eng.dim * c.eng.elem);
+    if (need > c.stored_budget) return false;
+    if (need <= c.stored_cap) return true;
+    release_stored(c);
+    if (hipMalloc(&c.stored, need) != hipSuccess) {
+        (void)hipGetLastError();
+        c.stored = nullptr;
+        return false;
+    }
+    c.stored_cap = need;
+    return true;
+}
+
+// Where pass i of an nl-pass sweep writes: its stored slot, the last pass
+// (and every pass when nothing is stored) the engine state.
+void* pass_out(Circuit& c, bool stored, size_t i, size_t nl) {
+    return stored && i + 1 < nl ? static_cast<char*>(c.stored) + i * (c.eng.dim * c.eng.elem) : c.eng.amp;
+}
+
+// Forward sweep; with `store`, every pass but the last writes its own stored
+// slot (same traffic as in place), for the reverse sweep to read.
+template <typename R>
+bool run_forward_tiled(Circuit& c, const double* params, bool store) {
+    Engine& e = c.eng;
+    c.stored_valid = false;
+    if (!e.reset_basis(0)) return false;
+    std::vector<PassSeg> segv;
+    std::vector<Launch> launches;
+    plan_tiles(c, params, false, segv, launches);
+    Circuit::DescSlot* ds = upload_descs(c, segv);
+    if (!ds) return false;
+    const PassSeg* dsegs = static_cast<const PassSeg*>(ds->dev);
+    const size_t nl = launches.size(), bytes = e.dim * e.elem;
+    const bool st = store && ensure_stored(c, nl);
+    for (size_t i = 0; i < nl; ++i) {
+        const Launch& l = launches[i];
+        void* src = i == 0 ? e.amp : pass_out(c, st, i - 1, nl);
+        void* dst = pass_out(c, st, i, nl);
+        if (!l.tile) {
+            if (src != dst && !hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, e.stream), "state copy"))
+                return false;
+            if (!launch_two_qubit(c, c.segs[l.seg2], false, dst, nullptr)) return false;
+            continue;
+        }
+        const size_t lds = ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>);
+        const unsigned grid = (unsigned)std::min<uint64_t>(l.d.ntiles, 2048);
+        hipFuncSetAttribute((const void*)k_tile_fwd<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_tile_fwd<R>, dim3(grid), dim3(512), lds, e.stream, (const cv<R>*)src, (cv<R>*)dst, l.d,
+                           dsegs);
+        if (!hip_ok(hipGetLastError(), "tiled forward")) return false;
+    }
+    ds->pending = hip_ok(hipEventRecord(ds->ev, e.stream), "event record");
+    c.stored_valid = ds->pending && st;
+    return ds->pending;
+}
+
+template <typename R>
+bool run_reverse_tiled(Circuit& c, const double* params) {
+    Engine& e = c.eng;
+    std::vector<PassSeg> segv;
+    std::vector<Launch> launches;
+    plan_tiles(c, params, true, segv, launches);
+    Circuit::DescSlot* ds = upload_descs(c, segv);
+    if (!ds) return false;
+    const PassSeg* dsegs = static_cast<const PassSeg*>(ds->dev);
+    // with the forward sweep's stored outputs, psi is read, never taken back
+    const bool st = c.stored_valid;
+    const size_t nl = launches.size();
+    for (size_t k = 0; k < nl; ++k) {
+        const Launch& l = launches[k];
+        const size_t i = nl - 1 - k;  // the pass's place in the forward sweep
+        if (!l.tile) {
+            if (!launch_two_qubit(c, c.segs[l.seg2], true, st ? nullptr : e.amp, c.lam)) return false;
+            continue;
+        }
+        const size_t lds = 2 * ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>) + (size_t)8 * kPassSegMax * kSubMax * sizeof(double);
+        auto kern = st ? k_tile_rev<R, true> : k_tile_rev<R, false>;
+        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(kern, dim3(c.nblk), dim3(512), lds, e.stream, (cv<R>*)pass_out(c, st, i, nl),
+                           (cv<R>*)c.lam, l.d, dsegs, c.gpart, (int)c.nblk);
+        if (!hip_ok(hipGetLastError(), "tiled reverse")) return false;
+    }
+    c.stored_valid = false;
+    if (c.nslots > 0) {
+        hipLaunchKernelGGL(k_finalize_slots, dim3(c.nslots * kSubMax), dim3(256), 0, e.stream, c.gpart, (int)c.nblk,
+                           c.nslots, c.gout);
+        if (!hip_ok(hipGetLastError(), "gradient finalize")) return false;
+    }
+    ds->pending = hip_ok(hipEventRecord(ds->ev, e.stream), "event record");
+    return ds->pending;
+}
+
+template <typename R>
+bool run_forward(Circuit& c, const double* params, bool store = false) {
+    return use_tiles(c) ? run_forward_tiled<R>(c, params, store) : run_forward_segments<R>(c, params);
+}
+
+template <typename R>
+bool run_reverse(Circuit& c, const double* params) {
+    return use_tiles(c) ? run_reverse_tiled<R>(c, params) : run_reverse_segments<R>(c, params);
 }
 
 Circuit* circ_of(void* p) { return static_cast<Circuit*>(p); }
@@ -640,13 +1468,23 @@ int iq_circuit_add_gate(void* cp, int kind, size_t q0, size_t q1, const int64_t*
 
 size_t iq_circuit_num_gates(void* c) { return c ? circ_of(c)->gates.size() : 0; }
 
-// Number of HBM passes of the compiled tape: forward segments (the reverse
-// sweep has the same count, each pass touching both states).
+// Number of HBM passes of the compiled tape's forward sweep: tile stages
+// (and standalone two-qubit gates) when tiled, else segments.  The reverse
+// sweep runs the same passes backwards, each touching both states.  The plan
+// does not depend on parameter values, so it is counted with zeros.
 size_t iq_circuit_num_segments(void* cp) {
     if (!cp) return 0;
     Circuit& c = *circ_of(cp);
     if (!c.compiled) compile(c);
-    return c.segs.size();
+    if (!use_tiles(c)) return c.segs.size();
+    int64_t np = 0;
+    for (const TapeGate& g : c.gates)
+        for (int64_t p : g.p) np = std::max(np, p + 1);
+    const std::vector<double> zeros((size_t)np, 0.0);
+    std::vector<PassSeg> segv;
+    std::vector<Launch> launches;
+    plan_tiles(c, zeros.data(), false, segv, launches);
+    return launches.size();
 }
 
 int iq_circuit_set_observable(void* cp, size_t nterms, const double* coeffs, const char* ops, const size_t* qubits,
@@ -725,9 +1563,9 @@ int iq_circuit_gradient(void* cp, const double* params, size_t nparams, double* 
     DeviceGuard dg(c.eng.dev);
     if (!prepare(c, params, nparams) || !ensure_grad_buffers(c)) return -1;
     const bool ok = c.prec == kC128
-                        ? (run_forward<double>(c, params) && apply_observable<double>(c, true, c.eng.red_out) &&
+                        ? (run_forward<double>(c, params, true) && apply_observable<double>(c, true, c.eng.red_out) &&
                            run_reverse<double>(c, params))
-                        : (run_forward<float>(c, params) && apply_observable<float>(c, true, c.eng.red_out) &&
+                        : (run_forward<float>(c, params, true) && apply_observable<float>(c, true, c.eng.red_out) &&
                            run_reverse<float>(c, params));
     if (!ok) return -1;
     double e = 0.0;
@@ -736,11 +1574,30 @@ int iq_circuit_gradient(void* cp, const double* params, size_t nparams, double* 
         return -1;
     if (!c.eng.fetch(1, &e)) return -1;
     for (size_t j = 0; j < nparams; ++j) grad_out[j] = 0.0;
+    const bool tiled = use_tiles(c);
     for (const auto& s : c.segs) {
         if (s.kind != 0) continue;
-        for (size_t k = 0; k < s.subs.size(); ++k) {
-            const int64_t p = sub_param(c.gates[s.subs[k].gate], s.subs[k].which);
-            if (p >= 0) grad_out[p] += c.hgout[(size_t)s.slot * kSubMax + k];
+        const double* slot = c.hgout + (size_t)s.slot * kSubMax;
+        if (!tiled) {  // per sub-step gradients
+            for (size_t k = 0; k < s.subs.size(); ++k) {
+                const int64_t p = sub_param(c.gates[s.subs[k].gate], s.subs[k].which);
+                if (p >= 0) grad_out[p] += slot[k];
+            }
+            continue;
+        }
+        // the run's overlaps T_X, T_Y, T_Z: sub-step k's gradient is
+        // Im<lambda| V G V^dagger psi> with V the run's later sub-steps, and
+        // V G V^dagger = wx X + wy Y + wz Z
+        M2 V = {{1.0, 0.0, 0.0, 1.0}};
+        for (int k = (int)s.subs.size() - 1; k >= 0; --k) {
+            const TapeGate& tg = c.gates[s.subs[k].gate];
+            const int64_t p = sub_param(tg, s.subs[k].which);
+            if (p >= 0) {
+                const M2 h = mul(mul(V, pauli(sub_gen(tg, s.subs[k].which))), dag(V));
+                const double wx = h.m[1].real(), wy = -h.m[1].imag(), wz = 0.5 * (h.m[0].real() - h.m[3].real());
+                grad_out[p] += wx * slot[0] + wy * slot[1] + wz * slot[2];
+            }
+            V = mul(V, sub_matrix(tg, s.subs[k].which, params));
         }
     }
     if (energy_out) *energy_out = e;
@@ -798,5 +1655,14 @@ void iq_circuit_get_amplitudes(void* cp, double* re, double* im) {
 }
 
 void* iq_circuit_stream(void* cp) { return cp ? (void*)circ_of(cp)->eng.stream : nullptr; }
+
+int iq_circuit_set_stored_budget(void* cp, size_t bytes) {
+    if (!cp) return -1;
+    Circuit& c = *circ_of(cp);
+    DeviceGuard dg(c.eng.dev);
+    c.stored_budget = bytes;
+    if (c.stored_cap > bytes) release_stored(c);
+    return 0;
+}
 
 }  // extern "C"

@@ -58,15 +58,20 @@ def test_hea_forward_energy_and_gradient(A, oracle_lib, precision):
         assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 1e-4
 
 
-def test_mixed_gates_against_central_differences(A, oracle_lib):
-    """RX / U3 / fixed matrices / CX through the adjoint, vs FP64 central differences."""
-    n = 5
+@pytest.mark.parametrize("n,wires", [(5, [0, 1, 2, 3, 4]), (17, [0, 6, 13, 16, 9])])
+def test_mixed_gates_against_central_differences(A, oracle_lib, n, wires):
+    """RX / U3 / fixed matrices / CX through the adjoint, vs FP64 central differences.
+    At 17 qubits the tape runs as tiled stages (wires in both tile passes, two-qubit
+    gates between stages)."""
     rng = np.random.default_rng(11)
     fixed1 = orc.haar_unitary(rng, 2)
     fixed2 = orc.haar_unitary(rng, 4)
-    spec = [("rx", [0], [0]), ("u3", [1], [1, 2, 3]), ("cx", [1, 3], []), ("ry", [2], [4]), ("m1", [4], []),
-            ("rz", [3], [5]), ("m2", [4, 0], []), ("u3", [0], [6, 7, 8]), ("cz", [2, 4], []), ("rx", [4], [9]),
-            ("cx", [4, 2], []), ("ry", [1], [10]), ("rz", [1], [11]), ("u3", [3], [12, 13, 14])]
+    w = wires
+    spec = [("rx", [w[0]], [0]), ("u3", [w[1]], [1, 2, 3]), ("cx", [w[1], w[3]], []), ("ry", [w[2]], [4]),
+            ("m1", [w[4]], []), ("rz", [w[3]], [5]), ("m2", [w[4], w[0]], []), ("u3", [w[0]], [6, 7, 8]),
+            ("cz", [w[2], w[4]], []), ("rx", [w[4]], [9]), ("cx", [w[4], w[2]], []), ("ry", [w[1]], [10]),
+            ("rz", [w[1]], [11]), ("u3", [w[3]], [12, 13, 14])]
+    print(f"mixed tape n={n}", flush=True)
     c = A.GpuCircuit(n, "complex128")
     for name, qs, ps in spec:
         if name == "m1":
@@ -75,7 +80,7 @@ def test_mixed_gates_against_central_differences(A, oracle_lib):
             c.add(A.MATRIX2, qs, matrix=fixed2)
         else:
             c.add(name, qs, ps)
-    terms = workloads.tfim_terms(n) + [(0.7, "XY", [0, 3]), (-0.4, "YZ", [2, 4])]
+    terms = workloads.tfim_terms(n) + [(0.7, "XY", [w[0], w[3]]), (-0.4, "YZ", [w[2], w[4]])]
     c.set_observable(terms)
 
     def make(p):
@@ -106,6 +111,83 @@ def test_mixed_gates_against_central_differences(A, oracle_lib):
     assert np.max(np.abs(g - fd)) < 1e-7
     ps = c.parameter_shift(p)
     assert np.max(np.abs(ps - orc.grad_parameter_shift(oracle_lib, n, make, terms, p))) < 1e-11
+
+
+@pytest.mark.parametrize("stored", [True, False])
+def test_tiled_stages_16q_against_oracle(A, oracle_lib, stored):
+    """16 qubits, 3 layers (tiled stages): complex128 energy, state and every
+    gradient component against the oracle's FP64 parameter shift, with the
+    reverse sweep reading stored pass outputs or taking psi back."""
+    n, layers = 16, 3
+    rng = np.random.default_rng(16)
+    p = rng.uniform(0, 2 * np.pi, 2 * n * layers)
+    c = _hea(A, n, layers, "complex128")
+    if not stored:
+        c.set_stored_budget(0)
+    # per layer: the low qubits in one tile pass, the rest and the CZ ring in
+    # a second
+    assert c.num_segments == 2 * layers
+    e_ref = orc.energy(oracle_lib, n, orc.hea_ops(n, layers, p), orc.tfim_terms(n))
+    assert abs(c.expectation(p) - e_ref) < 1e-12
+    psi_ref = oracle_lib.statevector(n)
+    for qs, m in orc.hea_ops(n, layers, p):
+        (psi_ref.apply_1q(qs[0], m) if len(qs) == 1 else psi_ref.apply_2q(qs[0], qs[1], m))
+    assert np.max(np.abs(c.amplitudes() - psi_ref.get_amplitudes())) < 1e-12
+    e, g = c.gradient(p)
+    assert abs(e - e_ref) < 1e-12
+    g_ref = orc.grad_parameter_shift(oracle_lib, n, lambda x: orc.hea_ops(n, layers, x), orc.tfim_terms(n), p)
+    assert np.max(np.abs(g - g_ref)) < 1e-10
+    c64 = _hea(A, n, layers, "complex64")
+    if not stored:
+        c64.set_stored_budget(0)
+    e64, g64 = c64.gradient(p)
+    assert abs(e64 - e_ref) < 1e-5
+    assert np.linalg.norm(g64 - g_ref) / np.linalg.norm(g_ref) < 1e-4
+
+
+@pytest.mark.parametrize("stored", [True, False])
+def test_repeated_runs_on_one_qubit_in_a_pass(A, oracle_lib, stored):
+    """U3 U3 RX on one qubit is seven sub-steps, two runs in the same tile pass
+    (the second overlapping the first's register group): 17 qubits, complex128,
+    against FP64 central differences and the oracle's parameter shift."""
+    n = 17
+    spec = []
+    k = 0
+    for q in (0, 3, 14, 16):
+        spec += [("u3", [q], [k, k + 1, k + 2]), ("u3", [q], [k + 3, k + 4, k + 5]), ("rx", [q], [k + 6])]
+        k += 7
+    spec += [("cz", [0, 14], []), ("ry", [3], [k]), ("cz", [3, 16], []), ("u3", [14], [k + 1, k + 2, k + 3]),
+             ("u3", [14], [k + 4, k + 5, k + 6])]
+    nparams = k + 7
+    c = A.GpuCircuit(n, "complex128")
+    if not stored:
+        c.set_stored_budget(0)
+    for name, qs, ps in spec:
+        c.add(name, qs, ps)
+    terms = workloads.tfim_terms(n) + [(0.6, "XY", [0, 14]), (-0.3, "YZ", [3, 16])]
+    c.set_observable(terms)
+
+    def make(p):
+        ops = []
+        for name, qs, ps in spec:
+            if name == "u3":
+                ops.append((qs, gates.u3(p[ps[0]], p[ps[1]], p[ps[2]])))
+            elif name == "rx":
+                ops.append((qs, gates.rx(p[ps[0]])))
+            elif name == "ry":
+                ops.append((qs, gates.ry(p[ps[0]])))
+            else:
+                ops.append((qs, gates.CZ))
+        return ops
+
+    p = np.random.default_rng(17).uniform(0, 2 * np.pi, nparams)
+    print(f"repeated runs n={n} stored={stored}", flush=True)
+    e, g = c.gradient(p)
+    assert abs(e - orc.energy(oracle_lib, n, make(p), terms)) < 1e-12
+    g_ref = orc.grad_parameter_shift(oracle_lib, n, make, terms, p)
+    assert np.max(np.abs(g - g_ref)) < 1e-10
+    fd = orc.grad_central_difference(oracle_lib, n, make, terms, p, h=1e-6)
+    assert np.max(np.abs(g - fd)) < 1e-7
 
 
 def test_shared_parameters_accumulate(A):
