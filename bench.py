@@ -140,7 +140,13 @@ def adjoint_vs_shift(torch, n: int, layers: int, steps: int, shift_params: int):
     rel_ps = float(np.linalg.norm(ps64 - ps128) / np.linalg.norm(ps128))
     n_amp = 1 << n
     segs = fast.num_segments
-    adj_bytes = segs * 2 * n_amp * 8 + 3 * n_amp * 8 + segs * 4 * n_amp * 8
+    # HBM traffic of the tiled sweeps: a forward pass reads and writes the
+    # state (2 N s), the observable's three tile passes move 8 N s, a reverse
+    # pass reads the stored psi and reads and writes lambda (3 N s)
+    adj_bytes = segs * 2 * n_amp * 8 + 8 * n_amp * 8 + segs * 3 * n_amp * 8
+    # the same gradient gate by gate: each gate a forward pass, each gate a
+    # reverse pass over psi and lambda, one observable pass
+    gate_bytes = fast.num_gates * 2 * n_amp * 8 + 2 * n_amp * 8 + fast.num_gates * 4 * n_amp * 8
     return {
         "workload": f"{n} qubits, {layers} layers RY+RZ+CZ ring ({fast.num_gates} gates, {ans.num_params} params), "
                     f"TFIM open chain, complex64 state, FP64 accumulation",
@@ -154,8 +160,11 @@ def adjoint_vs_shift(torch, n: int, layers: int, steps: int, shift_params: int):
         "rel_err_adjoint_c64_vs_param_shift_c128": rel_adj_vs_ps,
         "rel_err_param_shift_c64_vs_c128": rel_ps,
         "energy_c64": e64, "energy_c128": e128,
-        "hbm_passes_per_gradient": 2 * segs + 1,
+        "tile_passes_per_sweep": segs,
+        "adjoint_hbm_GB_per_gradient": round(adj_bytes / 1e9, 2),
         "adjoint_effective_TBps": round(adj_bytes / t_adj / 1e12, 3),
+        "gate_by_gate_GB_per_gradient": round(gate_bytes / 1e9, 1),
+        "gate_by_gate_equivalent_TBps": round(gate_bytes / t_adj / 1e12, 2),
     }
 
 

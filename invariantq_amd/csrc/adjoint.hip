@@ -141,7 +141,9 @@ __global__ void __launch_bounds__(256) k_cz_set(cv<R>* __restrict__ psi, cv<R>* 
 
 // lambda = H psi and E = Re <psi|H psi> in one pass:
 // (H psi)[k] = sum_t c_t phase_t(k ^ x_t) psi[k ^ x_t], phase(j) = i^{nY} (-1)^{popcount(j & (Y|Z))}.
-template <typename R, bool WRITE>
+// ACCUM: the table's part of H psi is added to lambda; the energy partials
+// are this part's share either way.
+template <typename R, bool WRITE, bool ACCUM = false>
 __global__ void __launch_bounds__(256) k_apply_h(const cv<R>* __restrict__ psi, cv<R>* __restrict__ lam, uint64_t count,
                                                  ObsTable h, double* __restrict__ part) {
     double e[1] = {0.0};
@@ -169,8 +171,14 @@ __global__ void __launch_bounds__(256) k_apply_h(const cv<R>* __restrict__ psi, 
         e[0] += (double)p.x * sr + (double)p.y * si;
         if (WRITE) {
             cv<R> o;
-            o.x = (R)sr;
-            o.y = (R)si;
+            if (ACCUM) {
+                o = lam[k];
+                o.x = (R)((double)o.x + sr);
+                o.y = (R)((double)o.y + si);
+            } else {
+                o.x = (R)sr;
+                o.y = (R)si;
+            }
             lam[k] = o;
         }
     }
@@ -572,6 +580,103 @@ __global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>
     }
 }
 
+// ------------------------------------------------------ tiled observable
+// The terms of H are grouped by flip mask x; a group whose x lies on the tile
+// bits of some pass is applied there: psi's tile is staged in LDS, the
+// partner of element j is element j ^ x_tile, and a term's sign bits split
+// into tile bits (zl) and bits fixed by the tile index (zfix, one parity per
+// tile and term).  Every launch adds its groups' part of H psi to lambda (the
+// first one writes it) and leaves its part of Re <psi|H psi> in `part`.
+struct ObsGroupT {
+    uint32_t xl;  // flip mask on tile bits
+    int first, count;  // terms, relative to the pass's first term
+};
+
+struct ObsTermT {
+    uint64_t zfix;  // sign bits outside the tile (global positions)
+    uint32_t zl;    // sign bits on tile bits
+    int ny;
+    double coeff;
+};
+
+constexpr int kObsTermsMax = 2048;  // terms per launch (their per-tile coefficients sit in LDS)
+
+struct ObsPass {
+    PassDesc d;  // geometry only
+    int gfirst, gcount, tfirst, tcount;
+};
+
+template <typename R, bool WRITE, bool FIRST>
+__global__ void __launch_bounds__(512) k_obs_tile(const cv<R>* __restrict__ psi, cv<R>* __restrict__ lam, ObsPass op,
+                                                  const ObsGroupT* __restrict__ groups,
+                                                  const ObsTermT* __restrict__ terms, double* __restrict__ part) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double wsum[16];
+    const PassDesc& d = op.d;
+    const uint32_t T = blockDim.x, tid = threadIdx.x, TS = 1u << d.ts_bits;
+    cv<R>* tile = reinterpret_cast<cv<R>*>(smem);
+    double* csg = reinterpret_cast<double*>(tile + TS);  // each term's coefficient with its tile sign
+    const ObsGroupT* gp = groups + op.gfirst;
+    const ObsTermT* tp = terms + op.tfirst;
+    double e = 0.0;
+    for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        tile_in<R>(d, t, psi, tile, TS);
+        const uint64_t base = pass_global(d, t, 0);
+        for (int k = tid; k < op.tcount; k += T) csg[k] = (__popcll(base & tp[k].zfix) & 1) ? -tp[k].coeff : tp[k].coeff;
+        __syncthreads();
+        for (uint32_t j = tid; j < TS; j += T) {
+            double sr = 0.0, si = 0.0;
+            for (int gi = 0; gi < op.gcount; ++gi) {
+                // a wave-uniform index: the table reads below become scalar loads
+                const int g = __builtin_amdgcn_readfirstlane(gi);
+                const uint32_t jp = j ^ gp[g].xl;
+                const cv<R> v = tile[lds_pos(jp)];
+                double cr = 0.0, ci = 0.0;  // sum of c_t * phase_t(partner)
+                const int k0 = gp[g].first, k1 = k0 + gp[g].count;
+                for (int k = k0; k < k1; ++k) {
+                    // the coefficient with its sign bit flipped by the partner's parity
+                    const double c = csg[k];
+                    const int hi = __double2hiint(c) ^ (int)((__popc(jp & tp[k].zl) & 1u) << 31);
+                    const double sc = __hiloint2double(hi, __double2loint(c));
+                    switch (tp[k].ny & 3) {
+                        case 0: cr += sc; break;
+                        case 1: ci += sc; break;
+                        case 2: cr -= sc; break;
+                        default: ci -= sc; break;
+                    }
+                }
+                sr += cr * (double)v.x - ci * (double)v.y;
+                si += cr * (double)v.y + ci * (double)v.x;
+            }
+            const cv<R> p = tile[lds_pos(j)];
+            e += (double)p.x * sr + (double)p.y * si;
+            if (WRITE) {
+                const uint64_t at = pass_global(d, t, j);
+                cv<R> o;
+                if (FIRST) {
+                    o.x = (R)sr;
+                    o.y = (R)si;
+                } else {
+                    o = lam[at];
+                    o.x = (R)((double)o.x + sr);
+                    o.y = (R)((double)o.y + si);
+                }
+                lam[at] = o;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+    if ((tid & 63) == 0) wsum[tid >> 6] = e;
+    __syncthreads();
+    if (tid == 0) {
+        double s2 = 0.0;
+        for (uint32_t w = 0; w < (T >> 6); ++w) s2 += wsum[w];
+        part[blockIdx.x] = s2;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_finalize_slots(const double* __restrict__ part, int nblocks, int nslots,
                                                         double* __restrict__ out) {
     // one block per output: slot = blockIdx.x / kSubMax, sub-step = blockIdx.x % kSubMax
@@ -676,11 +781,25 @@ struct Circuit {
     size_t stored_cap = 0;   // bytes
     size_t stored_budget = size_t(16) << 30;
     bool stored_valid = false;  // the last forward sweep filled them
+    // tiled observable: launches over tile passes, then the groups no pass
+    // holds through global gathers; energy partials of every launch in opart
+    std::vector<ObsPass> opass;
+    void* otab_dev = nullptr;
+    const ObsGroupT* ogroups = nullptr;
+    const ObsTermT* oterms = nullptr;
+    ObsTable far{};
+    void* far_dev = nullptr;
+    double* opart = nullptr;
+    size_t opart_len = 0;
+    bool obs_tiled = false;
 
     ~Circuit() {
         if (eng.stream) hipStreamSynchronize(eng.stream);
         if (lam) hipFree(lam);
         if (stored) hipFree(stored);
+        if (otab_dev) hipFree(otab_dev);
+        if (far_dev) hipFree(far_dev);
+        if (opart) hipFree(opart);
         if (gpart) hipFree(gpart);
         if (gout) hipFree(gout);
         if (hgout) hipHostFree(hgout);
@@ -908,8 +1027,36 @@ bool run_forward_segments(Circuit& c, const double* params) {
 }
 
 template <typename R>
+bool apply_observable_tiled(Circuit& c, bool write, double* energy_dev) {
+    Engine& e = c.eng;
+    size_t off = 0;
+    bool first = true;
+    for (const ObsPass& op : c.opass) {
+        const unsigned grid = (unsigned)std::min<uint64_t>(op.d.ntiles, 2048);
+        const size_t lds = ((size_t)1 << op.d.ts_bits) * sizeof(cv<R>) + (size_t)op.tcount * sizeof(double);
+        auto kern = !write ? k_obs_tile<R, false, true> : (first ? k_obs_tile<R, true, true> : k_obs_tile<R, true, false>);
+        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(512), lds, e.stream, (const cv<R>*)e.amp, (cv<R>*)c.lam, op,
+                           c.ogroups, c.oterms, c.opart + off);
+        if (!hip_ok(hipGetLastError(), "tiled observable")) return false;
+        off += grid;
+        first = false;
+    }
+    if (c.far.ngroups > 0) {
+        const unsigned b = nblocks_for(e.dim);
+        auto kern = !write ? k_apply_h<R, false> : (first ? k_apply_h<R, true, false> : k_apply_h<R, true, true>);
+        hipLaunchKernelGGL(kern, dim3(b), dim3(256), 0, e.stream, (const cv<R>*)e.amp, (cv<R>*)c.lam, e.dim, c.far,
+                           c.opart + off);
+        if (!hip_ok(hipGetLastError(), "observable")) return false;
+        off += b;
+    }
+    return hip_ok(launch_finalize(c.opart, (unsigned)off, 1, energy_dev, e.stream), "finalize");
+}
+
+template <typename R>
 bool apply_observable(Circuit& c, bool write, double* energy_dev) {
     Engine& e = c.eng;
+    if (c.obs_tiled) return apply_observable_tiled<R>(c, write, energy_dev);
     const unsigned b = nblocks_for(e.dim);
     if (write)
         hipLaunchKernelGGL((k_apply_h<R, true>), dim3(b), dim3(256), 0, e.stream, (const cv<R>*)e.amp, (cv<R>*)c.lam, e.dim,
@@ -1214,6 +1361,134 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
         }
         launches.push_back(l);
     }
+}
+
+// Observable over tile passes: each flip-mask group goes to the first pass
+// whose tile bits hold its flip mask; groups no pass holds (or with more terms
+// than one launch takes) stay in the global-gather table `far`.  Inputs are
+// the grouped table of iq_circuit_set_observable.
+bool plan_observable_tiles(Circuit& c, const std::vector<uint64_t>& xm, const std::vector<int>& start,
+                           const std::vector<uint64_t>& zy, const std::vector<int>& ny, const std::vector<double>& co) {
+    const TileGeom g = tile_geom(c);
+    const int npass = pass_of(g, c.n, c.n - 1) + 1;
+    std::vector<PassDesc> pd(npass);
+    for (int k = 0; k < npass; ++k) pd[k] = pass_desc(g, c.n, k);
+    auto split = [&](const PassDesc& d, uint64_t m, uint32_t& tile, uint64_t& fixed) {
+        tile = 0;
+        fixed = 0;
+        for (int q = 0; q < c.n; ++q)
+            if ((m >> q) & 1) {
+                const int lb = local_bit(d, q);
+                if (lb >= 0) tile |= 1u << lb;
+                else fixed |= 1ULL << q;
+            }
+    };
+    std::vector<std::vector<int>> by_pass(npass);
+    std::vector<int> far;
+    for (size_t gi = 0; gi < xm.size(); ++gi) {
+        int at = -1;
+        for (int k = 0; k < npass && at < 0; ++k) {
+            uint32_t tl;
+            uint64_t fx;
+            split(pd[k], xm[gi], tl, fx);
+            if (fx == 0) at = k;
+        }
+        if (at < 0 || start[gi + 1] - start[gi] > kObsTermsMax) far.push_back((int)gi);
+        else by_pass[at].push_back((int)gi);
+    }
+    std::vector<ObsGroupT> G;
+    std::vector<ObsTermT> Tm;
+    c.opass.clear();
+    for (int k = 0; k < npass; ++k) {
+        ObsPass op{};
+        op.d = pd[k];
+        auto flush = [&]() {
+            if (op.gcount > 0) c.opass.push_back(op);
+            op.gfirst = (int)G.size();
+            op.tfirst = (int)Tm.size();
+            op.gcount = op.tcount = 0;
+        };
+        flush();
+        for (int gi : by_pass[k]) {
+            const int nt = start[gi + 1] - start[gi];
+            if (op.tcount + nt > kObsTermsMax) flush();
+            ObsGroupT gg{};
+            uint64_t fx;
+            split(pd[k], xm[gi], gg.xl, fx);
+            gg.first = op.tcount;
+            gg.count = nt;
+            for (int t = start[gi]; t < start[gi + 1]; ++t) {
+                ObsTermT tt{};
+                split(pd[k], zy[t], tt.zl, tt.zfix);
+                tt.ny = ny[t];
+                tt.coeff = co[t];
+                Tm.push_back(tt);
+            }
+            G.push_back(gg);
+            ++op.gcount;
+            op.tcount += nt;
+        }
+        flush();
+    }
+    // far groups as their own flip-mask table
+    std::vector<uint64_t> fxm, fzy;
+    std::vector<int> fstart, fny;
+    std::vector<double> fco;
+    for (int gi : far) {
+        fxm.push_back(xm[gi]);
+        fstart.push_back((int)fzy.size());
+        for (int t = start[gi]; t < start[gi + 1]; ++t) {
+            fzy.push_back(zy[t]);
+            fny.push_back(ny[t]);
+            fco.push_back(co[t]);
+        }
+    }
+    fstart.push_back((int)fzy.size());
+    if (c.otab_dev) hipFree(c.otab_dev);
+    if (c.far_dev) hipFree(c.far_dev);
+    if (c.opart) hipFree(c.opart);
+    c.otab_dev = c.far_dev = nullptr;
+    c.opart = nullptr;
+    c.far = ObsTable{};
+    c.obs_tiled = false;
+    const size_t gbytes = std::max<size_t>(16, G.size() * sizeof(ObsGroupT));
+    const size_t tbytes = std::max<size_t>(16, Tm.size() * sizeof(ObsTermT));
+    const size_t toff = (gbytes + 255) & ~size_t(255);
+    if (!hip_ok(hipMalloc(&c.otab_dev, toff + tbytes), "observable tiles")) return false;
+    char* od = static_cast<char*>(c.otab_dev);
+    if ((!G.empty() && !hip_ok(hipMemcpy(od, G.data(), G.size() * sizeof(ObsGroupT), hipMemcpyHostToDevice), "upload")) ||
+        (!Tm.empty() && !hip_ok(hipMemcpy(od + toff, Tm.data(), Tm.size() * sizeof(ObsTermT), hipMemcpyHostToDevice), "upload")))
+        return false;
+    c.ogroups = reinterpret_cast<const ObsGroupT*>(od);
+    c.oterms = reinterpret_cast<const ObsTermT*>(od + toff);
+    size_t parts = 0;
+    for (const ObsPass& op : c.opass) parts += std::min<uint64_t>(op.d.ntiles, 2048);
+    if (!far.empty()) {
+        const size_t F = fxm.size(), FT = fzy.size();
+        const size_t o1 = 0, o2 = o1 + F * 8, o3 = (o2 + (F + 1) * 4 + 7) & ~size_t(7), o4 = o3 + FT * 8,
+                     o5 = (o4 + FT * 4 + 7) & ~size_t(7), bytes = o5 + FT * 8;
+        std::vector<char> host(bytes, 0);
+        std::memcpy(host.data() + o1, fxm.data(), F * 8);
+        std::memcpy(host.data() + o2, fstart.data(), (F + 1) * 4);
+        std::memcpy(host.data() + o3, fzy.data(), FT * 8);
+        std::memcpy(host.data() + o4, fny.data(), FT * 4);
+        std::memcpy(host.data() + o5, fco.data(), FT * 8);
+        if (!hip_ok(hipMalloc(&c.far_dev, bytes), "observable table") ||
+            !hip_ok(hipMemcpy(c.far_dev, host.data(), bytes, hipMemcpyHostToDevice), "observable upload"))
+            return false;
+        char* d = static_cast<char*>(c.far_dev);
+        c.far.xmask = reinterpret_cast<const uint64_t*>(d + o1);
+        c.far.start = reinterpret_cast<const int*>(d + o2);
+        c.far.zy = reinterpret_cast<const uint64_t*>(d + o3);
+        c.far.ny = reinterpret_cast<const int*>(d + o4);
+        c.far.coeff = reinterpret_cast<const double*>(d + o5);
+        c.far.ngroups = (int)F;
+        parts += nblocks_for(c.eng.dim);
+    }
+    if (!hip_ok(hipMalloc((void**)&c.opart, std::max<size_t>(parts, 1) * sizeof(double)), "energy partials")) return false;
+    c.opart_len = parts;
+    c.obs_tiled = true;
+    return true;
 }
 
 Circuit::DescSlot* upload_descs(Circuit& c, const std::vector<PassSeg>& segv) {
@@ -1541,6 +1816,8 @@ int iq_circuit_set_observable(void* cp, size_t nterms, const double* coeffs, con
     c.obs.coeff = reinterpret_cast<const double*>(d + oco);
     c.obs.ngroups = (int)G;
     c.terms = terms;
+    c.obs_tiled = false;
+    if (use_tiles(c) && !plan_observable_tiles(c, xm, start, zy, ny, co)) return -1;
     return 0;
 }
 
