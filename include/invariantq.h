@@ -195,6 +195,9 @@ int    iq_circuit_gradient(void* c, const double* params, size_t nparams, double
 /* Parameter-shift gradient on the same engine for params listed in `which` (all if NULL). */
 int    iq_circuit_parameter_shift(void* c, const double* params, size_t nparams, const int64_t* which,
                                   size_t nwhich, double* grad_out);
+/* The state after the last evaluation: the circuit's output after an expectation or parameter
+ * shift; after a gradient, the output when pass outputs were stored (see below), else |0..0>
+ * up to rounding (the reverse sweep takes psi back in place). */
 void   iq_circuit_get_amplitudes(void* c, double* real_out, double* imag_out);
 void*  iq_circuit_stream(void* c);
 /* Device memory a gradient may hold for the forward sweep's pass outputs

@@ -162,6 +162,9 @@ class GpuCircuit:
         return g[:count]
 
     def amplitudes(self) -> np.ndarray:
+        """The state after the last evaluation: the circuit's output after
+        `expectation` or `parameter_shift`; after `gradient`, the output when
+        the pass outputs were stored, else |0..0> up to rounding."""
         n = 1 << self.num_qubits
         re, im = np.empty(n), np.empty(n)
         binding._NATIVE_LIB.iq_circuit_get_amplitudes(self._ptr, _dp(re), _dp(im))

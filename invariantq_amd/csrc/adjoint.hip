@@ -1923,8 +1923,10 @@ int iq_circuit_parameter_shift(void* cp, const double* params, size_t nparams, c
     return 0;
 }
 
-// Amplitudes after the most recent forward pass (or adjoint sweep: the reverse
-// sweep returns psi to |0..0> up to rounding).
+// The engine state: after an expectation or parameter shift, the circuit's
+// output state.  After a gradient, the output state when the forward sweep's
+// pass outputs were stored (the reverse sweep only reads them), otherwise psi
+// taken back to |0..0> up to rounding.
 void iq_circuit_get_amplitudes(void* cp, double* re, double* im) {
     if (!cp || !re || !im) return;
     DeviceGuard dg(circ_of(cp)->eng.dev);
