@@ -299,6 +299,54 @@ def circuit_workload(binding, torch, steps: int, warmup: int, cpu: bool):
     return out
 
 
+def bandwidth_workload(binding, torch, reps: int = 5):
+    """Plain passes over 16 GiB (a 30-qubit complex128 state) on the same GPU:
+    what read-only and read+write streaming reach here, beside the fused pass.
+    torch kernels are timed with events on torch's stream; the engine's
+    normalize (one read-only norm pass, then one in-place scale pass) with
+    wall time after a synchronize."""
+    nbytes = 16 * 2**30
+    x = torch.empty(2**30, dtype=torch.complex128, device="cuda")
+    x.real.fill_(1.0)
+    x.imag.fill_(0.5)
+    y = torch.empty_like(x)
+
+    def events(f):
+        f()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            f()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps / 1e3
+
+    t_mul = events(lambda: x.mul_(1.0000001))
+    t_copy = events(lambda: y.copy_(x))
+    t_sum = events(lambda: x.real.sum())
+    del x, y
+    torch.cuda.empty_cache()
+    sv = binding.NativeStateVector(30)
+    sv.normalize()
+    sv.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sv.normalize()
+    sv.synchronize()
+    t_norm = (time.perf_counter() - t0) / reps
+    del sv
+    return {
+        "workload": "plain streaming passes over 16 GiB (2^30 complex128 amplitudes)",
+        "torch_inplace_mul_TBps": round(2 * nbytes / t_mul / 1e12, 3),
+        "torch_copy_TBps": round(2 * nbytes / t_copy / 1e12, 3),
+        "torch_sum_real_part_read_TBps": round(nbytes / t_sum / 1e12, 3),
+        "engine_normalize_TBps": round(3 * nbytes / t_norm / 1e12, 3),
+        "note": "normalize = read-only norm pass + in-place scale pass (3 x 16 GiB); "
+                "rocprofv3 splits it per kernel",
+    }
+
+
 def density_matrix_workload(binding, torch, cpu: bool):
     """14-qubit density matrix (2^28 complex128 entries): depth-10 U3 + brick CX circuit,
     depolarizing (p=1e-3) on touched qubits after every gate, amplitude (1e-3) and phase
@@ -465,7 +513,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--shift-params", type=int, default=24)
     ap.add_argument("--only-fused", action="store_true", help="skip the c64, adjoint and CPU legs")
-    ap.add_argument("--workload", default="fused", choices=["fused", "circuit20", "dm14", "batch4096", "adjoint24"],
+    ap.add_argument("--workload", default="fused", choices=["fused", "circuit20", "dm14", "batch4096", "adjoint24", "bandwidth"],
                     help="fused: the headline line; circuit20 / dm14 / batch4096 / adjoint24: the other "
                          "workloads, each printed as one JSON line")
     a = ap.parse_args()
@@ -494,6 +542,9 @@ def main():
         return
     if a.workload == "adjoint24":
         print(json.dumps(adjoint_vs_shift(torch, 24, 12, max(1, a.steps), a.shift_params)), flush=True)
+        return
+    if a.workload == "bandwidth":
+        print(json.dumps(bandwidth_workload(binding, torch)), flush=True)
         return
     n = a.qubits
     N = 1 << n
