@@ -199,15 +199,16 @@ __global__ void __launch_bounds__(256) k_apply_h(const cv<R>* __restrict__ psi, 
 constexpr int kPassSegMax = 32;
 
 struct PassSeg {
-    double u[kSubMax][8];  // forward: u[0] = the run's product; reverse: U^dagger of each sub-step
-    int gen[kSubMax];
-    int count;
-    int tbit;  // tile bit of the run's qubit
-    int slot;  // gradient slot (reverse)
-    int pad;
+    double u[8];  // the run's product (forward) or its inverse (reverse sweep), re/im row-major
+    int tbit;     // tile bit of the run's qubit
+    int slot;     // gradient slot (reverse)
 };
 
 constexpr int kGroupMax = 16;
+// Runs per register group: up to 4 qubits for complex64, 3 for complex128.
+// With 2^13-amplitude tiles and 512 threads, a group of 4 is one 16-element
+// sub-cube per thread; wider groups would leave threads idle.
+constexpr int kGroupRuns = 4;
 
 // Runs of a pass sit in the descriptor array sorted by tile bit and cut into
 // groups of up to 4 consecutive tile bits, one run per bit: a group is applied
@@ -247,15 +248,49 @@ __device__ __forceinline__ uint64_t pass_global(const PassDesc& d, uint64_t t, u
 // 8-byte slots whichever tile bits the group holds.
 __device__ __forceinline__ uint32_t lds_pos(uint32_t j) { return j ^ ((j >> 4) & 15u); }
 
-// tile in: 16 elements per thread in flight, every register assigned
+// CZ set on tile t: the phase of element j is (-1)^(c ^ <j & M> ^ sum_s
+// <j & (j >> s) & qmask_s>), where c (edges between two fixed qubits) and M
+// (tile bits paired with a fixed qubit that is set) depend only on the tile.
+// The sign is applied as elements move between HBM and LDS, not in a sweep.
+struct CzTile {
+    uint32_t M, c;
+};
+
+__device__ __forceinline__ CzTile cz_setup(const PassDesc& d, uint64_t t) {
+    const uint64_t base = pass_global(d, t, 0);
+    uint32_t M = 0, c = 0;
+    for (int k = 0; k < d.ncross; ++k) M ^= (uint32_t)((base >> d.czb[k]) & 1u) << d.cza[k];
+    for (int k = d.ncross; k < d.ncross + d.nfixed; ++k) c ^= (uint32_t)((base >> d.cza[k]) & (base >> d.czb[k]) & 1u);
+    return CzTile{(uint32_t)__builtin_amdgcn_readfirstlane(M), (uint32_t)__builtin_amdgcn_readfirstlane(c)};
+}
+
+__device__ __forceinline__ bool cz_odd(const PassDesc& d, CzTile z, uint32_t j) {
+    uint32_t par = z.c ^ __popc(j & z.M);
+    for (int k = 0; k < d.nshift; ++k) par ^= __popc(j & (j >> d.qshift[k]) & d.qmask[k]);
+    return (par & 1u) != 0;
+}
+
+template <typename R>
+__device__ __forceinline__ cv<R> negate_if(cv<R> v, bool neg) {
+    v.x = neg ? -v.x : v.x;
+    v.y = neg ? -v.y : v.y;
+    return v;
+}
+
+// tile in: 16 elements per thread in flight, every register assigned; with
+// `cz`, each element takes its CZ sign on the way
 template <typename R>
 __device__ __forceinline__ void tile_in(const PassDesc& d, uint64_t t, const cv<R>* __restrict__ src, cv<R>* dst,
-                                        uint32_t TS) {
+                                        uint32_t TS, bool cz = false, CzTile z = CzTile{0, 0}) {
     const uint32_t T = blockDim.x, tid = threadIdx.x;
     for (uint32_t j0 = tid; j0 < TS; j0 += 16 * T) {
         cv<R> v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) v[u] = src[pass_global(d, t, min(j0 + u * T, TS - 1))];
+        if (cz) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v[u] = negate_if<R>(v[u], cz_odd(d, z, j0 + u * T));
+        }
 #pragma unroll
         for (int u = 0; u < 16; ++u)
             if (j0 + u * T < TS) dst[lds_pos(j0 + u * T)] = v[u];
@@ -264,39 +299,12 @@ __device__ __forceinline__ void tile_in(const PassDesc& d, uint64_t t, const cv<
 
 template <typename R>
 __device__ __forceinline__ void tile_out(const PassDesc& d, uint64_t t, const cv<R>* src, cv<R>* __restrict__ dst,
-                                         uint32_t TS) {
+                                         uint32_t TS, bool cz = false, CzTile z = CzTile{0, 0}) {
     const uint32_t T = blockDim.x, tid = threadIdx.x;
-    for (uint32_t j = tid; j < TS; j += T) dst[pass_global(d, t, j)] = src[lds_pos(j)];
-}
-
-// CZ set on tile t: the phase of element j is (-1)^(c ^ <j & M> ^ sum_s
-// <j & (j >> s) & qmask_s>), where c (edges between two fixed qubits) and M
-// (tile bits paired with a fixed qubit that is set) depend only on the tile.
-template <typename R>
-__device__ __forceinline__ void cz_tile(const PassDesc& d, uint64_t t, cv<R>* tp, cv<R>* tl, uint32_t TS) {
-    const uint32_t T = blockDim.x, tid = threadIdx.x;
-    const uint64_t base = pass_global(d, t, 0);
-    uint32_t M = 0, c = 0;
-    for (int k = 0; k < d.ncross; ++k) M ^= (uint32_t)((base >> d.czb[k]) & 1u) << d.cza[k];
-    for (int k = d.ncross; k < d.ncross + d.nfixed; ++k) c ^= (uint32_t)((base >> d.cza[k]) & (base >> d.czb[k]) & 1u);
-    M = __builtin_amdgcn_readfirstlane(M);
-    c = __builtin_amdgcn_readfirstlane(c);
-    for (uint32_t j = tid; j < TS; j += T) {
-        uint32_t par = c ^ __popc(j & M);
-        for (int k = 0; k < d.nshift; ++k) par ^= __popc(j & (j >> d.qshift[k]) & d.qmask[k]);
-        if (par & 1u) {
-            const uint32_t at = lds_pos(j);
-            cv<R> v = tp[at];
-            v.x = -v.x;
-            v.y = -v.y;
-            tp[at] = v;
-            if (tl) {
-                cv<R> w = tl[at];
-                w.x = -w.x;
-                w.y = -w.y;
-                tl[at] = w;
-            }
-        }
+    if (cz) {
+        for (uint32_t j = tid; j < TS; j += T) dst[pass_global(d, t, j)] = negate_if<R>(src[lds_pos(j)], cz_odd(d, z, j));
+    } else {
+        for (uint32_t j = tid; j < TS; j += T) dst[pass_global(d, t, j)] = src[lds_pos(j)];
     }
 }
 
@@ -319,7 +327,7 @@ __device__ __forceinline__ void fwd_group(cv<R>* tile, uint32_t TS, int t0, cons
 #pragma unroll
     for (int i = 0; i < G; ++i)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) u[i][e] = (R)sg[i].u[0][e];
+        for (int e = 0; e < 8; ++e) u[i][e] = (R)sg[i].u[e];
     for (uint32_t s = tid; s < cubes; s += T) {
         const uint32_t base = ((s & ~lo) << G) | (s & lo);
         cv<R> x[D];
@@ -344,7 +352,7 @@ __device__ __forceinline__ void fwd_group(cv<R>* tile, uint32_t TS, int t0, cons
 // on the host.  Products are summed over a sub-cube in the state's precision,
 // then into FP64 per-thread sums.
 template <typename R, int G>
-__device__ __forceinline__ void sub_overlaps(const cv<R> (&x)[1 << G], const cv<R> (&y)[1 << G], double (&acc)[4][3]) {
+__device__ __forceinline__ void sub_overlaps(const cv<R> (&x)[1 << G], const cv<R> (&y)[1 << G], double (&acc)[kGroupRuns][3]) {
     constexpr uint32_t D = 1u << G;
 #pragma unroll
     for (int i = 0; i < G; ++i) {
@@ -369,12 +377,12 @@ __device__ __forceinline__ void load_runs(const PassSeg* __restrict__ sg, R (&u)
 #pragma unroll
     for (int i = 0; i < G; ++i)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) u[i][e] = (R)sg[i].u[0][e];
+        for (int e = 0; e < 8; ++e) u[i][e] = (R)sg[i].u[e];
 }
 
 template <typename R, int G>
 __device__ __forceinline__ void rev_group(cv<R>* tp, cv<R>* tl, uint32_t TS, int t0, const PassSeg* __restrict__ sg,
-                                          double (&acc)[4][3]) {
+                                          double (&acc)[kGroupRuns][3]) {
     constexpr uint32_t D = 1u << G;
     const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
     R u[G][8];
@@ -407,7 +415,7 @@ __device__ __forceinline__ void rev_group(cv<R>* tp, cv<R>* tl, uint32_t TS, int
 // Overlaps of a group's runs only: psi and lambda are read, not changed.
 template <typename R, int G>
 __device__ __forceinline__ void rev_overlaps(const cv<R>* tp, const cv<R>* tl, uint32_t TS, int t0,
-                                             double (&acc)[4][3]) {
+                                             double (&acc)[kGroupRuns][3]) {
     constexpr uint32_t D = 1u << G;
     const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
     for (uint32_t s = tid; s < cubes; s += T) {
@@ -444,6 +452,16 @@ __device__ __forceinline__ void lam_back(cv<R>* tl, uint32_t TS, int t0, const P
     }
 }
 
+// Group width dispatch (register groups hold up to 4 complex64 or 3
+// complex128 qubits; the complex128 instantiation of 4 is never run).
+#define IQ_GROUP_SWITCH(gs, CALL)                                          \
+    switch (gs) {                                                          \
+        case 1: CALL(1); break;                                            \
+        case 2: CALL(2); break;                                            \
+        case 3: CALL(3); break;                                            \
+        default: if (sizeof(R) == 4) CALL(sizeof(R) == 4 ? 4 : 3); break;  \
+    }
+
 template <typename R>
 __global__ void __launch_bounds__(512) k_tile_fwd(const cv<R>* __restrict__ src, cv<R>* __restrict__ dst, PassDesc d,
                                                   const PassSeg* __restrict__ segs) {
@@ -459,38 +477,23 @@ __global__ void __launch_bounds__(512) k_tile_fwd(const cv<R>* __restrict__ src,
             const int g = __builtin_amdgcn_readfirstlane(gi);
             const int t0 = d.gt0[g];
             const PassSeg* sg = ps + d.gfirst[g];
-            switch (d.gsize[g]) {
-                case 1: fwd_group<R, 1>(tile, TS, t0, sg); break;
-                case 2: fwd_group<R, 2>(tile, TS, t0, sg); break;
-                case 3: fwd_group<R, 3>(tile, TS, t0, sg); break;
-                default: if (sizeof(R) == 4) fwd_group<R, sizeof(R) == 4 ? 4 : 3>(tile, TS, t0, sg); break;
-            }
+#define IQ_CALL(G) fwd_group<R, G>(tile, TS, t0, sg)
+            IQ_GROUP_SWITCH(d.gsize[g], IQ_CALL)
+#undef IQ_CALL
             __syncthreads();
         }
-        if (d.ncz) {
-            cz_tile<R>(d, t, tile, (cv<R>*)nullptr, TS);
-            __syncthreads();
-        }
-        tile_out<R>(d, t, tile, dst, TS);
+        // the stage's CZ set comes last: its signs go on with the store
+        tile_out<R>(d, t, tile, dst, TS, d.ncz != 0, d.ncz ? cz_setup(d, t) : CzTile{0, 0});
         __syncthreads();
     }
 }
 
-// Group width dispatch (register groups hold up to 4 complex64 or 3
-// complex128 qubits).
-#define IQ_GROUP_SWITCH(gs, CALL)                               \
-    switch (gs) {                                               \
-        case 1: CALL(1); break;                                 \
-        case 2: CALL(2); break;                                 \
-        case 3: CALL(3); break;                                 \
-        default: if (sizeof(R) == 4) CALL(sizeof(R) == 4 ? 4 : 3); break; \
-    }
 
 template <typename R>
-__device__ __forceinline__ void flush_overlaps(const double (&acc)[4][3], int gs, double* wred, int wave, int lane,
+__device__ __forceinline__ void flush_overlaps(const double (&acc)[kGroupRuns][3], int gs, double* wred, int wave, int lane,
                                                int first) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < kGroupRuns; ++i) {
         if (i >= gs) continue;
 #pragma unroll
         for (int st = 0; st < 3; ++st) {
@@ -523,13 +526,11 @@ __global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>
     __syncthreads();
     const PassSeg* ps = segs + d.first;
     for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
-        tile_in<R>(d, t, psi, tp, TS);
-        tile_in<R>(d, t, lam, tl, TS);
+        // the CZ set is undone first: its signs go on with the loads
+        const CzTile z = d.ncz ? cz_setup(d, t) : CzTile{0, 0};
+        tile_in<R>(d, t, psi, tp, TS, d.ncz != 0, z);
+        tile_in<R>(d, t, lam, tl, TS, d.ncz != 0, z);
         __syncthreads();
-        if (d.ncz) {
-            cz_tile<R>(d, t, tp, tl, TS);
-            __syncthreads();
-        }
         if (STORED && d.gundo == 0) {
             // no qubit repeats in the pass: every group's overlaps at the
             // pass's end (psi as stored), then lambda back through the groups
@@ -537,7 +538,7 @@ __global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>
                 // a wave-uniform index: the descriptor reads below become scalar loads
                 const int g = __builtin_amdgcn_readfirstlane(gi);
                 const int t0 = d.gt0[g], gs = d.gsize[g];
-                double acc[4][3] = {};
+                double acc[kGroupRuns][3] = {};
 #define IQ_CALL(G) rev_overlaps<R, G>(tp, tl, TS, t0, acc)
                 IQ_GROUP_SWITCH(gs, IQ_CALL)
 #undef IQ_CALL
@@ -560,7 +561,7 @@ __global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>
                 const int g = __builtin_amdgcn_readfirstlane(gi);
                 const int t0 = d.gt0[g], gs = d.gsize[g];
                 const PassSeg* sg = ps + d.gfirst[g];
-                double acc[4][3] = {};
+                double acc[kGroupRuns][3] = {};
 #define IQ_CALL(G) rev_group<R, G>(tp, tl, TS, t0, sg, acc)
                 IQ_GROUP_SWITCH(gs, IQ_CALL)
 #undef IQ_CALL
@@ -1186,7 +1187,7 @@ int local_bit(const PassDesc& d, int q) {
 }
 
 // Files a CZ set into the pass descriptor by where each edge's qubits sit
-// (the split cz_tile evaluates).  A repeated edge cancels: masks combine by XOR.
+// (the split cz_odd evaluates).  A repeated edge cancels: masks combine by XOR.
 void set_cz(PassDesc& d, const std::vector<std::pair<int, int>>& edges) {
     d.ncz = (int)edges.size();
     d.ncross = d.nfixed = d.nshift = 0;
@@ -1263,10 +1264,9 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
         for (const auto& sub : sg.subs) m = mul(sub_matrix(c.gates[sub.gate], sub.which, params), m);
         if (reverse) m = dag(m);
         for (int i = 0; i < 4; ++i) {
-            ps.u[0][2 * i] = m.m[i].real();
-            ps.u[0][2 * i + 1] = m.m[i].imag();
+            ps.u[2 * i] = m.m[i].real();
+            ps.u[2 * i + 1] = m.m[i].imag();
         }
-        ps.count = 1;
         segv.push_back(ps);
     };
     std::vector<Launch> fwd;  // in forward order; reversed at the end for the sweep
@@ -1281,7 +1281,7 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
             continue;
         }
         int last = -1;  // index in fwd of the stage's last pass
-        const int gmax = c.prec == kC64 ? 4 : 3;
+        const int gmax = c.prec == kC64 ? kGroupRuns : 3;
         for (int k = 0; k < npass; ++k) {
             const PassDesc dk = pass_desc(g, c.n, k);
             std::vector<int> mine;
@@ -1289,15 +1289,29 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
                 if (pass_of(g, c.n, c.segs[ri].q) == k) mine.push_back(ri);
             if (mine.empty()) continue;
             // runs on different qubits commute: sort by tile bit (stable, so a
-            // qubit's own runs keep their order) and cut into groups of up to
-            // gmax consecutive tile bits
+            // qubit's own runs keep their order); cut the chains of consecutive
+            // tile bits (one run each) into groups of at most gmax, of about
+            // equal size (a lone short group costs a whole LDS round trip)
             std::stable_sort(mine.begin(), mine.end(), [&](int a, int b2) {
                 return tile_bit(g, dk, k, c.segs[a].q) < tile_bit(g, dk, k, c.segs[b2].q);
             });
+            auto tb_of = [&](int ri) { return tile_bit(g, dk, k, c.segs[ri].q); };
+            std::vector<std::pair<size_t, int>> groups;  // first entry of mine, size
+            for (size_t i = 0; i < mine.size();) {
+                size_t j = i + 1;
+                while (j < mine.size() && tb_of(mine[j]) == tb_of(mine[j - 1]) + 1) ++j;
+                const int len = (int)(j - i), ng = (len + gmax - 1) / gmax;
+                size_t at = i;
+                for (int q = 0; q < ng; ++q) {
+                    const int sz = len / ng + (q < len % ng ? 1 : 0);
+                    groups.push_back({at, sz});
+                    at += sz;
+                }
+                i = j;
+            }
             Launch l;
             l.d = dk;
             std::vector<int> chunk;
-            int last_tb = -2;
             auto emit = [&]() {
                 if (chunk.empty()) return;
                 l.d.nseg = (int)chunk.size();
@@ -1307,27 +1321,14 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
                 l = Launch{};
                 l.d = dk;
                 chunk.clear();
-                last_tb = -2;
             };
-            for (int ri : mine) {
-                const int tb = tile_bit(g, dk, k, c.segs[ri].q);
-                const bool extend = l.d.ngroups > 0 && l.d.gsize[l.d.ngroups - 1] < gmax && tb == last_tb + 1;
-                if (!extend) {
-                    if (l.d.ngroups == kGroupMax || (int)chunk.size() == kPassSegMax) emit();
-                    l.d.gt0[l.d.ngroups] = (uint8_t)tb;
-                    l.d.gsize[l.d.ngroups] = 0;
-                    l.d.gfirst[l.d.ngroups] = (uint8_t)chunk.size();
-                    ++l.d.ngroups;
-                } else if ((int)chunk.size() == kPassSegMax) {
-                    emit();
-                    l.d.gt0[0] = (uint8_t)tb;
-                    l.d.gsize[0] = 0;
-                    l.d.gfirst[0] = 0;
-                    l.d.ngroups = 1;
-                }
-                chunk.push_back(ri);
-                ++l.d.gsize[l.d.ngroups - 1];
-                last_tb = tb;
+            for (const auto& gr : groups) {
+                if (l.d.ngroups == kGroupMax || (int)chunk.size() + gr.second > kPassSegMax) emit();
+                l.d.gt0[l.d.ngroups] = (uint8_t)tb_of(mine[gr.first]);
+                l.d.gsize[l.d.ngroups] = (uint8_t)gr.second;
+                l.d.gfirst[l.d.ngroups] = (uint8_t)chunk.size();
+                ++l.d.ngroups;
+                for (int q = 0; q < gr.second; ++q) chunk.push_back(mine[gr.first + q]);
             }
             emit();
         }
