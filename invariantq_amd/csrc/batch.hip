@@ -597,6 +597,7 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
     std::vector<int64_t> off(ncirc);
     int64_t total = 0;
     int nmax = 0;
+    int bused = 6;  // widest tile of the batch: the LDS each workgroup needs
     for (size_t c = 0; c < ncirc; ++c) {
         if (nq[c] < 2 || nq[c] > 30 || layers[c] < 1 || 3 * nq[c] > kMaxSlots) {
             set_error(kErrInvalidArgument, "circuit %zu: unsupported shape (n=%d, layers=%d)", c, nq[c], layers[c]);
@@ -610,6 +611,7 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
             return -1;
         }
         if (tb < 6) tb = 6;
+        bused = std::max(bused, std::min(tb, nq[c] > tb ? tb : std::max(nq[c], 6)));
         cs[c] = BatchCircuit{nq[c], layers[c], tb, 0, total, (int64_t)c};
         off[c] = total;
         total += 2LL * nq[c] * layers[c];
@@ -626,8 +628,15 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
     int dev = 0, cus = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int nblk = num_blocks > 0 ? num_blocks : std::max(1, cus);
     const size_t elem = B.prec == kC128 ? 16 : 8;
+    // LDS for the widest tile in use (psi and lambda) plus the per-wave
+    // gradient slots; as many workgroups as fit a CU at that size run at once
+    const size_t lds = 2 * ((size_t)1 << bused) * elem + (size_t)(T / 64) * kMaxSlots * sizeof(double);
+    const void* kern = B.prec == kC128 ? (const void*)k_batch_hea<double> : (const void*)k_batch_hea<float>;
+    hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, T, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    const int nblk = num_blocks > 0 ? num_blocks : std::max(1, cus * per_cu);
     const uint64_t scratch_elems = nmax ? (1ULL << nmax) : 1;
     if (!grow(reinterpret_cast<void**>(&B.circs), B.circ_cap, ncirc * sizeof(BatchCircuit)) ||
         !grow(reinterpret_cast<void**>(&B.params), B.param_cap, (size_t)total * 8) ||
@@ -649,20 +658,16 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
     else
         hipLaunchKernelGGL(k_prepare_rots<float>, dim3((unsigned)ncirc), dim3(256), 0, st, B.circs, B.params,
                            (float*)B.rots);
-    const size_t lds = 2 * ((size_t)1 << bmax) * elem + (size_t)(T / 64) * kMaxSlots * sizeof(double);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     hipEventRecord(e0, st);
-    if (B.prec == kC128) {
-        hipFuncSetAttribute((const void*)k_batch_hea<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (B.prec == kC128)
         hipLaunchKernelGGL(k_batch_hea<double>, dim3(nblk), dim3(T), lds, st, B.circs, (int)ncirc, (const double*)B.rots, B.params,
-                           B.grads, B.energies, (double2*)B.scratch, scratch_elems, B.queue, hx, bmax);
-    } else {
-        hipFuncSetAttribute((const void*)k_batch_hea<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                           B.grads, B.energies, (double2*)B.scratch, scratch_elems, B.queue, hx, bused);
+    else
         hipLaunchKernelGGL(k_batch_hea<float>, dim3(nblk), dim3(T), lds, st, B.circs, (int)ncirc, (const float*)B.rots, B.params,
-                           B.grads, B.energies, (float2*)B.scratch, scratch_elems, B.queue, hx, bmax);
-    }
+                           B.grads, B.energies, (float2*)B.scratch, scratch_elems, B.queue, hx, bused);
     if (!hip_ok(hipGetLastError(), "batch launch")) return -1;
     hipEventRecord(e1, st);
     std::vector<double> en(ncirc);
