@@ -556,11 +556,12 @@ def main():
     value = bytes_block * a.blocks / wall / 1e12
     mean_k = np.array([np.mean(x) for x in per_kernel])
     low = np.array([q[-1] < 10 for q in targets])
-    # The launcher streams (non-temporal loads/stores) unless three or more
-    # targets sit inside one 128-byte line (qubits 0-2 for complex128); the two
-    # cases are two instantiations of the kernel, listed apart by rocprofv3.
+    # The launcher streams (non-temporal loads/stores, two batches per
+    # iteration) unless three or more targets sit inside one 128-byte line
+    # (qubits 0-2 for complex128: four batches, no streaming); the two cases are
+    # two instantiations of the kernel, listed apart by rocprofv3.
     streaming = np.array([sum(t < 3 for t in q) <= 2 for q in targets])
-    kinds = {"k_gate_mfma<double, 4, false, true>": streaming,
+    kinds = {"k_gate_mfma<double, 2, false, true>": streaming,
              "k_gate_mfma<double, 4, false, false>": ~streaming,
              "k_gate_mfma, blocks on qubits 0-9": low, "k_gate_mfma, unrestricted blocks": ~low}
     kstats = {}

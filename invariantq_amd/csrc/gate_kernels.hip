@@ -377,13 +377,34 @@ inline bool mfma_whole_lines(const MfmaArgs& a, bool f64) {
     return low <= 2;
 }
 
-// Visit order of super-batches.  Measured through the engine on 200 random
-// 5-qubit target sets at 30 qubits (dense unitary blocks, same GPU): for
-// complex128, when a target lies at qubit 10 or above, waves that run together
-// do better taking super-batches four apart (rotation 2: 5.32 vs 5.19 TB/s on
-// those sets), while sets within qubits 0-9 lose 8% with it.  complex64 loses
-// with any rotation (4.75 vs 5.28 TB/s), so it keeps the plain order.
-inline uint32_t mfma_rotation(const MfmaArgs& a, bool f64) { return (f64 && a.pos[4] >= 10) ? 2u : 0u; }
+// Launch defaults per target set, from sweeps through the engine at 30
+// qubits (dense Haar blocks on the same GPU).  complex128 sets whose
+// instructions cover whole lines (the streaming case, 191 of the benchmark's
+// 200 blocks): two batches per iteration, 524288 waves, and waves that run
+// together taking super-batches two apart (rotation 1).  That gives 5.50 /
+// 5.86 TB/s on unrestricted / qubit 0-9 sets, against 5.29 / 5.70 with four
+// batches, 131072 waves and the rotation below; the fewer VGPRs per wave let
+// more waves share a SIMD.  Sets with three or more targets inside a line
+// (not streamed) measured 6.34 ms with four batches, 131072 waves, rotation 2
+// when a target lies at qubit 10 or above and 0 otherwise, against 7.09 ms
+// with the streaming defaults, so they keep those.  complex64: four batches,
+// 262144 waves, no rotation (any rotation lost, 4.75 vs 5.28 TB/s).  Only
+// large states were swept, so the rotation applies from 2^16 super-batches.
+struct MfmaDefaults {
+    int batches;
+    uint64_t waves;
+};
+
+inline MfmaDefaults mfma_defaults(const MfmaArgs& a, bool f64) {
+    if (!f64) return {4, 262144};
+    return mfma_whole_lines(a, f64) ? MfmaDefaults{2, 524288} : MfmaDefaults{4, 131072};
+}
+
+inline uint32_t mfma_rotation(const MfmaArgs& a, bool f64, uint64_t nsuper) {
+    if (!f64 || nsuper < (1ULL << 16)) return 0u;
+    if (mfma_whole_lines(a, f64)) return 1u;
+    return a.pos[4] >= 10 ? 2u : 0u;
+}
 
 template <typename R, int UB, bool WLDS>
 hipError_t launch_mfma(void* psi, const MfmaArgs& args, hipStream_t st, const void* dmat) {
@@ -393,12 +414,11 @@ hipError_t launch_mfma(void* psi, const MfmaArgs& args, hipStream_t st, const vo
     uint32_t bits = 0;
     while ((1ULL << bits) < nsuper) ++bits;
     a.bits = ((1ULL << bits) == nsuper) ? bits : 0;  // a rotation needs a power-of-two count
-    a.rot = g_mfma_rot >= 0 ? (uint32_t)g_mfma_rot : mfma_rotation(a, f64);
+    a.rot = g_mfma_rot >= 0 ? (uint32_t)g_mfma_rot : mfma_rotation(a, f64, nsuper);
     if (a.bits == 0) a.rot = 0;
-    // Grid from the 30-qubit sweep over 200 random target sets: 131072 waves
-    // for c128 and 262144 for c64 (four and two super-batches per wave at 30
-    // qubits).  8192 waves ran 1.5% slower for c128, 65536 the same.
-    const uint64_t target = g_mfma_waves > 0 ? (uint64_t)g_mfma_waves : (f64 ? 131072ULL : 262144ULL);
+    // grid: the per-set default (for the streaming c128 case 262144 waves ran
+    // 1.4% slower and 1048576 5% slower than 524288)
+    const uint64_t target = g_mfma_waves > 0 ? (uint64_t)g_mfma_waves : mfma_defaults(a, f64).waves;
     const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nsuper, target));
     const uint64_t blocks = (waves + 3) / 4;
     if (blocks > 0x7fffffffULL) return hipErrorInvalidValue;
@@ -419,11 +439,9 @@ hipError_t run_mfma(void* psi, int n, hipStream_t st, const GateDesc& g) {
     fill_mfma_args(a, n, g.q, std::is_same<R, double>::value, s_in, s_out);
     if (a.nbatches == 0 || !g.dmat) return hipErrorInvalidValue;
     constexpr bool f64 = std::is_same<R, double>::value;
-    // defaults from the 30-qubit sweep: four batches in flight per wave (1 KB
-    // runs per stream), fragments in VGPRs, for both precisions
-    int ub = g_mfma_batches ? g_mfma_batches : 4;
+    // batches in flight per wave: the per-set default, fragments in VGPRs
+    int ub = g_mfma_batches ? g_mfma_batches : mfma_defaults(a, f64).batches;
     const bool lds = g_mfma_lds >= 0 ? g_mfma_lds != 0 : false;
-    (void)f64;
     while (ub > 1 && a.nbatches % ub) ub >>= 1;
     // small and mid-size states: more waves before more batches per wave,
     // keeping at least 32768 waves in the grid (measured on fused 24-qubit
