@@ -93,7 +93,34 @@ __device__ __forceinline__ void matvec(const R (&xr)[D], const R (&xi)[D], Get m
     }
 }
 
-template <typename R, int K, bool M3>
+// Amplitude load/store with an optional non-temporal (streaming) hint; the
+// builtins need a native vector type, so the pair is viewed as one.
+template <typename R>
+__device__ __forceinline__ cv<R> ld_stream(const cv<R>* p, bool nt) {
+    typedef R nv __attribute__((ext_vector_type(2)));
+    if (!nt) return *p;
+    const nv v = __builtin_nontemporal_load(reinterpret_cast<const nv*>(p));
+    cv<R> o;
+    o.x = v[0];
+    o.y = v[1];
+    return o;
+}
+template <typename R>
+__device__ __forceinline__ void st_stream(cv<R>* p, cv<R> v, bool nt) {
+    typedef R nv __attribute__((ext_vector_type(2)));
+    if (!nt) {
+        *p = v;
+        return;
+    }
+    nv w;
+    w[0] = v.x;
+    w[1] = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<nv*>(p));
+}
+
+// NT: streaming loads and stores, for target sets with no target inside a
+// 128-byte line (each load instruction then covers whole lines).
+template <typename R, int K, bool M3, bool NT = false>
 __global__ void __launch_bounds__(256) k_gate_direct_ptr(cv<R>* __restrict__ psi, uint64_t ngroups, DirectArgs<K> a,
                                                          const R* __restrict__ mat) {
     constexpr int D = 1 << K;
@@ -103,7 +130,7 @@ __global__ void __launch_bounds__(256) k_gate_direct_ptr(cv<R>* __restrict__ psi
     R xr[D], xi[D];
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-        const cv<R> v = psi[base + a.off[s]];
+        const cv<R> v = ld_stream<R>(psi + base + a.off[s], NT);
         xr[s] = v.x;
         xi[s] = v.y;
     }
@@ -113,11 +140,12 @@ __global__ void __launch_bounds__(256) k_gate_direct_ptr(cv<R>* __restrict__ psi
             cv<R> o;
             o.x = yr;
             o.y = yi;
-            psi[base + a.off[r]] = o;
+            st_stream<R>(psi + base + a.off[r], o, NT);
         });
 }
 
-template <typename R, int K, bool M3>
+// NT as in k_gate_direct_ptr.
+template <typename R, int K, bool M3, bool NT = false>
 __global__ void __launch_bounds__(256) k_gate_direct_val(cv<R>* __restrict__ psi, uint64_t ngroups, DirectArgs<K> a,
                                                          ValMat<R, (M3 ? 3 : 2) * (1 << (2 * K))> mat) {
     constexpr int D = 1 << K;
@@ -127,7 +155,7 @@ __global__ void __launch_bounds__(256) k_gate_direct_val(cv<R>* __restrict__ psi
     R xr[D], xi[D];
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-        const cv<R> v = psi[base + a.off[s]];
+        const cv<R> v = ld_stream<R>(psi + base + a.off[s], NT);
         xr[s] = v.x;
         xi[s] = v.y;
     }
@@ -137,7 +165,7 @@ __global__ void __launch_bounds__(256) k_gate_direct_val(cv<R>* __restrict__ psi
             cv<R> o;
             o.x = yr;
             o.y = yi;
-            psi[base + a.off[r]] = o;
+            st_stream<R>(psi + base + a.off[r], o, NT);
         });
 }
 
@@ -202,31 +230,6 @@ struct Mfma<float> {
 // The matrix fragments (48 per lane) stay in VGPRs for the whole kernel, so
 // the VALU only does addressing and the final combination, and memory traffic
 // of one batch overlaps the matrix-core work of another wave.
-// Amplitude load/store with an optional non-temporal (streaming) hint; the
-// builtins need a native vector type, so the pair is viewed as one.
-template <typename R>
-__device__ __forceinline__ cv<R> ld_stream(const cv<R>* p, bool nt) {
-    typedef R nv __attribute__((ext_vector_type(2)));
-    if (!nt) return *p;
-    const nv v = __builtin_nontemporal_load(reinterpret_cast<const nv*>(p));
-    cv<R> o;
-    o.x = v[0];
-    o.y = v[1];
-    return o;
-}
-template <typename R>
-__device__ __forceinline__ void st_stream(cv<R>* p, cv<R> v, bool nt) {
-    typedef R nv __attribute__((ext_vector_type(2)));
-    if (!nt) {
-        *p = v;
-        return;
-    }
-    nv w;
-    w[0] = v.x;
-    w[1] = v.y;
-    __builtin_nontemporal_store(w, reinterpret_cast<nv*>(p));
-}
-
 template <typename R, int UB, bool WLDS, bool NT>
 __global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, MfmaArgs a, const R* __restrict__ frag) {
     using M = Mfma<R>;
@@ -330,6 +333,8 @@ inline unsigned grid_for(uint64_t items, unsigned block) {
     return (unsigned)std::max<uint64_t>(1, g);
 }
 
+static int g_direct_nt = 1;  // streaming hint in the direct kernel: 1 where lines are whole, 0 never
+
 template <typename R, int K, bool M3>
 hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
     DirectArgs<K> a;
@@ -337,13 +342,28 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
     const uint64_t ngroups = 1ULL << (n - K);
     const unsigned blocks = grid_for(ngroups, 256);
     constexpr int D = 1 << K;
+    // streaming when no target sits inside a 128-byte line and the state is
+    // well beyond the caches (measured on a 14-qubit density matrix, 2^28
+    // complex128 entries: one-qubit Kraus channels 5.43 against 5.27 TB/s,
+    // one-qubit unitaries 5.65 against 5.45)
+    const uint32_t line_bits = std::is_same<R, double>::value ? 3u : 4u;
+    bool whole = n >= 24 && g_direct_nt != 0;
+    for (int m = 0; m < K; ++m) whole = whole && g.q[m] >= line_bits;
     if constexpr (K <= 3) {
         ValMat<R, (M3 ? 3 : 2) * D * D> mv;
         pack_gate_matrix(std::is_same<R, double>::value ? kC128 : kC64, K, M3, g.re, g.im, mv.v);
-        hipLaunchKernelGGL((k_gate_direct_val<R, K, M3>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups, a, mv);
+        if (whole)
+            hipLaunchKernelGGL((k_gate_direct_val<R, K, M3, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups,
+                               a, mv);
+        else
+            hipLaunchKernelGGL((k_gate_direct_val<R, K, M3, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups,
+                               a, mv);
+    } else if (whole) {
+        hipLaunchKernelGGL((k_gate_direct_ptr<R, K, M3, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups,
+                           a, (const R*)g.dmat);
     } else {
-        hipLaunchKernelGGL((k_gate_direct_ptr<R, K, M3>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups, a,
-                           (const R*)g.dmat);
+        hipLaunchKernelGGL((k_gate_direct_ptr<R, K, M3, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups,
+                           a, (const R*)g.dmat);
     }
     return hipGetLastError();
 }
@@ -501,12 +521,14 @@ hipError_t run_diag(void* psi, int n, hipStream_t st, const uint32_t* q, const d
 bool use_three_mult(int K) { return K >= 4; }
 
 // Matrix-core kernel options chosen by the autotuner (or left at the defaults).
+// `nontemporal` 0 also keeps the direct kernel off the streaming hint.
 void set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontemporal, int grid_waves,
                       int batch_rotation) {
     g_mfma_batches = (batches_per_iteration == 1 || batches_per_iteration == 2 || batches_per_iteration == 4)
                          ? batches_per_iteration : 0;
     g_mfma_lds = (matrix_in_lds == 0 || matrix_in_lds == 1) ? matrix_in_lds : -1;
     g_mfma_nt = (nontemporal == 0 || nontemporal == 1) ? nontemporal : -1;
+    g_direct_nt = nontemporal == 0 ? 0 : 1;
     g_mfma_waves = grid_waves > 0 ? grid_waves : 0;
     g_mfma_rot = (batch_rotation >= 0 && batch_rotation < 32) ? batch_rotation : -1;
 }
