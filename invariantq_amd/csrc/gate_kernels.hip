@@ -24,6 +24,7 @@
 
 #include "kernels.hpp"
 #include "layout.hpp"
+#include "memio.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -91,31 +92,6 @@ __device__ __forceinline__ void matvec(const R (&xr)[D], const R (&xi)[D], Get m
             put(r, yra + yrb, yia + yib);
         }
     }
-}
-
-// Amplitude load/store with an optional non-temporal (streaming) hint; the
-// builtins need a native vector type, so the pair is viewed as one.
-template <typename R>
-__device__ __forceinline__ cv<R> ld_stream(const cv<R>* p, bool nt) {
-    typedef R nv __attribute__((ext_vector_type(2)));
-    if (!nt) return *p;
-    const nv v = __builtin_nontemporal_load(reinterpret_cast<const nv*>(p));
-    cv<R> o;
-    o.x = v[0];
-    o.y = v[1];
-    return o;
-}
-template <typename R>
-__device__ __forceinline__ void st_stream(cv<R>* p, cv<R> v, bool nt) {
-    typedef R nv __attribute__((ext_vector_type(2)));
-    if (!nt) {
-        *p = v;
-        return;
-    }
-    nv w;
-    w[0] = v.x;
-    w[1] = v.y;
-    __builtin_nontemporal_store(w, reinterpret_cast<nv*>(p));
 }
 
 // NT: streaming loads and stores, for target sets with no target inside a

@@ -1,0 +1,40 @@
+// Amplitude loads and stores with an optional non-temporal (streaming) hint.
+// A pass that touches each amplitude of a large state once gains nothing from
+// keeping it in L2, and caching it evicts lines still in flight; the hint is
+// used only where every instruction covers whole 128-byte lines.  The
+// builtins need a native vector type, so the pair is viewed as one.
+#pragma once
+
+#include "common.hpp"
+
+namespace iqg {
+
+template <typename R>
+__device__ __forceinline__ cv<R> ld_stream(const cv<R>* p, bool nt) {
+    typedef R nv __attribute__((ext_vector_type(2)));
+    if (!nt) return *p;
+    const nv v = __builtin_nontemporal_load(reinterpret_cast<const nv*>(p));
+    cv<R> o;
+    o.x = v[0];
+    o.y = v[1];
+    return o;
+}
+
+template <typename R>
+__device__ __forceinline__ void st_stream(cv<R>* p, cv<R> v, bool nt) {
+    typedef R nv __attribute__((ext_vector_type(2)));
+    if (!nt) {
+        *p = v;
+        return;
+    }
+    nv w;
+    w[0] = v.x;
+    w[1] = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<nv*>(p));
+}
+
+// States from this many amplitudes up are far beyond L2 and the Infinity
+// Cache, where whole-state streaming passes use the hint.
+constexpr uint64_t kStreamMinCount = 1ULL << 26;
+
+}  // namespace iqg

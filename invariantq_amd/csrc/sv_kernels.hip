@@ -5,6 +5,7 @@
 // order: results are bit-reproducible run to run (no atomics on the result path).
 
 #include "kernels.hpp"
+#include "memio.hpp"
 #include "reduce.hpp"
 
 #include <algorithm>
@@ -19,45 +20,73 @@ inline unsigned blocks_for(uint64_t items, unsigned block, uint64_t cap) {
     return (unsigned)g;
 }
 
-template <typename R>
-__global__ void __launch_bounds__(256) k_init_basis(cv<R>* __restrict__ psi, uint64_t count, uint64_t index) {
+// Whole-state passes: a grid-stride loop with four independent elements per
+// thread and iteration (four loads in flight), then a tail of single ones.
+// NT adds the streaming hint (states of kStreamMinCount amplitudes and up).
+template <int U, typename Body>
+__device__ __forceinline__ void grid_stride(uint64_t count, Body&& body) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        cv<R> v;
-        v.x = (i == index) ? (R)1 : (R)0;
-        v.y = 0;
-        psi[i] = v;
-    }
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (U - 1) * stride < count; i += U * stride) body(i, stride, U);
+    for (; i < count; i += stride) body(i, stride, 1);
 }
 
-template <typename R>
+template <typename R, bool NT>
+__global__ void __launch_bounds__(256) k_init_basis(cv<R>* __restrict__ psi, uint64_t count, uint64_t index) {
+    grid_stride<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u >= n) break;
+            const uint64_t j = i + u * stride;
+            cv<R> v;
+            v.x = (j == index) ? (R)1 : (R)0;
+            v.y = 0;
+            st_stream<R>(psi + j, v, NT);
+        }
+    });
+}
+
+template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_collapse(cv<R>* __restrict__ psi, uint64_t count, int target, int outcome,
                                                   double factor) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const R f = (R)factor;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        cv<R> v = psi[i];
-        if ((int)((i >> target) & 1ULL) == outcome) {
-            v.x *= f;
-            v.y *= f;
-        } else {
-            v.x = 0;
-            v.y = 0;
+    grid_stride<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+        cv<R> v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (u < n) v[u] = ld_stream<R>(psi + i + u * stride, NT);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u >= n) break;
+            const uint64_t j = i + u * stride;
+            if ((int)((j >> target) & 1ULL) == outcome) {
+                v[u].x *= f;
+                v[u].y *= f;
+            } else {
+                v[u].x = 0;
+                v[u].y = 0;
+            }
+            st_stream<R>(psi + j, v[u], NT);
         }
-        psi[i] = v;
-    }
+    });
 }
 
-template <typename R>
+template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_scale(cv<R>* __restrict__ psi, uint64_t count, double factor) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const R f = (R)factor;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        cv<R> v = psi[i];
-        v.x *= f;
-        v.y *= f;
-        psi[i] = v;
-    }
+    grid_stride<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+        cv<R> v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (u < n) v[u] = ld_stream<R>(psi + i + u * stride, NT);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u >= n) break;
+            v[u].x *= f;
+            v[u].y *= f;
+            st_stream<R>(psi + i + u * stride, v[u], NT);
+        }
+    });
 }
 
 template <typename R>
@@ -111,15 +140,21 @@ __global__ void __launch_bounds__(256) k_finalize(const double* __restrict__ par
     }
 }
 
-template <typename R>
+template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_red_norm(const cv<R>* __restrict__ psi, uint64_t count, double* __restrict__ part) {
     double v[1] = {0.0};
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        const cv<R> a = psi[i];
-        const double x = (double)a.x, y = (double)a.y;
-        v[0] += x * x + y * y;
-    }
+    grid_stride<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+        cv<R> a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (u < n) a[u] = ld_stream<R>(psi + i + u * stride, NT);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {  // in element order, as a single-element loop would add them
+            if (u >= n) break;
+            const double x = (double)a[u].x, y = (double)a[u].y;
+            v[0] += x * x + y * y;
+        }
+    });
     block_sum_store<1>(v, 1, part);
 }
 
@@ -404,11 +439,29 @@ __global__ void __launch_bounds__(256) k_upload(const double2* __restrict__ src,
 
 }  // namespace
 
+// One launcher for the whole-state element-wise kernels: cached or streaming
+// instantiation by state size, complex128 or complex64 by precision.
+#define IQ_LAUNCH_PASS(KERNEL, count, prec, blocks, st, ptr_cast, ...)                                         \
+    do {                                                                                                      \
+        const bool nt_ = (count) >= kStreamMinCount;                                                          \
+        if ((prec) == kC128) {                                                                                \
+            if (nt_) hipLaunchKernelGGL((KERNEL<double, true>), dim3(blocks), dim3(256), 0, st, (ptr_cast<double>)__VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<double, false>), dim3(blocks), dim3(256), 0, st, (ptr_cast<double>)__VA_ARGS__); \
+        } else {                                                                                              \
+            if (nt_) hipLaunchKernelGGL((KERNEL<float, true>), dim3(blocks), dim3(256), 0, st, (ptr_cast<float>)__VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<float, false>), dim3(blocks), dim3(256), 0, st, (ptr_cast<float>)__VA_ARGS__); \
+        }                                                                                                     \
+    } while (0)
+
+template <typename R>
+using amp_ptr = cv<R>*;
+template <typename R>
+using camp_ptr = const cv<R>*;
+
 hipError_t launch_init_basis(void* psi, int n, int prec, uint64_t index, hipStream_t st) {
     const uint64_t count = 1ULL << n;
     const unsigned b = blocks_for(count, 256, 65536);
-    if (prec == kC128) hipLaunchKernelGGL(k_init_basis<double>, dim3(b), dim3(256), 0, st, (double2*)psi, count, index);
-    else hipLaunchKernelGGL(k_init_basis<float>, dim3(b), dim3(256), 0, st, (float2*)psi, count, index);
+    IQ_LAUNCH_PASS(k_init_basis, count, prec, b, st, amp_ptr, psi, count, index);
     return hipGetLastError();
 }
 
@@ -423,16 +476,13 @@ hipError_t launch_upload(void* dst, const void* src_host, size_t bytes, hipStrea
 hipError_t launch_collapse(void* psi, int n, int prec, int target, int outcome, double factor, hipStream_t st) {
     const uint64_t count = 1ULL << n;
     const unsigned b = blocks_for(count, 256, 65536);
-    if (prec == kC128)
-        hipLaunchKernelGGL(k_collapse<double>, dim3(b), dim3(256), 0, st, (double2*)psi, count, target, outcome, factor);
-    else hipLaunchKernelGGL(k_collapse<float>, dim3(b), dim3(256), 0, st, (float2*)psi, count, target, outcome, factor);
+    IQ_LAUNCH_PASS(k_collapse, count, prec, b, st, amp_ptr, psi, count, target, outcome, factor);
     return hipGetLastError();
 }
 
 hipError_t launch_scale(void* psi, uint64_t count, int prec, double factor, hipStream_t st) {
     const unsigned b = blocks_for(count, 256, 65536);
-    if (prec == kC128) hipLaunchKernelGGL(k_scale<double>, dim3(b), dim3(256), 0, st, (double2*)psi, count, factor);
-    else hipLaunchKernelGGL(k_scale<float>, dim3(b), dim3(256), 0, st, (float2*)psi, count, factor);
+    IQ_LAUNCH_PASS(k_scale, count, prec, b, st, amp_ptr, psi, count, factor);
     return hipGetLastError();
 }
 
@@ -470,8 +520,8 @@ hipError_t launch_finalize(const double* part, unsigned nblocks, int m, double* 
 
 hipError_t reduce_norm(const void* psi, uint64_t count, int prec, double* part, double* out, hipStream_t st) {
     const unsigned b = blocks_for(count, kRedThreads, kRedBlocks);
-    if (prec == kC128) hipLaunchKernelGGL(k_red_norm<double>, dim3(b), dim3(kRedThreads), 0, st, (const double2*)psi, count, part);
-    else hipLaunchKernelGGL(k_red_norm<float>, dim3(b), dim3(kRedThreads), 0, st, (const float2*)psi, count, part);
+    static_assert(kRedThreads == 256, "the pass launcher uses 256 threads");
+    IQ_LAUNCH_PASS(k_red_norm, count, prec, b, st, camp_ptr, psi, count, part);
     return finalize(part, b, 1, out, st);
 }
 
