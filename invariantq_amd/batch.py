@@ -39,6 +39,7 @@ class BatchRunner:
         self.precision = precision
         self.prec_code = binding.PRECISIONS[precision]
         L.iq_clear_error()
+        self._free = L.iq_batch_free  # held here: module globals may be gone when __del__ runs at exit
         self._ptr = L.iq_batch_create(self.prec_code)
         if not self._ptr:
             binding.check_error()
@@ -46,8 +47,8 @@ class BatchRunner:
         self.max_tile_bits = L.iq_batch_max_tile_bits(self.prec_code)
 
     def __del__(self):
-        if getattr(self, "_ptr", None) and binding.HAS_NATIVE_CORE:
-            binding._NATIVE_LIB.iq_batch_free(self._ptr)
+        if getattr(self, "_ptr", None):
+            self._free(self._ptr)
             self._ptr = None
 
     def run(self, specs: Sequence[Tuple[int, int, np.ndarray]], hx: float = 0.5,
