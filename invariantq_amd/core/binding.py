@@ -239,6 +239,7 @@ class NativeStateVector:
             raise ValueError(f"unknown precision {precision!r}")
         self.precision = "complex128" if PRECISIONS[precision] == 0 else "complex64"
         L.iq_clear_error()
+        self._free = L.iq_statevector_free  # held here: module globals may be gone when __del__ runs at exit
         self._ptr = L.iq_statevector_create_precision(num_qubits, PRECISIONS[precision])
         if not self._ptr:
             msg = (L.iq_last_error_message() or b"").decode()
@@ -246,8 +247,8 @@ class NativeStateVector:
             raise MemoryError(f"Failed to allocate StateVector on the GPU: {msg}")
 
     def __del__(self):
-        if getattr(self, "_ptr", None) and HAS_NATIVE_CORE:
-            _NATIVE_LIB.iq_statevector_free(self._ptr)
+        if getattr(self, "_ptr", None):
+            self._free(self._ptr)
             self._ptr = None
 
     def reset(self):
@@ -430,6 +431,7 @@ class NativeDensityMatrix:
         self.dim = 1 << num_qubits
         L = lib()
         L.iq_clear_error()
+        self._free = L.iq_density_matrix_free  # held here, as for the statevector
         self._ptr = L.iq_density_matrix_create(num_qubits)
         if not self._ptr:
             msg = (L.iq_last_error_message() or b"").decode()
@@ -437,8 +439,8 @@ class NativeDensityMatrix:
             raise MemoryError(f"Failed to allocate DensityMatrix on the GPU: {msg}")
 
     def __del__(self):
-        if getattr(self, "_ptr", None) and HAS_NATIVE_CORE:
-            _NATIVE_LIB.iq_density_matrix_free(self._ptr)
+        if getattr(self, "_ptr", None):
+            self._free(self._ptr)
             self._ptr = None
 
     def reset(self):
