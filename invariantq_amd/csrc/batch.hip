@@ -235,6 +235,17 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             }
             __syncthreads();
         };
+        // Tile bits [first, end) cut into groups of at most gm, of about equal
+        // size (a lone one-qubit group costs nearly a full LDS round trip)
+        auto for_groups = [&](int first, int end, int gm, auto&& f) __attribute__((always_inline)) {
+            const int len = end - first, ng = (len + gm - 1) / gm;
+            int t0 = first;
+            for (int k = 0; k < ng; ++k) {
+                const int g = len / ng + (k < len % ng ? 1 : 0);
+                f(t0, g);
+                t0 += g;
+            }
+        };
         // dispatch on the group width; callers pass 1 <= g <= GM
         auto rot_group = [&](cv<R>* tile, int t0, int g, int qoff, int l) __attribute__((always_inline)) {
             static_assert(GM >= 3 && GM <= 4, "group width");
@@ -388,14 +399,14 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         }
         for (int l = 0; l < L; ++l) {
             if (resident) {
-                for (int t0 = 0; t0 < n; t0 += GM) rot_group(tp, t0, min(GM, n - t0), 0, l);
+                for_groups(0, n, GM, [&](int t0, int g) { rot_group(tp, t0, g, 0, l); });
                 cz(0, 0, false);
             } else {
                 tile_pass(1, false, [&](uint64_t) {  // pass A: qubits 0..b-1
-                    for (int t0 = 0; t0 < b; t0 += GM) rot_group(tp, t0, min(GM, b - t0), 0, l);
+                    for_groups(0, b, GM, [&](int t0, int g) { rot_group(tp, t0, g, 0, l); });
                 });
                 tile_pass(2, false, [&](uint64_t t) {  // pass B: qubits b..n-1, then the CZ ring
-                    for (int t0 = b - h; t0 < b; t0 += GM) rot_group(tp, t0, min(GM, b - t0), h, l);
+                    for_groups(b - h, b, GM, [&](int t0, int g) { rot_group(tp, t0, g, h, l); });
                     cz(2, t, false);
                 });
             }
@@ -481,14 +492,14 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             clear_slots();
             if (resident) {
                 cz(0, 0, true);
-                for (int t0 = 0; t0 < n; t0 += GA) adj_group(t0, min(GA, n - t0), 0, l);
+                for_groups(0, n, GA, [&](int t0, int g) { adj_group(t0, g, 0, l); });
             } else {
                 tile_pass(2, true, [&](uint64_t t) {  // pass B': CZ ring, qubits n-1..b
                     cz(2, t, true);
-                    for (int t0 = b - h; t0 < b; t0 += GA) adj_group(t0, min(GA, b - t0), h, l);
+                    for_groups(b - h, b, GA, [&](int t0, int g) { adj_group(t0, g, h, l); });
                 });
                 tile_pass(1, true, [&](uint64_t) {  // pass A': qubits b-1..0
-                    for (int t0 = 0; t0 < b; t0 += GA) adj_group(t0, min(GA, b - t0), 0, l);
+                    for_groups(0, b, GA, [&](int t0, int g) { adj_group(t0, g, 0, l); });
                 });
             }
             flush_slots(l);
