@@ -280,7 +280,10 @@ __global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, Mfma
     }
 }
 
-template <typename R, int K>
+// Diagonal operator: each amplitude times the entry its target bits select.
+// Four amplitudes in flight per thread; NT streams them (large states: each
+// amplitude is touched once).
+template <typename R, int K, bool NT = false>
 __global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t count, DirectArgs<K> a,
                                               ValMat<double, 2 * (1 << K)> d) {
     constexpr int D = 1 << K;
@@ -290,18 +293,26 @@ __global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t 
         sim[threadIdx.x] = d.v[D + threadIdx.x];
     }
     __syncthreads();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    auto one = [&](uint64_t i, cv<R> v) __attribute__((always_inline)) {
         uint32_t loc = 0;
 #pragma unroll
         for (int m = 0; m < K; ++m) loc |= (uint32_t)((i >> a.pos[m]) & 1u) << (K - 1 - m);
-        const cv<R> v = psi[i];
         const R dr = (R)sre[loc], di = (R)sim[loc];
         cv<R> o;
         o.x = dr * v.x - di * v.y;
         o.y = dr * v.y + di * v.x;
-        psi[i] = o;
+        st_stream<R>(psi + i, o, NT);
+    };
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < count; i += 4 * stride) {
+        cv<R> v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = ld_stream<R>(psi + i + u * stride, NT);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) one(i + u * stride, v[u]);
     }
+    for (; i < count; i += stride) one(i, ld_stream<R>(psi + i, NT));
 }
 
 inline unsigned grid_for(uint64_t items, unsigned block) {
@@ -488,11 +499,16 @@ hipError_t run_diag(void* psi, int n, hipStream_t st, const uint32_t* q, const d
     }
     const uint64_t count = 1ULL << n;
     const unsigned blocks = (unsigned)std::min<uint64_t>(grid_for(count, 256), 256ULL * 64ULL);
-    hipLaunchKernelGGL((k_diag<R, K>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d);
+    if (count >= kStreamMinCount && g_direct_nt != 0)
+        hipLaunchKernelGGL((k_diag<R, K, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d);
+    else
+        hipLaunchKernelGGL((k_diag<R, K, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d);
     return hipGetLastError();
 }
 
 }  // namespace
+
+bool stream_hint_enabled() { return g_direct_nt != 0; }
 
 bool use_three_mult(int K) { return K >= 4; }
 
@@ -504,7 +520,7 @@ void set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontempo
                          ? batches_per_iteration : 0;
     g_mfma_lds = (matrix_in_lds == 0 || matrix_in_lds == 1) ? matrix_in_lds : -1;
     g_mfma_nt = (nontemporal == 0 || nontemporal == 1) ? nontemporal : -1;
-    g_direct_nt = nontemporal == 0 ? 0 : 1;
+    g_direct_nt = nontemporal == 0 ? 0 : 1;  // also the whole-state passes and k_diag
     g_mfma_waves = grid_waves > 0 ? grid_waves : 0;
     g_mfma_rot = (batch_rotation >= 0 && batch_rotation < 32) ? batch_rotation : -1;
 }

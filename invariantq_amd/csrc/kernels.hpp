@@ -39,6 +39,9 @@ hipError_t launch_gate(void* psi, int n, int prec, hipStream_t st, const GateDes
 hipError_t launch_diag(void* psi, int n, int prec, hipStream_t st, int K, const uint32_t* q,
                        const double* dre, const double* dim);
 int choose_path(int n, int prec, const GateDesc& g);
+// Streaming (non-temporal) hints in whole-state passes: on unless turned off
+// through the matrix-core options (nontemporal = 0).
+bool stream_hint_enabled();
 
 // State initialisation and element-wise passes.
 hipError_t launch_init_basis(void* psi, int n, int prec, uint64_t index, hipStream_t st);

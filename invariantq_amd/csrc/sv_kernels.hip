@@ -443,7 +443,7 @@ __global__ void __launch_bounds__(256) k_upload(const double2* __restrict__ src,
 // instantiation by state size, complex128 or complex64 by precision.
 #define IQ_LAUNCH_PASS(KERNEL, count, prec, blocks, st, ptr_cast, ...)                                         \
     do {                                                                                                      \
-        const bool nt_ = (count) >= kStreamMinCount;                                                          \
+        const bool nt_ = (count) >= kStreamMinCount && stream_hint_enabled();                                 \
         if ((prec) == kC128) {                                                                                \
             if (nt_) hipLaunchKernelGGL((KERNEL<double, true>), dim3(blocks), dim3(256), 0, st, (ptr_cast<double>)__VA_ARGS__); \
             else hipLaunchKernelGGL((KERNEL<double, false>), dim3(blocks), dim3(256), 0, st, (ptr_cast<double>)__VA_ARGS__); \

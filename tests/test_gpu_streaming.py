@@ -67,3 +67,47 @@ def test_streaming_kernels_bit_exact(oracle_lib, precision):
             else:
                 ref.apply_fused(qs, m)
         assert np.max(np.abs(streamed - ref.get_amplitudes())) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+def test_streaming_diagonal_and_whole_state_passes_bit_exact(oracle_lib, precision):
+    """From 2^26 amplitudes the diagonal-operator kernel and the whole-state
+    passes (basis fill, normalisation) stream too: same bits as cached."""
+    from invariantq_amd.core import binding
+    n = 26
+    rng = np.random.default_rng(26)
+    ph = rng.uniform(0, 2 * np.pi, 6)
+    ops = [([7], haar_unitary(rng, 2)),
+           ([5], np.diag(np.exp(1j * ph[:2]))),
+           ([3, 20], np.diag(np.exp(1j * ph[2:6]))),
+           ([0, 25], haar_unitary(rng, 4))]
+    print(f"streaming diagonal / whole-state passes, n={n} {precision}", flush=True)
+
+    def run():
+        sv = binding.NativeStateVector(n, precision=precision)
+        for qs, m in ops:
+            if len(qs) == 1:
+                sv.apply_1q(qs[0], m)
+            else:
+                sv.apply_2q(qs[0], qs[1], m)
+        sv.normalize()
+        out = sv.get_amplitudes()
+        del sv
+        return out
+
+    try:
+        binding.set_mfma_options()
+        streamed = run()
+        binding.set_mfma_options(nontemporal=0)
+        cached = run()
+    finally:
+        binding.set_mfma_options()
+    assert np.array_equal(streamed, cached)
+    if precision == "complex128":
+        ref = oracle_lib.statevector(n)
+        for qs, m in ops:
+            if len(qs) == 1:
+                ref.apply_1q(qs[0], m)
+            else:
+                ref.apply_2q(qs[0], qs[1], m)
+        assert np.max(np.abs(streamed - ref.get_amplitudes())) < 1e-12
