@@ -498,7 +498,8 @@ hipError_t run_diag(void* psi, int n, hipStream_t st, const uint32_t* q, const d
         d.v[(1 << K) + l] = dim[l];
     }
     const uint64_t count = 1ULL << n;
-    const unsigned blocks = (unsigned)std::min<uint64_t>(grid_for(count, 256), 256ULL * 64ULL);
+    // one thread per four amplitudes: every load of a thread issued at once
+    const unsigned blocks = (unsigned)std::min<uint64_t>(grid_for((count + 3) / 4, 256), 1ULL << 30);
     if (count >= kStreamMinCount && g_direct_nt != 0)
         hipLaunchKernelGGL((k_diag<R, K, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d);
     else

@@ -458,9 +458,13 @@ using amp_ptr = cv<R>*;
 template <typename R>
 using camp_ptr = const cv<R>*;
 
+// element-wise passes: one thread per four amplitudes (every load of a thread
+// in flight at once)
+inline unsigned pass_blocks(uint64_t count) { return blocks_for((count + 3) / 4, 256, 1ULL << 30); }
+
 hipError_t launch_init_basis(void* psi, int n, int prec, uint64_t index, hipStream_t st) {
     const uint64_t count = 1ULL << n;
-    const unsigned b = blocks_for(count, 256, 65536);
+    const unsigned b = pass_blocks(count);
     IQ_LAUNCH_PASS(k_init_basis, count, prec, b, st, amp_ptr, psi, count, index);
     return hipGetLastError();
 }
@@ -475,13 +479,13 @@ hipError_t launch_upload(void* dst, const void* src_host, size_t bytes, hipStrea
 
 hipError_t launch_collapse(void* psi, int n, int prec, int target, int outcome, double factor, hipStream_t st) {
     const uint64_t count = 1ULL << n;
-    const unsigned b = blocks_for(count, 256, 65536);
+    const unsigned b = pass_blocks(count);
     IQ_LAUNCH_PASS(k_collapse, count, prec, b, st, amp_ptr, psi, count, target, outcome, factor);
     return hipGetLastError();
 }
 
 hipError_t launch_scale(void* psi, uint64_t count, int prec, double factor, hipStream_t st) {
-    const unsigned b = blocks_for(count, 256, 65536);
+    const unsigned b = pass_blocks(count);
     IQ_LAUNCH_PASS(k_scale, count, prec, b, st, amp_ptr, psi, count, factor);
     return hipGetLastError();
 }
