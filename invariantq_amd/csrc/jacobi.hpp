@@ -10,10 +10,20 @@
 // right of its diagonal and the column of its first occurrence.  A rotation at
 // (p, q) changes rows p and q and the entries (i, p), (i, q) of the rows
 // above them, so only those are updated, and a row is rescanned only when its
-// recorded maximum sat in a changed column.  The pivots, and therefore the
-// arithmetic and the result, are those of the full scan, bit for bit.
+// recorded maximum sat in a changed column.
+//
+// The moduli are std::abs values (a hypot call, about twenty times the cost
+// of a squared modulus).  hypot is within 1 ulp of |z| and re^2 + im^2 within
+// a few ulps of |z|^2, so an entry whose squared modulus is below
+// (1 - 1e-14) times the row's largest cannot have the row's largest (or a
+// tied) std::abs: a scan computes std::abs only for the entries above that
+// bound.  Rows whose largest squared modulus is below 1e-280, where squares
+// lose precision to underflow, are scanned with std::abs throughout.  The
+// pivots, and therefore the arithmetic and the result, are those of the full
+// scan, bit for bit.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstddef>
@@ -25,12 +35,20 @@ inline std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> 
                                               double tol) {
     using cplx = std::complex<double>;
     auto at = [&](size_t i, size_t j) -> cplx& { return A[i * d + j]; };
+    constexpr double kSlack = 1.0 - 1e-14;  // squared-modulus filter, relative
+    constexpr double kFloor = 1e-280;       // squared moduli below this: no filter
+    auto sq = [](const cplx& z) { return z.real() * z.real() + z.imag() * z.imag(); };
     std::vector<double> rmax(d, 0.0);  // largest |A(i, j)|, j > i
     std::vector<size_t> rarg(d, 0);    // first j attaining it
+    std::vector<double> rsq(d, 0.0);   // squared modulus of A(i, rarg[i])
     auto rescan = [&](size_t i) {
+        double top = 0.0;
+        for (size_t j = i + 1; j < d; ++j) top = std::max(top, sq(at(i, j)));
+        const double bound = top >= kFloor ? top * kSlack : -1.0;
         double best = 0.0;
         size_t arg = i + 1;
         for (size_t j = i + 1; j < d; ++j) {
+            if (sq(at(i, j)) < bound) continue;
             const double v = std::abs(at(i, j));
             if (v > best) {
                 best = v;
@@ -39,6 +57,7 @@ inline std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> 
         }
         rmax[i] = best;
         rarg[i] = arg;
+        rsq[i] = arg < d ? sq(at(i, arg)) : 0.0;
     };
     // row i's entries in columns j1 (and j2) changed: rescan when its recorded
     // maximum sat in one of them, else offer the new values (ties go to the
@@ -49,10 +68,13 @@ inline std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> 
             return;
         }
         auto offer = [&](size_t j) {
+            const double s2 = sq(at(i, j));
+            if (rsq[i] >= kFloor && s2 < rsq[i] * kSlack) return;  // below the recorded maximum
             const double v = std::abs(at(i, j));
             if (v > rmax[i] || (v == rmax[i] && v > 0.0 && j < rarg[i])) {
                 rmax[i] = v;
                 rarg[i] = j;
+                rsq[i] = s2;
             }
         };
         offer(j1);

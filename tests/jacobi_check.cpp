@@ -1,8 +1,12 @@
 // CPU check of invariantq_amd/csrc/jacobi.hpp (row-maximum pivot search)
 // against the oracle's full-scan restatement of the reference's capped
 // Jacobi (core/src/statevector.cpp:333-393), bit for bit, on random Hermitian
-// matrices, rank-deficient density matrices and matrices with exactly tied
-// off-diagonal moduli.  Prints "ok" or the first mismatch.
+// matrices, rank-deficient density matrices, matrices with exactly tied and
+// 1-ulp-apart off-diagonal moduli, matrices whose squared moduli underflow,
+// pairs with equal std::abs but differently ordered squares (the edge of the
+// squared-modulus filter), and a 1024 x 1024 half-cut reduced state.  Prints
+// "ok" or the first mismatch.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -79,8 +83,85 @@ int main() {
             Gm[0] = Gm[d * d - 1] = 0.5;
             Gm[d - 1] = Gm[(d - 1) * d] = 0.5;
             bad += check(Gm, d, iters, 1e-12, "ghz");
-            cases += 4;
+            // near ties: moduli 1 ulp apart, with phases that make hypot round
+            std::vector<std::complex<double>> Nt(d * d, 0.0);
+            const double m0 = 0.25, mags[3] = {std::nextafter(m0, 0.0), m0, std::nextafter(m0, 1.0)};
+            for (size_t i = 0; i < d; ++i)
+                for (size_t j = i; j < d; ++j) {
+                    const double mag = mags[U(g) % 3];
+                    const double c = std::cos(0.1 * U(g) + 0.05), s = std::sin(0.1 * U(g) + 0.05);
+                    std::complex<double> v = U(g) < 2 ? std::complex<double>(mag, 0.0) : std::complex<double>(mag * c, mag * s);
+                    if (i == j) v = {0.5 * U(g), 0.0};
+                    Nt[i * d + j] = v;
+                    Nt[j * d + i] = std::conj(v);
+                }
+            bad += check(Nt, d, iters, 1e-13, "near-ties");
+            // squared moduli below the underflow floor: a scaled random matrix
+            // (every row) and one with only some rows scaled
+            std::vector<std::complex<double>> S(A), M(A);
+            for (size_t i = 0; i < d; ++i)
+                for (size_t j = 0; j < d; ++j) {
+                    S[i * d + j] *= 1e-160;
+                    if ((i | j) & 1) M[i * d + j] *= 1e-150;
+                }
+            bad += check(S, d, iters, 1e-300, "tiny");
+            bad += check(M, d, iters, 1e-300, "mixed-tiny");
+            cases += 7;
         }
+    }
+    {
+        // the squared-modulus filter's edge: two entries of row 0 with equal
+        // std::abs, the later one with the larger re^2 + im^2 (in both the
+        // plain and the fused evaluation).  The full scan takes the earlier.
+        std::uniform_real_distribution<double> ang(0.1, 1.4), mod(0.29, 0.31);
+        int found = 0;
+        for (int attempt = 0; attempt < 100000 && found < 8; ++attempt) {
+            const double r = mod(g), t = ang(g);
+            const std::complex<double> a(r * std::cos(t), r * std::sin(t));
+            const double m = std::abs(a);
+            const std::complex<double> b(m, 0.0);
+            volatile double x2 = a.real() * a.real(), y2 = a.imag() * a.imag(), m2 = m * m;
+            const double sa_plain = x2 + y2, sa_fma = std::fma(a.real(), a.real(), y2);
+            if (std::abs(b) != m || !(sa_plain < m2) || !(sa_fma < m2)) continue;
+            for (size_t d : {3, 16}) {
+                std::vector<std::complex<double>> E(d * d, 0.0);
+                for (size_t i = 0; i < d; ++i)
+                    for (size_t j = i; j < d; ++j) {
+                        std::complex<double> v(0.01 * N(g), i == j ? 0.0 : 0.01 * N(g));
+                        if (i == 0 && j == 1) v = a;
+                        if (i == 0 && j == 2) v = b;
+                        E[i * d + j] = v;
+                        E[j * d + i] = std::conj(v);
+                    }
+                bad += check(E, d, 100, 1e-12, "filter-edge");
+                ++cases;
+            }
+            ++found;
+        }
+        if (found < 8) {
+            printf("filter-edge: only %d pairs found\n", found);
+            return 1;
+        }
+    }
+    {
+        // a 10-qubit half cut of a random 20-qubit state (d = 1024, the
+        // reference's 100-rotation cap)
+        const size_t d = 1024, e = 1024;
+        std::vector<std::complex<double>> psi(d * e), R(d * d);
+        double nrm = 0.0;
+        for (auto& x : psi) {
+            x = {N(g), N(g)};
+            nrm += std::norm(x);
+        }
+        for (auto& x : psi) x /= std::sqrt(nrm);
+        for (size_t i = 0; i < d; ++i)
+            for (size_t j = 0; j < d; ++j) {
+                std::complex<double> a = 0.0;
+                for (size_t r = 0; r < e; ++r) a += psi[i * e + r] * std::conj(psi[j * e + r]);
+                R[i * d + j] = a;
+            }
+        bad += check(R, d, 100, 1e-12, "half-cut-1024");
+        ++cases;
     }
     if (bad) {
         printf("%d of %d cases differ\n", bad, cases);
