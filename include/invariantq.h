@@ -105,12 +105,18 @@ void        iq_clear_error(void);
 const char* iq_backend_name(void);
 /* Gate kernel selection: 0 automatic, 1 direct (group per lane), 2 LDS-tiled, 3 matrix-core (5-qubit blocks). */
 void        iq_set_gate_path(int path);
-/* Matrix-core kernel options: groups-of-16 batches per iteration (1, 2, 4; 0 = default),
- * matrix fragments in LDS (1) or VGPRs (0; -1 = default), non-temporal loads and stores
- * (1) or cached (0; -1 = by target set: on when every instruction covers whole 128-byte
- * lines), waves in the grid (0 = default: 131072 for complex128, 262144 for complex64),
- * rotation of the super-batch visit order in bits (-1 = default: 2 for complex128 when a
- * target is at qubit 10 or above, else 0). */
+/* Matrix-core kernel options. Defaults are per target set; 0 / -1 restores them.
+ * - batches_per_iteration: groups-of-16 batches per wave iteration (1, 2, 4). Default:
+ *   2 for complex128 sets whose instructions cover whole 128-byte lines, else 4.
+ * - matrix_in_lds: matrix fragments in LDS (1) or VGPRs (0; default).
+ * - nontemporal: streaming loads and stores (1) or cached (0). Default: on when every
+ *   instruction covers whole 128-byte lines. 0 also keeps the direct, diagonal and
+ *   whole-state kernels off the hint.
+ * - grid_waves: waves in the grid. Default: 524288 for whole-line complex128 sets,
+ *   131072 for the other complex128 sets, 262144 for complex64.
+ * - batch_rotation: rotation of the super-batch visit order, in bits. Default (from
+ *   2^16 super-batches): 1 for whole-line complex128 sets, 2 for the others with a
+ *   target at qubit 10 or above, else 0. */
 void        iq_set_mfma_options(int batches_per_iteration, int matrix_in_lds, int nontemporal, int grid_waves,
                                 int batch_rotation);
 

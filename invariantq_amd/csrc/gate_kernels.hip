@@ -395,8 +395,12 @@ inline bool mfma_whole_lines(const MfmaArgs& a, bool f64) {
 // (not streamed) measured 6.34 ms with four batches, 131072 waves, rotation 2
 // when a target lies at qubit 10 or above and 0 otherwise, against 7.09 ms
 // with the streaming defaults, so they keep those.  complex64: four batches,
-// 262144 waves, no rotation (any rotation lost, 4.75 vs 5.28 TB/s).  Only
-// large states were swept, so the rotation applies from 2^16 super-batches.
+// 262144 waves, no rotation (any rotation lost, 4.75 vs 5.28 TB/s).  complex64
+// lanes moving two adjacent amplitudes per 16-byte access (the even and odd
+// groups of two batches, where qubit 0 is not a target) halve the memory
+// instructions but measured 5.28 against 5.38 TB/s at every batch count, grid
+// and rotation tried, so each lane moves one.  Only large states were swept,
+// so the rotation applies from 2^16 super-batches.
 struct MfmaDefaults {
     int batches;
     uint64_t waves;
