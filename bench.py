@@ -355,6 +355,101 @@ def bandwidth_workload(binding, torch, reps: int = 5):
     }
 
 
+def primitives_workload(binding, torch, n: int = 30, reps: int = 3):
+    """Each statevector entry point of the C ABI on one 2^n complex128 state
+    (16 GiB at 30 qubits), against its algorithmic bytes (N amplitudes of
+    16 B): gates read and write every amplitude (2 N 16); reductions read it
+    once (N 16); collapse reads the kept half and writes all (1.5 N 16);
+    probabilities read the state and write 8 B each; sampling reads the state
+    and writes the 8 B CDF.  Enqueued calls are timed with events on the
+    handle's stream around `reps` calls; calls that return data with wall time
+    (they synchronize).  Exports are timed through the raw C calls into host
+    buffers touched beforehand, so they include the device-to-host copy."""
+    from invariantq_amd import gates, workloads
+    N, s = 1 << n, 16
+    lib = binding.lib()
+    dp = binding._dp
+    sv = binding.NativeStateVector(n)
+    layer = workloads.u3_layer(n, seed=31)
+    for op in layer:
+        sv.apply_1q(op.qubits[0], op.matrix)
+    st = torch.cuda.ExternalStream(sv.stream)
+    rng = np.random.default_rng(30)
+    u2 = workloads.haar_unitary(rng, 4)
+    u3m = workloads.haar_unitary(rng, 8)
+    u4m = workloads.haar_unitary(rng, 16)
+
+    def dev(f):
+        f()
+        sv.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        for _ in range(reps):
+            f()
+        b.record(st)
+        sv.synchronize()
+        return a.elapsed_time(b) / reps / 1e3
+
+    def wall(f, k=reps):
+        f()
+        sv.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            f()
+        sv.synchronize()  # calls that end with enqueued work (normalize's scale, reset)
+        return (time.perf_counter() - t0) / k
+
+    rows = []
+
+    def row(name, ref, nbytes, t, how):
+        rows.append({"row": name, "reference": ref, "algorithmic_GB": round(nbytes / 1e9, 3),
+                     "ms": round(t * 1e3, 3), "TBps": round(nbytes / t / 1e12, 3),
+                     "pct_hbm_peak": round(100 * nbytes / t / 8e12, 1), "timed": how})
+
+    g1 = gates.u3(0.3, 1.1, 2.0)
+    for t in (0, 10, 29):
+        row(f"a2 apply_1q target {t}", "statevector.cpp:26-47", 2 * N * s, dev(lambda: sv.apply_1q(t, g1)), "events")
+    for q0, q1 in ((0, 1), (5, 17), (28, 29)):
+        row(f"a3 apply_2q dense ({q0},{q1})", "statevector.cpp:49-87", 2 * N * s,
+            dev(lambda: sv.apply_2q(q0, q1, u2)), "events")
+    row("a3 apply_2q CX (3,20)", "statevector.cpp:49-87", 2 * N * s, dev(lambda: sv.apply_2q(3, 20, gates.CX)), "events")
+    row("a3 apply_2q CZ (7,25), diagonal", "statevector.cpp:49-87", 2 * N * s,
+        dev(lambda: sv.apply_2q(7, 25, gates.CZ)), "events")
+    row("a4 apply_fused k=3 (4,12,27)", "statevector.cpp:89-152", 2 * N * s,
+        dev(lambda: sv.apply_fused([4, 12, 27], u3m)), "events")
+    row("a4 apply_fused k=4 (0,2,15,22), LDS-tiled", "statevector.cpp:89-152", 2 * N * s,
+        dev(lambda: sv.apply_fused([0, 2, 15, 22], u4m)), "events")
+    row("a4 apply_fused k=4 (5,9,18,26)", "statevector.cpp:89-152", 2 * N * s,
+        dev(lambda: sv.apply_fused([5, 9, 18, 26], u4m)), "events")
+    row("a5 measure_prob target 17", "statevector.cpp:154-169", N * s, wall(lambda: sv.measure_prob(17, 0)), "wall")
+    row("a7 norm_squared", "statevector.cpp:206-214", N * s, wall(lambda: sv.norm_squared()), "wall")
+    row("a7 normalize", "statevector.cpp:216-224", 3 * N * s, wall(lambda: sv.normalize()), "wall")
+    row("a10 expval_pauli XZY on (2,9,21)", "statevector.cpp:258-291", N * s,
+        wall(lambda: sv.expectation_value_pauli("XZY", [2, 9, 21])), "wall")
+    terms = workloads.tfim_terms(n)
+    row(f"a10 TFIM <H>, {len(terms)} strings in one call (bytes of one read)", "statevector.cpp:258-291", N * s,
+        wall(lambda: sv.expectation_value_pauli_sum(terms)), "wall")
+    shots = 10000
+    row(f"a9 sample {shots} shots", "statevector.cpp:235-256", N * s + 8 * N + 16 * shots,
+        wall(lambda: sv.sample(shots, 42)), "wall")
+    host = np.empty(N, dtype=np.float64)
+    host.fill(0.0)
+    row("a8 get_probabilities (kernel + device-to-host copy)", "api.cpp:85-90, statevector.cpp:226-233",
+        N * s + 8 * N, wall(lambda: lib.iq_statevector_get_probabilities(sv._ptr, dp(host)), 1), "wall")
+    host2 = np.empty(N, dtype=np.float64)
+    host2.fill(0.0)
+    row("a11 get_amplitudes (kernel + device-to-host copy)", "api.cpp:74-83", N * s + 16 * N,
+        wall(lambda: lib.iq_statevector_get_amplitudes(sv._ptr, dp(host), dp(host2)), 1), "wall")
+    del host, host2
+    row("a6 measure target 29 (probability + collapse)", "statevector.cpp:171-190", 2.5 * N * s,
+        wall(lambda: sv.measure(29, 0.3)), "wall")
+    row("a1 reset", "statevector.cpp:9-17, api.cpp:24-28", N * s, wall(lambda: sv.reset()), "wall")
+    binding.check_error()
+    del sv
+    return {"workload": f"statevector entry points on one {n}-qubit complex128 state ({N * s / 2**30:.0f} GiB)",
+            "rows": rows}
+
+
 def density_matrix_workload(binding, torch, cpu: bool):
     """14-qubit density matrix (2^28 complex128 entries): depth-10 U3 + brick CX circuit,
     depolarizing (p=1e-3) on touched qubits after every gate, amplitude (1e-3) and phase
@@ -521,7 +616,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--shift-params", type=int, default=24)
     ap.add_argument("--only-fused", action="store_true", help="skip the c64, adjoint and CPU legs")
-    ap.add_argument("--workload", default="fused", choices=["fused", "circuit20", "dm14", "batch4096", "adjoint24", "bandwidth"],
+    ap.add_argument("--workload", default="fused",
+                    choices=["fused", "circuit20", "dm14", "batch4096", "adjoint24", "bandwidth", "primitives"],
                     help="fused: the headline line; circuit20 / dm14 / batch4096 / adjoint24: the other "
                          "workloads, each printed as one JSON line")
     a = ap.parse_args()
@@ -553,6 +649,9 @@ def main():
         return
     if a.workload == "bandwidth":
         print(json.dumps(bandwidth_workload(binding, torch)), flush=True)
+        return
+    if a.workload == "primitives":
+        print(json.dumps(primitives_workload(binding, torch)), flush=True)
         return
     n = a.qubits
     N = 1 << n

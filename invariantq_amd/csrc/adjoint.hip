@@ -142,10 +142,13 @@ __global__ void __launch_bounds__(256) k_cz_set(cv<R>* __restrict__ psi, cv<R>* 
 // lambda = H psi and E = Re <psi|H psi> in one pass:
 // (H psi)[k] = sum_t c_t phase_t(k ^ x_t) psi[k ^ x_t], phase(j) = i^{nY} (-1)^{popcount(j & (Y|Z))}.
 // ACCUM: the table's part of H psi is added to lambda; the energy partials
-// are this part's share either way.
-template <typename R, bool WRITE, bool ACCUM = false>
+// are this part's share either way.  SKIP (energy only): a term whose
+// right-hand amplitude psi[k ^ x] has |.|^2 < 1e-18 adds nothing, as in the
+// reference's <psi|P|psi> (core/src/statevector.cpp:273-275).
+template <typename R, bool WRITE, bool ACCUM = false, bool SKIP = false>
 __global__ void __launch_bounds__(256) k_apply_h(const cv<R>* __restrict__ psi, cv<R>* __restrict__ lam, uint64_t count,
                                                  ObsTable h, double* __restrict__ part) {
+    static_assert(!(SKIP && WRITE), "the skip applies to the energy only");
     double e[1] = {0.0};
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
@@ -154,6 +157,7 @@ __global__ void __launch_bounds__(256) k_apply_h(const cv<R>* __restrict__ psi, 
             const uint64_t j = k ^ h.xmask[g];
             const cv<R> v = psi[j];
             const double vr = v.x, vi = v.y;
+            if (SKIP && vr * vr + vi * vi < 1e-18) continue;
             double cr = 0.0, ci = 0.0;  // sum of c_t * phase_t(j)
             for (int t = h.start[g]; t < h.start[g + 1]; ++t) {
                 const double sg = (__popcll(j & h.zy[t]) & 1) ? -h.coeff[t] : h.coeff[t];
@@ -588,16 +592,21 @@ __global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>
 // into tile bits (zl) and bits fixed by the tile index (zfix, one parity per
 // tile and term).  Every launch adds its groups' part of H psi to lambda (the
 // first one writes it) and leaves its part of Re <psi|H psi> in `part`.
+// A group's terms come as two runs: [first, first + nre) add to the real
+// part of the phase sum (i^nY = +-1), the rest to the imaginary part (i^nY =
+// +-i); the sign of i^nY is folded into the coefficient.  Each run starts
+// with its terms that have no sign bits on the tile (nre0 and nim0 of them),
+// whose sum is one value per tile.
 struct ObsGroupT {
     uint32_t xl;  // flip mask on tile bits
-    int first, count;  // terms, relative to the pass's first term
+    int first, count, nre;  // terms, relative to the pass's first term
+    int nre0, nim0;
 };
 
 struct ObsTermT {
     uint64_t zfix;  // sign bits outside the tile (global positions)
     uint32_t zl;    // sign bits on tile bits
-    int ny;
-    double coeff;
+    double coeff;   // times the sign of i^nY
 };
 
 constexpr int kObsTermsMax = 2048;  // terms per launch (their per-tile coefficients sit in LDS)
@@ -611,59 +620,223 @@ template <typename R, bool WRITE, bool FIRST>
 __global__ void __launch_bounds__(512) k_obs_tile(const cv<R>* __restrict__ psi, cv<R>* __restrict__ lam, ObsPass op,
                                                   const ObsGroupT* __restrict__ groups,
                                                   const ObsTermT* __restrict__ terms, double* __restrict__ part) {
+    // Each thread holds E tile elements (j = chunk + e T + tid) in registers, so
+    // a term's data (read from LDS) serves E elements: the term loop is
+    // arithmetic on independent elements instead of a chain of table loads.
+    constexpr int E = 8;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double wsum[16];
     const PassDesc& d = op.d;
     const uint32_t T = blockDim.x, tid = threadIdx.x, TS = 1u << d.ts_bits;
     cv<R>* tile = reinterpret_cast<cv<R>*>(smem);
     double* csg = reinterpret_cast<double*>(tile + TS);  // each term's coefficient with its tile sign
+    uint32_t* zls = reinterpret_cast<uint32_t*>(csg + op.tcount);  // each term's sign bits on tile bits
     const ObsGroupT* gp = groups + op.gfirst;
     const ObsTermT* tp = terms + op.tfirst;
+    for (int k = tid; k < op.tcount; k += T) zls[k] = tp[k].zl;
     double e = 0.0;
     for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
         tile_in<R>(d, t, psi, tile, TS);
         const uint64_t base = pass_global(d, t, 0);
         for (int k = tid; k < op.tcount; k += T) csg[k] = (__popcll(base & tp[k].zfix) & 1) ? -tp[k].coeff : tp[k].coeff;
         __syncthreads();
-        for (uint32_t j = tid; j < TS; j += T) {
-            double sr = 0.0, si = 0.0;
+        for (uint32_t c0 = 0; c0 < TS; c0 += E * T) {
+            uint32_t jj[E];
+            double sr[E], si[E];
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                jj[q] = c0 + q * T + tid;
+                sr[q] = 0.0;
+                si[q] = 0.0;
+            }
             for (int gi = 0; gi < op.gcount; ++gi) {
-                // a wave-uniform index: the table reads below become scalar loads
+                // a wave-uniform index: the group table reads below become scalar loads
                 const int g = __builtin_amdgcn_readfirstlane(gi);
-                const uint32_t jp = j ^ gp[g].xl;
-                const cv<R> v = tile[lds_pos(jp)];
-                double cr = 0.0, ci = 0.0;  // sum of c_t * phase_t(partner)
-                const int k0 = gp[g].first, k1 = k0 + gp[g].count;
-                for (int k = k0; k < k1; ++k) {
-                    // the coefficient with its sign bit flipped by the partner's parity
-                    const double c = csg[k];
-                    const int hi = __double2hiint(c) ^ (int)((__popc(jp & tp[k].zl) & 1u) << 31);
-                    const double sc = __hiloint2double(hi, __double2loint(c));
-                    switch (tp[k].ny & 3) {
-                        case 0: cr += sc; break;
-                        case 1: ci += sc; break;
-                        case 2: cr -= sc; break;
-                        default: ci -= sc; break;
+                const uint32_t xl = gp[g].xl;
+                double vr[E], vi[E], cr[E], ci[E];  // partner amplitude; sum of c_t * phase_t(partner)
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    vr[q] = vi[q] = 0.0;
+                    if (jj[q] < TS) {
+                        const cv<R> v = tile[lds_pos(jj[q] ^ xl)];
+                        vr[q] = (double)v.x;
+                        vi[q] = (double)v.y;
                     }
                 }
-                sr += cr * (double)v.x - ci * (double)v.y;
-                si += cr * (double)v.y + ci * (double)v.x;
-            }
-            const cv<R> p = tile[lds_pos(j)];
-            e += (double)p.x * sr + (double)p.y * si;
-            if (WRITE) {
-                const uint64_t at = pass_global(d, t, j);
-                cv<R> o;
-                if (FIRST) {
-                    o.x = (R)sr;
-                    o.y = (R)si;
-                } else {
-                    o = lam[at];
-                    o.x = (R)((double)o.x + sr);
-                    o.y = (R)((double)o.y + si);
+                const int k0 = gp[g].first, km = k0 + gp[g].nre, k1 = k0 + gp[g].count;
+                const int kr = k0 + gp[g].nre0, ki = km + gp[g].nim0;
+                double cr0 = 0.0, ci0 = 0.0;  // the terms without sign bits on the tile
+                for (int k = k0; k < kr; ++k) cr0 += csg[k];
+                for (int k = km; k < ki; ++k) ci0 += csg[k];
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    cr[q] = cr0;
+                    ci[q] = ci0;
                 }
-                lam[at] = o;
+                uint32_t jx[E];
+#pragma unroll
+                for (int q = 0; q < E; ++q) jx[q] = jj[q] ^ xl;
+                // one run of terms into acc: each term is the same for every
+                // lane (scalar registers); its coefficient's sign bit is
+                // flipped by the partner's parity
+                auto run = [&](int ka, int kb, double(&acc)[E]) __attribute__((always_inline)) {
+#pragma unroll 2
+                    for (int k = ka; k < kb; ++k) {
+                        const double c = csg[k];
+                        const uint32_t zl = __builtin_amdgcn_readfirstlane(zls[k]);
+                        const int chi = __builtin_amdgcn_readfirstlane(__double2hiint(c)),
+                                  clo = __builtin_amdgcn_readfirstlane(__double2loint(c));
+#pragma unroll
+                        for (int q = 0; q < E; ++q) {
+                            const int hi = chi ^ (int)((__popc(jx[q] & zl) & 1u) << 31);
+                            acc[q] += __hiloint2double(hi, clo);
+                        }
+                    }
+                };
+                run(kr, km, cr);
+                run(ki, k1, ci);
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    sr[q] += cr[q] * vr[q] - ci[q] * vi[q];
+                    si[q] += cr[q] * vi[q] + ci[q] * vr[q];
+                }
             }
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                if (jj[q] >= TS) continue;
+                const cv<R> p = tile[lds_pos(jj[q])];
+                e += (double)p.x * sr[q] + (double)p.y * si[q];
+                if (WRITE) {
+                    const uint64_t at = pass_global(d, t, jj[q]);
+                    cv<R> o;
+                    if (FIRST) {
+                        o.x = (R)sr[q];
+                        o.y = (R)si[q];
+                    } else {
+                        o = lam[at];
+                        o.x = (R)((double)o.x + sr[q]);
+                        o.y = (R)((double)o.y + si[q]);
+                    }
+                    lam[at] = o;
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+    if ((tid & 63) == 0) wsum[tid >> 6] = e;
+    __syncthreads();
+    if (tid == 0) {
+        double s2 = 0.0;
+        for (uint32_t w = 0; w < (T >> 6); ++w) s2 += wsum[w];
+        part[blockIdx.x] = s2;
+    }
+}
+
+// Energy only: Re sum_i psi_i * S_i with S_i = sum_g f_g(i) conj(psi[i ^ x_g])
+// and f_g(i) = sum_t c_t i^{nY} (-1)^{popcount(i & zy_t)}: the reference's
+// sum over i (core/src/statevector.cpp:265-288) per term.  Summing on the
+// reference's side puts each phase on the element's own index and SKIP
+// (the reference's |psi_i|^2 < 1e-18 test) on the element itself, once per
+// element instead of once per group; a group whose terms have no sign bits on
+// the tile has one phase sum per tile (every single-qubit X of a field term).
+template <typename R, bool SKIP>
+__global__ void __launch_bounds__(512) k_obs_energy_tile(const cv<R>* __restrict__ psi, ObsPass op,
+                                                         const ObsGroupT* __restrict__ groups,
+                                                         const ObsTermT* __restrict__ terms, double* __restrict__ part) {
+    constexpr int E = 4;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double wsum[16];
+    const PassDesc& d = op.d;
+    const uint32_t T = blockDim.x, tid = threadIdx.x, TS = 1u << d.ts_bits;
+    cv<R>* tile = reinterpret_cast<cv<R>*>(smem);
+    double* csg = reinterpret_cast<double*>(tile + TS);
+    uint32_t* zls = reinterpret_cast<uint32_t*>(csg + op.tcount);
+    const ObsGroupT* gp = groups + op.gfirst;
+    const ObsTermT* tp = terms + op.tfirst;
+    for (int k = tid; k < op.tcount; k += T) zls[k] = tp[k].zl;
+    double e = 0.0;
+    for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        tile_in<R>(d, t, psi, tile, TS);
+        const uint64_t base = pass_global(d, t, 0);
+        for (int k = tid; k < op.tcount; k += T) csg[k] = (__popcll(base & tp[k].zfix) & 1) ? -tp[k].coeff : tp[k].coeff;
+        __syncthreads();
+        for (uint32_t c0 = 0; c0 < TS; c0 += E * T) {
+            uint32_t jj[E];
+            double pr[E], pi[E], sr[E], si[E];
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                jj[q] = c0 + q * T + tid;
+                pr[q] = pi[q] = sr[q] = si[q] = 0.0;
+                if (jj[q] < TS) {
+                    const cv<R> v = tile[lds_pos(jj[q])];
+                    pr[q] = (double)v.x;
+                    pi[q] = (double)v.y;
+                    if (SKIP && pr[q] * pr[q] + pi[q] * pi[q] < 1e-18) pr[q] = pi[q] = 0.0;
+                }
+            }
+            for (int gi = 0; gi < op.gcount; ++gi) {
+                const int g = __builtin_amdgcn_readfirstlane(gi);
+                const uint32_t xl = gp[g].xl;
+                const int k0 = gp[g].first, km = k0 + gp[g].nre, k1 = k0 + gp[g].count;
+                double wr[E], wi[E];
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    const cv<R> w = tile[lds_pos((jj[q] ^ xl) & (TS - 1))];
+                    wr[q] = (double)w.x;
+                    wi[q] = (double)w.y;
+                }
+                // the terms without sign bits on the tile: one sum per tile
+                const int kr = k0 + gp[g].nre0, ki = km + gp[g].nim0;
+                double fr0 = 0.0, fi0 = 0.0;
+                for (int k = k0; k < kr; ++k) fr0 += csg[k];
+                for (int k = km; k < ki; ++k) fi0 += csg[k];
+                if (kr == km && ki == k1) {
+                    if (km == k1) {
+#pragma unroll
+                        for (int q = 0; q < E; ++q) {
+                            sr[q] += fr0 * wr[q];
+                            si[q] -= fr0 * wi[q];
+                        }
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < E; ++q) {
+                            sr[q] += fr0 * wr[q] + fi0 * wi[q];
+                            si[q] += fi0 * wr[q] - fr0 * wi[q];
+                        }
+                    }
+                    continue;
+                }
+                double fr[E], fi[E];
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    fr[q] = fr0;
+                    fi[q] = fi0;
+                }
+                auto run = [&](int ka, int kb, double(&acc)[E]) __attribute__((always_inline)) {
+                    for (int k = ka; k < kb; ++k) {
+                        const double c = csg[k];
+                        const uint32_t zl = __builtin_amdgcn_readfirstlane(zls[k]);
+                        const int chi = __builtin_amdgcn_readfirstlane(__double2hiint(c)),
+                                  clo = __builtin_amdgcn_readfirstlane(__double2loint(c));
+#pragma unroll
+                        for (int q = 0; q < E; ++q) {
+                            const int hi = chi ^ (int)((__popc(jj[q] & zl) & 1u) << 31);
+                            acc[q] += __hiloint2double(hi, clo);
+                        }
+                    }
+                };
+                run(kr, km, fr);
+                run(ki, k1, fi);
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    sr[q] += fr[q] * wr[q] + fi[q] * wi[q];
+                    si[q] += fi[q] * wr[q] - fr[q] * wi[q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < E; ++q) e += pr[q] * sr[q] - pi[q] * si[q];
         }
         __syncthreads();
     }
@@ -743,6 +916,39 @@ struct Segment {
     int slot = -1;
 };
 
+// A Pauli sum over tile passes: launches over the passes whose tile bits hold
+// a group's flip mask, then the groups no pass holds through global gathers;
+// energy partials of every launch in opart.
+struct ObsPlan {
+    std::vector<ObsPass> opass;
+    void* otab_dev = nullptr;
+    const ObsGroupT* ogroups = nullptr;
+    const ObsTermT* oterms = nullptr;
+    ObsTable far{};
+    void* far_dev = nullptr;
+    double* opart = nullptr;
+    size_t opart_len = 0;
+    bool obs_tiled = false;
+
+    ObsPlan() = default;
+    ObsPlan(const ObsPlan&) = delete;
+    ObsPlan& operator=(const ObsPlan&) = delete;
+    void release() {
+        if (otab_dev) hipFree(otab_dev);
+        if (far_dev) hipFree(far_dev);
+        if (opart) hipFree(opart);
+        otab_dev = far_dev = nullptr;
+        opart = nullptr;
+        ogroups = nullptr;
+        oterms = nullptr;
+        far = ObsTable{};
+        opass.clear();
+        opart_len = 0;
+        obs_tiled = false;
+    }
+    ~ObsPlan() { release(); }
+};
+
 struct Circuit {
     int n = 0, prec = kC128;
     Engine eng;
@@ -782,25 +988,12 @@ struct Circuit {
     size_t stored_cap = 0;   // bytes
     size_t stored_budget = size_t(16) << 30;
     bool stored_valid = false;  // the last forward sweep filled them
-    // tiled observable: launches over tile passes, then the groups no pass
-    // holds through global gathers; energy partials of every launch in opart
-    std::vector<ObsPass> opass;
-    void* otab_dev = nullptr;
-    const ObsGroupT* ogroups = nullptr;
-    const ObsTermT* oterms = nullptr;
-    ObsTable far{};
-    void* far_dev = nullptr;
-    double* opart = nullptr;
-    size_t opart_len = 0;
-    bool obs_tiled = false;
+    ObsPlan oplan;  // the observable over tile passes (16 to 34 qubits)
 
     ~Circuit() {
         if (eng.stream) hipStreamSynchronize(eng.stream);
         if (lam) hipFree(lam);
         if (stored) hipFree(stored);
-        if (otab_dev) hipFree(otab_dev);
-        if (far_dev) hipFree(far_dev);
-        if (opart) hipFree(opart);
         if (gpart) hipFree(gpart);
         if (gout) hipFree(gout);
         if (hgout) hipHostFree(hgout);
@@ -1027,37 +1220,53 @@ bool run_forward_segments(Circuit& c, const double* params) {
     return true;
 }
 
+// The planned Pauli sum over the state `amp` (dim amplitudes): Re <psi|H psi>
+// into energy_dev and, with `write`, H psi into lam.  `skip` (energy only)
+// applies the reference's |psi_i|^2 < 1e-18 skip.
 template <typename R>
-bool apply_observable_tiled(Circuit& c, bool write, double* energy_dev) {
-    Engine& e = c.eng;
+bool launch_observable_plan(const ObsPlan& p, const void* amp, void* lam, uint64_t dim, hipStream_t st, bool write,
+                            bool skip, double* energy_dev) {
+    if (write && skip) return false;
     size_t off = 0;
     bool first = true;
-    for (const ObsPass& op : c.opass) {
+    for (const ObsPass& op : p.opass) {
         const unsigned grid = (unsigned)std::min<uint64_t>(op.d.ntiles, 2048);
-        const size_t lds = ((size_t)1 << op.d.ts_bits) * sizeof(cv<R>) + (size_t)op.tcount * sizeof(double);
-        auto kern = !write ? k_obs_tile<R, false, true> : (first ? k_obs_tile<R, true, true> : k_obs_tile<R, true, false>);
-        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(512), lds, e.stream, (const cv<R>*)e.amp, (cv<R>*)c.lam, op,
-                           c.ogroups, c.oterms, c.opart + off);
+        const size_t lds = ((size_t)1 << op.d.ts_bits) * sizeof(cv<R>) + (size_t)op.tcount * (sizeof(double) + 4);
+        if (!write) {
+            auto kern = skip ? k_obs_energy_tile<R, true> : k_obs_energy_tile<R, false>;
+            hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(512), lds, st, (const cv<R>*)amp, op, p.ogroups, p.oterms,
+                               p.opart + off);
+        } else {
+            auto kern = first ? k_obs_tile<R, true, true> : k_obs_tile<R, true, false>;
+            hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(512), lds, st, (const cv<R>*)amp, (cv<R>*)lam, op, p.ogroups,
+                               p.oterms, p.opart + off);
+        }
         if (!hip_ok(hipGetLastError(), "tiled observable")) return false;
         off += grid;
         first = false;
     }
-    if (c.far.ngroups > 0) {
-        const unsigned b = nblocks_for(e.dim);
-        auto kern = !write ? k_apply_h<R, false> : (first ? k_apply_h<R, true, false> : k_apply_h<R, true, true>);
-        hipLaunchKernelGGL(kern, dim3(b), dim3(256), 0, e.stream, (const cv<R>*)e.amp, (cv<R>*)c.lam, e.dim, c.far,
-                           c.opart + off);
+    if (p.far.ngroups > 0) {
+        const unsigned b = nblocks_for(dim);
+        auto kern = skip ? k_apply_h<R, false, false, true>
+                         : !write ? k_apply_h<R, false> : (first ? k_apply_h<R, true, false> : k_apply_h<R, true, true>);
+        hipLaunchKernelGGL(kern, dim3(b), dim3(256), 0, st, (const cv<R>*)amp, (cv<R>*)lam, dim, p.far, p.opart + off);
         if (!hip_ok(hipGetLastError(), "observable")) return false;
         off += b;
     }
-    return hip_ok(launch_finalize(c.opart, (unsigned)off, 1, energy_dev, e.stream), "finalize");
+    return hip_ok(launch_finalize(p.opart, (unsigned)off, 1, energy_dev, st), "finalize");
+}
+
+template <typename R>
+bool apply_observable_tiled(Circuit& c, bool write, double* energy_dev) {
+    return launch_observable_plan<R>(c.oplan, c.eng.amp, c.lam, c.eng.dim, c.eng.stream, write, false, energy_dev);
 }
 
 template <typename R>
 bool apply_observable(Circuit& c, bool write, double* energy_dev) {
     Engine& e = c.eng;
-    if (c.obs_tiled) return apply_observable_tiled<R>(c, write, energy_dev);
+    if (c.oplan.obs_tiled) return apply_observable_tiled<R>(c, write, energy_dev);
     const unsigned b = nblocks_for(e.dim);
     if (write)
         hipLaunchKernelGGL((k_apply_h<R, true>), dim3(b), dim3(256), 0, e.stream, (const cv<R>*)e.amp, (cv<R>*)c.lam, e.dim,
@@ -1142,7 +1351,8 @@ struct TileGeom {
 // 64 KiB tiles: with 512 threads and register groups of up to 4 (complex64)
 // or 3 (complex128) qubits, every thread owns one sub-cube per group.  Runs
 // of 2^r amplitudes are 128-byte lines.
-TileGeom tile_geom(const Circuit& c) { return c.prec == kC64 ? TileGeom{13, 4} : TileGeom{12, 3}; }
+TileGeom tile_geom(int prec) { return prec == kC64 ? TileGeom{13, 4} : TileGeom{12, 3}; }
+TileGeom tile_geom(const Circuit& c) { return tile_geom(c.prec); }
 
 // tiled stages pay off once the state no longer fits a few tiles
 bool use_tiles(const Circuit& c) { return c.n >= 16 && c.n <= 34; }
@@ -1368,16 +1578,20 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
 // whose tile bits hold its flip mask; groups no pass holds (or with more terms
 // than one launch takes) stay in the global-gather table `far`.  Inputs are
 // the grouped table of iq_circuit_set_observable.
-bool plan_observable_tiles(Circuit& c, const std::vector<uint64_t>& xm, const std::vector<int>& start,
-                           const std::vector<uint64_t>& zy, const std::vector<int>& ny, const std::vector<double>& co) {
-    const TileGeom g = tile_geom(c);
-    const int npass = pass_of(g, c.n, c.n - 1) + 1;
+// Plans the Pauli sum (terms grouped by flip mask xm[g], group g owning terms
+// [start[g], start[g+1])) over the tile passes of an n-qubit state of `dim`
+// amplitudes; `p` must be empty (released).
+bool plan_observable_tiles(ObsPlan& p, int n, int prec, uint64_t dim, const std::vector<uint64_t>& xm,
+                           const std::vector<int>& start, const std::vector<uint64_t>& zy, const std::vector<int>& ny,
+                           const std::vector<double>& co) {
+    const TileGeom g = tile_geom(prec);
+    const int npass = pass_of(g, n, n - 1) + 1;
     std::vector<PassDesc> pd(npass);
-    for (int k = 0; k < npass; ++k) pd[k] = pass_desc(g, c.n, k);
+    for (int k = 0; k < npass; ++k) pd[k] = pass_desc(g, n, k);
     auto split = [&](const PassDesc& d, uint64_t m, uint32_t& tile, uint64_t& fixed) {
         tile = 0;
         fixed = 0;
-        for (int q = 0; q < c.n; ++q)
+        for (int q = 0; q < n; ++q)
             if ((m >> q) & 1) {
                 const int lb = local_bit(d, q);
                 if (lb >= 0) tile |= 1u << lb;
@@ -1399,12 +1613,12 @@ bool plan_observable_tiles(Circuit& c, const std::vector<uint64_t>& xm, const st
     }
     std::vector<ObsGroupT> G;
     std::vector<ObsTermT> Tm;
-    c.opass.clear();
+    p.opass.clear();
     for (int k = 0; k < npass; ++k) {
         ObsPass op{};
         op.d = pd[k];
         auto flush = [&]() {
-            if (op.gcount > 0) c.opass.push_back(op);
+            if (op.gcount > 0) p.opass.push_back(op);
             op.gfirst = (int)G.size();
             op.tfirst = (int)Tm.size();
             op.gcount = op.tcount = 0;
@@ -1418,13 +1632,22 @@ bool plan_observable_tiles(Circuit& c, const std::vector<uint64_t>& xm, const st
             split(pd[k], xm[gi], gg.xl, fx);
             gg.first = op.tcount;
             gg.count = nt;
-            for (int t = start[gi]; t < start[gi + 1]; ++t) {
-                ObsTermT tt{};
-                split(pd[k], zy[t], tt.zl, tt.zfix);
-                tt.ny = ny[t];
-                tt.coeff = co[t];
-                Tm.push_back(tt);
-            }
+            gg.nre = gg.nre0 = gg.nim0 = 0;
+            // the real-part run, then the imaginary-part run; in each, the
+            // terms without sign bits on the tile first
+            for (int imag = 0; imag < 2; ++imag)
+                for (int on_tile = 0; on_tile < 2; ++on_tile)
+                    for (int t = start[gi]; t < start[gi + 1]; ++t) {
+                        const int cls = ny[t] & 3;
+                        if ((cls & 1) != imag) continue;
+                        ObsTermT tt{};
+                        split(pd[k], zy[t], tt.zl, tt.zfix);
+                        if ((tt.zl != 0) != (on_tile != 0)) continue;
+                        tt.coeff = cls >= 2 ? -co[t] : co[t];
+                        Tm.push_back(tt);
+                        if (!imag) ++gg.nre;
+                        if (!on_tile) ++(imag ? gg.nim0 : gg.nre0);
+                    }
             G.push_back(gg);
             ++op.gcount;
             op.tcount += nt;
@@ -1445,25 +1668,25 @@ bool plan_observable_tiles(Circuit& c, const std::vector<uint64_t>& xm, const st
         }
     }
     fstart.push_back((int)fzy.size());
-    if (c.otab_dev) hipFree(c.otab_dev);
-    if (c.far_dev) hipFree(c.far_dev);
-    if (c.opart) hipFree(c.opart);
-    c.otab_dev = c.far_dev = nullptr;
-    c.opart = nullptr;
-    c.far = ObsTable{};
-    c.obs_tiled = false;
+    if (p.otab_dev) hipFree(p.otab_dev);
+    if (p.far_dev) hipFree(p.far_dev);
+    if (p.opart) hipFree(p.opart);
+    p.otab_dev = p.far_dev = nullptr;
+    p.opart = nullptr;
+    p.far = ObsTable{};
+    p.obs_tiled = false;
     const size_t gbytes = std::max<size_t>(16, G.size() * sizeof(ObsGroupT));
     const size_t tbytes = std::max<size_t>(16, Tm.size() * sizeof(ObsTermT));
     const size_t toff = (gbytes + 255) & ~size_t(255);
-    if (!hip_ok(hipMalloc(&c.otab_dev, toff + tbytes), "observable tiles")) return false;
-    char* od = static_cast<char*>(c.otab_dev);
+    if (!hip_ok(hipMalloc(&p.otab_dev, toff + tbytes), "observable tiles")) return false;
+    char* od = static_cast<char*>(p.otab_dev);
     if ((!G.empty() && !hip_ok(hipMemcpy(od, G.data(), G.size() * sizeof(ObsGroupT), hipMemcpyHostToDevice), "upload")) ||
         (!Tm.empty() && !hip_ok(hipMemcpy(od + toff, Tm.data(), Tm.size() * sizeof(ObsTermT), hipMemcpyHostToDevice), "upload")))
         return false;
-    c.ogroups = reinterpret_cast<const ObsGroupT*>(od);
-    c.oterms = reinterpret_cast<const ObsTermT*>(od + toff);
+    p.ogroups = reinterpret_cast<const ObsGroupT*>(od);
+    p.oterms = reinterpret_cast<const ObsTermT*>(od + toff);
     size_t parts = 0;
-    for (const ObsPass& op : c.opass) parts += std::min<uint64_t>(op.d.ntiles, 2048);
+    for (const ObsPass& op : p.opass) parts += std::min<uint64_t>(op.d.ntiles, 2048);
     if (!far.empty()) {
         const size_t F = fxm.size(), FT = fzy.size();
         const size_t o1 = 0, o2 = o1 + F * 8, o3 = (o2 + (F + 1) * 4 + 7) & ~size_t(7), o4 = o3 + FT * 8,
@@ -1474,21 +1697,21 @@ bool plan_observable_tiles(Circuit& c, const std::vector<uint64_t>& xm, const st
         std::memcpy(host.data() + o3, fzy.data(), FT * 8);
         std::memcpy(host.data() + o4, fny.data(), FT * 4);
         std::memcpy(host.data() + o5, fco.data(), FT * 8);
-        if (!hip_ok(hipMalloc(&c.far_dev, bytes), "observable table") ||
-            !hip_ok(hipMemcpy(c.far_dev, host.data(), bytes, hipMemcpyHostToDevice), "observable upload"))
+        if (!hip_ok(hipMalloc(&p.far_dev, bytes), "observable table") ||
+            !hip_ok(hipMemcpy(p.far_dev, host.data(), bytes, hipMemcpyHostToDevice), "observable upload"))
             return false;
-        char* d = static_cast<char*>(c.far_dev);
-        c.far.xmask = reinterpret_cast<const uint64_t*>(d + o1);
-        c.far.start = reinterpret_cast<const int*>(d + o2);
-        c.far.zy = reinterpret_cast<const uint64_t*>(d + o3);
-        c.far.ny = reinterpret_cast<const int*>(d + o4);
-        c.far.coeff = reinterpret_cast<const double*>(d + o5);
-        c.far.ngroups = (int)F;
-        parts += nblocks_for(c.eng.dim);
+        char* d = static_cast<char*>(p.far_dev);
+        p.far.xmask = reinterpret_cast<const uint64_t*>(d + o1);
+        p.far.start = reinterpret_cast<const int*>(d + o2);
+        p.far.zy = reinterpret_cast<const uint64_t*>(d + o3);
+        p.far.ny = reinterpret_cast<const int*>(d + o4);
+        p.far.coeff = reinterpret_cast<const double*>(d + o5);
+        p.far.ngroups = (int)F;
+        parts += nblocks_for(dim);
     }
-    if (!hip_ok(hipMalloc((void**)&c.opart, std::max<size_t>(parts, 1) * sizeof(double)), "energy partials")) return false;
-    c.opart_len = parts;
-    c.obs_tiled = true;
+    if (!hip_ok(hipMalloc((void**)&p.opart, std::max<size_t>(parts, 1) * sizeof(double)), "energy partials")) return false;
+    p.opart_len = parts;
+    p.obs_tiled = true;
     return true;
 }
 
@@ -1673,6 +1896,53 @@ bool prepare(Circuit& c, const double* params, size_t nparams) {
 }
 
 }  // namespace
+
+// Sum_t coeff_t Re <psi|P_t|psi> over a statevector handle's state through
+// the tile passes, with the reference's per-index skip
+// (core/src/statevector.cpp:258-291 term by term).  One pass per flip mask
+// (up to eight strings each) costs a read of the state per mask; the tile
+// passes read it once per pass (three at 30 qubits complex128) and take
+// every mask on the pass's tile bits.  Sets *used = false, touching nothing,
+// when the tile passes would not save a read (small states, few masks).
+bool pauli_sum_tiled(Engine& e, size_t nterms, const PauliTerm* terms, const double* coeffs, double* out, bool* used) {
+    *used = false;
+    if (e.n < 20 || e.n > 34 || nterms == 0) return true;
+    std::vector<size_t> order(nterms);
+    for (size_t i = 0; i < nterms; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return terms[a].xmask < terms[b].xmask; });
+    std::vector<uint64_t> xm, zy;
+    std::vector<int> start, ny;
+    std::vector<double> co;
+    for (size_t k = 0; k < nterms; ++k) {
+        const PauliTerm& t = terms[order[k]];
+        if (xm.empty() || xm.back() != t.xmask) {
+            xm.push_back(t.xmask);
+            start.push_back((int)zy.size());
+        }
+        zy.push_back(t.zymask);
+        ny.push_back(t.ny);
+        co.push_back(coeffs ? coeffs[order[k]] : 1.0);
+    }
+    start.push_back((int)zy.size());
+    size_t mask_passes = 0;  // of the per-mask reduction, eight strings per pass
+    for (size_t gi = 0; gi + 1 < start.size(); ++gi) mask_passes += (size_t)(start[gi + 1] - start[gi] + 7) / 8;
+    const TileGeom g = tile_geom(e.prec);
+    const int npass = pass_of(g, e.n, e.n - 1) + 1;
+    if (mask_passes <= (size_t)npass) return true;
+    ObsPlan plan;
+    if (!plan_observable_tiles(plan, e.n, e.prec, e.dim, xm, start, zy, ny, co)) return false;
+    const bool ok = e.prec == kC128
+                        ? launch_observable_plan<double>(plan, e.amp, nullptr, e.dim, e.stream, false, true, e.red_out)
+                        : launch_observable_plan<float>(plan, e.amp, nullptr, e.dim, e.stream, false, true, e.red_out);
+    // the plan's tables are freed after the stream has finished with them
+    if (!ok || !e.fetch(1, out)) {
+        e.sync();
+        return false;
+    }
+    *used = true;
+    return true;
+}
+
 }  // namespace iqg
 
 using namespace iqg;
@@ -1817,8 +2087,8 @@ int iq_circuit_set_observable(void* cp, size_t nterms, const double* coeffs, con
     c.obs.coeff = reinterpret_cast<const double*>(d + oco);
     c.obs.ngroups = (int)G;
     c.terms = terms;
-    c.obs_tiled = false;
-    if (use_tiles(c) && !plan_observable_tiles(c, xm, start, zy, ny, co)) return -1;
+    c.oplan.release();
+    if (use_tiles(c) && !plan_observable_tiles(c.oplan, c.n, c.prec, c.eng.dim, xm, start, zy, ny, co)) return -1;
     return 0;
 }
 

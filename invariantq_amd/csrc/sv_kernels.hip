@@ -158,18 +158,24 @@ __global__ void __launch_bounds__(256) k_red_norm(const cv<R>* __restrict__ psi,
     block_sum_store<1>(v, 1, part);
 }
 
-template <typename R>
+template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_red_bitprob(const cv<R>* __restrict__ psi, uint64_t count, int target,
                                                      double* __restrict__ part) {
     double v[2] = {0.0, 0.0};
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        const cv<R> a = psi[i];
-        const double x = (double)a.x, y = (double)a.y;
-        const double p = x * x + y * y;
-        if ((i >> target) & 1ULL) v[1] += p;
-        else v[0] += p;
-    }
+    grid_stride<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+        cv<R> a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (u < n) a[u] = ld_stream<R>(psi + i + u * stride, NT);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {  // in element order, as a single-element loop would add them
+            if (u >= n) break;
+            const double x = (double)a[u].x, y = (double)a[u].y;
+            const double p = x * x + y * y;
+            if (((i + u * stride) >> target) & 1ULL) v[1] += p;
+            else v[0] += p;
+        }
+    });
     block_sum_store<2>(v, 2, part);
 }
 
@@ -184,20 +190,21 @@ struct PauliGroup {
 
 // <psi|P|psi> = sum_i conj(psi[i ^ x]) * i^{nY} * (-1)^{popcount(i & (Y|Z))} * psi[i],
 // skipping |psi_i|^2 < 1e-18 (reference core/src/statevector.cpp:265-288).
-template <typename R>
+// With a flip mask x, index j (bit hbit, the highest of x, clear) and its
+// partner j ^ x are taken together, so every amplitude is read once; each of
+// the two still adds its own term, under its own skip test.  Four pairs (or
+// four amplitudes, x = 0) per thread and iteration are loaded at once.
+template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_red_pauli(const cv<R>* __restrict__ psi, uint64_t count, PauliGroup g,
-                                                   double* __restrict__ part) {
+                                                   int hbit, double* __restrict__ part) {
     double v[2 * kPauliPerGroup];
 #pragma unroll
     for (int m = 0; m < 2 * kPauliPerGroup; ++m) v[m] = 0.0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        const cv<R> a = psi[i];
+    // term of index i: conj(b) * phase_i * a with a = psi[i], b = psi[i ^ x]
+    auto term = [&](uint64_t i, const cv<R>& a, const cv<R>& b) __attribute__((always_inline)) {
         const double ar = (double)a.x, ai = (double)a.y;
-        if (ar * ar + ai * ai < 1e-18) continue;
-        const cv<R> b = psi[i ^ g.xmask];
+        if (ar * ar + ai * ai < 1e-18) return;
         const double br = (double)b.x, bi = (double)b.y;
-        // conj(b) * a
         const double pr = br * ar + bi * ai;
         const double pi = br * ai - bi * ar;
 #pragma unroll
@@ -214,6 +221,37 @@ __global__ void __launch_bounds__(256) k_red_pauli(const cv<R>* __restrict__ psi
             v[2 * s] += cr;
             v[2 * s + 1] += ci;
         }
+    };
+    if (g.xmask == 0) {
+        grid_stride<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+            cv<R> a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (u < n) a[u] = ld_stream<R>(psi + i + u * stride, NT);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (u < n) term(i + u * stride, a[u], a[u]);
+        });
+    } else {
+        const uint64_t lo = (1ULL << hbit) - 1;
+        grid_stride<4>(count >> 1, [&](uint64_t k, uint64_t stride, int n) {
+            uint64_t j[4];
+            cv<R> a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (u < n) {
+                    const uint64_t kk = k + u * stride;
+                    j[u] = ((kk & ~lo) << 1) | (kk & lo);
+                    a[u] = ld_stream<R>(psi + j[u], NT);
+                    b[u] = ld_stream<R>(psi + (j[u] ^ g.xmask), NT);
+                }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (u < n) {
+                    term(j[u], a[u], b[u]);
+                    term(j[u] ^ g.xmask, b[u], a[u]);
+                }
+        });
     }
     block_sum_store<2 * kPauliPerGroup>(v, 2 * g.nstr, part);
 }
@@ -532,9 +570,7 @@ hipError_t reduce_norm(const void* psi, uint64_t count, int prec, double* part, 
 hipError_t reduce_bit_probs(const void* psi, int n, int prec, int target, double* part, double* out, hipStream_t st) {
     const uint64_t count = 1ULL << n;
     const unsigned b = blocks_for(count, kRedThreads, kRedBlocks);
-    if (prec == kC128)
-        hipLaunchKernelGGL(k_red_bitprob<double>, dim3(b), dim3(kRedThreads), 0, st, (const double2*)psi, count, target, part);
-    else hipLaunchKernelGGL(k_red_bitprob<float>, dim3(b), dim3(kRedThreads), 0, st, (const float2*)psi, count, target, part);
+    IQ_LAUNCH_PASS(k_red_bitprob, count, prec, b, st, camp_ptr, psi, count, target, part);
     return finalize(part, b, 2, out, st);
 }
 
@@ -550,9 +586,26 @@ hipError_t reduce_pauli_group(const void* psi, int n, int prec, uint64_t xmask, 
     }
     const uint64_t count = 1ULL << n;
     const unsigned b = blocks_for(count, kRedThreads, kRedBlocks);
-    if (prec == kC128)
-        hipLaunchKernelGGL(k_red_pauli<double>, dim3(b), dim3(kRedThreads), 0, st, (const double2*)psi, count, g, part);
-    else hipLaunchKernelGGL(k_red_pauli<float>, dim3(b), dim3(kRedThreads), 0, st, (const float2*)psi, count, g, part);
+    // highest flipped bit: pairs (j, j ^ x) split on it
+    const int hbit = xmask ? 63 - __builtin_clzll(xmask) : 0;
+    // streaming when each load instruction covers whole 128-byte lines: the
+    // pairs' low index runs are 2^hbit amplitudes long
+    const int line_bits = prec == kC128 ? 3 : 4;
+    const bool nt = count >= kStreamMinCount && stream_hint_enabled() && (xmask == 0 || hbit >= line_bits);
+    if (prec == kC128) {
+        if (nt)
+            hipLaunchKernelGGL((k_red_pauli<double, true>), dim3(b), dim3(kRedThreads), 0, st, (const double2*)psi,
+                               count, g, hbit, part);
+        else
+            hipLaunchKernelGGL((k_red_pauli<double, false>), dim3(b), dim3(kRedThreads), 0, st, (const double2*)psi,
+                               count, g, hbit, part);
+    } else if (nt) {
+        hipLaunchKernelGGL((k_red_pauli<float, true>), dim3(b), dim3(kRedThreads), 0, st, (const float2*)psi, count,
+                           g, hbit, part);
+    } else {
+        hipLaunchKernelGGL((k_red_pauli<float, false>), dim3(b), dim3(kRedThreads), 0, st, (const float2*)psi, count,
+                           g, hbit, part);
+    }
     return finalize(part, b, 2 * nstr, out, st);
 }
 
