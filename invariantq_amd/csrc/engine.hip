@@ -695,9 +695,7 @@ void iq_statevector_sample(void* sv, size_t num_shots, uint64_t seed, uint64_t* 
     std::mt19937_64 gen(seed);
     std::uniform_real_distribution<double> uni(0.0, 1.0);
     for (size_t s = 0; s < num_shots; ++s) u[s] = uni(gen);
-    if (!hip_ok(launch_probs(e->amp, e->dim, e->prec, cdf, e->stream), "probs") ||
-        !hip_ok(launch_scan(cdf, cdf, e->dim, scratch, e->stream), "scan"))
-        return;
+    if (!hip_ok(launch_cdf(e->amp, e->dim, e->prec, cdf, scratch, e->stream), "cdf")) return;
     if (num_shots == 0) {
         e->sync();
         return;
@@ -736,9 +734,7 @@ void iq_statevector_sample_tasks(void* sv, size_t ntasks, const uint64_t* seeds,
     double* scratch = cdf + e->dim;
     double* du = scratch + nchunks;
     uint64_t* dout = reinterpret_cast<uint64_t*>(du + std::max<size_t>(total, 1));
-    if (!hip_ok(launch_probs(e->amp, e->dim, e->prec, cdf, e->stream), "probs") ||
-        !hip_ok(launch_scan(cdf, cdf, e->dim, scratch, e->stream), "scan"))
-        return;
+    if (!hip_ok(launch_cdf(e->amp, e->dim, e->prec, cdf, scratch, e->stream), "cdf")) return;
     std::vector<double> u(total);
     auto draw = [&](size_t t0, size_t t1) {
         for (size_t t = t0; t < t1; ++t) {
@@ -778,9 +774,7 @@ void iq_statevector_sample_uniforms(void* sv, size_t num_shots, const double* un
     double* scratch = cdf + e->dim;
     double* du = scratch + nchunks;
     uint64_t* dout = reinterpret_cast<uint64_t*>(du + std::max<size_t>(num_shots, 1));
-    if (!hip_ok(launch_probs(e->amp, e->dim, e->prec, cdf, e->stream), "probs") ||
-        !hip_ok(launch_scan(cdf, cdf, e->dim, scratch, e->stream), "scan"))
-        return;
+    if (!hip_ok(launch_cdf(e->amp, e->dim, e->prec, cdf, scratch, e->stream), "cdf")) return;
     if (num_shots) {
         if (!hip_ok(hipMemcpyAsync(du, uniforms, num_shots * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
             !hip_ok(launch_search(cdf, e->dim, du, num_shots, dout, e->stream), "search") ||
@@ -797,8 +791,7 @@ void iq_statevector_get_cdf(void* sv, double* cdf_out) {
     DeviceGuard g(e->dev);
     const size_t nchunks = scan_scratch_doubles(e->dim);
     if (!e->ensure_work(e->dim + nchunks + 2)) return;
-    if (!hip_ok(launch_probs(e->amp, e->dim, e->prec, e->work, e->stream), "probs") ||
-        !hip_ok(launch_scan(e->work, e->work, e->dim, e->work + e->dim, e->stream), "scan") ||
+    if (!hip_ok(launch_cdf(e->amp, e->dim, e->prec, e->work, e->work + e->dim, e->stream), "cdf") ||
         !hip_ok(hipMemcpyAsync(cdf_out, e->work, e->dim * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
         return;
     e->sync();

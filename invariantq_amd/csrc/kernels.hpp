@@ -71,7 +71,9 @@ hipError_t launch_dm_collapse(void* rho, int nq, int target, int outcome, double
 
 // Sampling: inclusive FP64 prefix sum of probs into cdf, then lower_bound per uniform.
 size_t scan_scratch_doubles(uint64_t count);
-hipError_t launch_scan(const double* probs, double* cdf, uint64_t count, double* scratch, hipStream_t st);
+// Inclusive FP64 CDF of |psi_i|^2 into cdf (count doubles); scratch holds
+// scan_scratch_doubles(count) chunk sums.
+hipError_t launch_cdf(const void* psi, uint64_t count, int prec, double* cdf, double* scratch, hipStream_t st);
 hipError_t launch_search(const double* cdf, uint64_t count, const double* u, uint64_t shots, uint64_t* out, hipStream_t st);
 
 // Reduced density matrix of `keep` (LSB-first local index), complex128 out [kd * kd].

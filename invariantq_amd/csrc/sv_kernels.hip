@@ -46,6 +46,9 @@ __global__ void __launch_bounds__(256) k_init_basis(cv<R>* __restrict__ psi, uin
     });
 }
 
+// Amplitudes whose target bit is the outcome are scaled; the others are
+// written as zeros without being read, so the pass moves the kept half twice
+// and the discarded half once.
 template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_collapse(cv<R>* __restrict__ psi, uint64_t count, int target, int outcome,
                                                   double factor) {
@@ -54,7 +57,7 @@ __global__ void __launch_bounds__(256) k_collapse(cv<R>* __restrict__ psi, uint6
         cv<R> v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (u < n) v[u] = ld_stream<R>(psi + i + u * stride, NT);
+            if (u < n && (int)(((i + u * stride) >> target) & 1ULL) == outcome) v[u] = ld_stream<R>(psi + i + u * stride, NT);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (u >= n) break;
@@ -326,47 +329,101 @@ __device__ __forceinline__ double block_exclusive_scan(double x, double* total) 
     return before + incl - x;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_chunks(const double* __restrict__ p, double* __restrict__ cdf,
-                                                              uint64_t count, double* __restrict__ totals) {
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
+// The CDF straight from the amplitudes, in two passes over the state and no
+// probability buffer: chunk sums of |psi_i|^2 (read only), an exclusive scan
+// of the chunk sums, then each chunk's prefix sums plus its offset (read the
+// state again, write the CDF).  Both passes stage a chunk's probabilities in
+// LDS from coalesced loads; thread t then adds its kScanItems consecutive
+// entries in index order and the block scans the thread sums, the same
+// arithmetic in both passes, so a chunk's sum in the first equals its last
+// prefix in the second.  Fixed order throughout: the CDF is the same bits on
+// every run.
+__device__ __forceinline__ uint32_t scan_lds(uint32_t i) { return i + (i >> 4); }  // one pad per 16 (banks)
+
+template <typename R>
+__device__ __forceinline__ void chunk_probs(const cv<R>* __restrict__ psi, uint64_t count, uint64_t base, double* sh) {
+    cv<R> v[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kScanThreads + threadIdx.x;
+        v[k] = psi[i < count ? i : count - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t j = (uint32_t)k * kScanThreads + threadIdx.x;
+        const double x = (double)v[k].x, y = (double)v[k].y;
+        sh[scan_lds(j)] = (base + j < count) ? x * x + y * y : 0.0;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kScanThreads) k_cdf_sums(const cv<R>* __restrict__ psi, uint64_t count,
+                                                           double* __restrict__ totals) {
+    __shared__ double sh[kScanChunk + kScanChunk / 16];
+    chunk_probs<R>(psi, count, (uint64_t)blockIdx.x * kScanChunk, sh);
+    __syncthreads();
+    double run = 0.0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) run += sh[scan_lds(threadIdx.x * kScanItems + k)];
+    double total = 0.0;
+    block_exclusive_scan(run, &total);
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kScanThreads) k_cdf_write(const cv<R>* __restrict__ psi, uint64_t count,
+                                                            const double* __restrict__ offsets, double* __restrict__ cdf) {
+    __shared__ double sh[kScanChunk + kScanChunk / 16];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    chunk_probs<R>(psi, count, base, sh);
+    __syncthreads();
     double loc[kScanItems];
     double run = 0.0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const uint64_t i = base + k;
-        run += (i < count) ? p[i] : 0.0;
+        run += sh[scan_lds(threadIdx.x * kScanItems + k)];
         loc[k] = run;
     }
     double total = 0.0;
-    const double off = block_exclusive_scan(run, &total);
+    const double off = block_exclusive_scan(run, &total);  // ends with a barrier: sh may be rewritten
+    const double chunk_off = offsets[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) sh[scan_lds(threadIdx.x * kScanItems + k)] = (off + loc[k]) + chunk_off;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const uint64_t i = base + k;
-        if (i < count) cdf[i] = off + loc[k];
-    }
-    if (threadIdx.x == 0) totals[blockIdx.x] = total;
-}
-
-// Exclusive scan of the chunk totals, in place, by one block.
-__global__ void __launch_bounds__(kScanThreads) k_scan_totals(double* __restrict__ totals, uint64_t nchunks) {
-    double carry = 0.0;
-    for (uint64_t b0 = 0; b0 < nchunks; b0 += kScanThreads) {
-        const uint64_t b = b0 + threadIdx.x;
-        const double x = (b < nchunks) ? totals[b] : 0.0;
-        double t = 0.0;
-        const double ex = block_exclusive_scan(x, &t);
-        if (b < nchunks) totals[b] = carry + ex;
-        carry += t;
+        const uint32_t j = (uint32_t)k * kScanThreads + threadIdx.x;
+        if (base + j < count) cdf[base + j] = sh[scan_lds(j)];
     }
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_add(double* __restrict__ cdf, uint64_t count,
-                                                           const double* __restrict__ offsets) {
-    const double off = offsets[blockIdx.x];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
-    for (uint64_t k = threadIdx.x; k < kScanChunk; k += kScanThreads) {
-        const uint64_t i = base + k;
-        if (i < count) cdf[i] += off;
+// Exclusive scan of the chunk sums, in place, by one block of
+// kTotalsThreads: thread t sums a contiguous segment, the block scans the
+// segment sums (a wave scan, then the wave sums), and each thread writes its
+// segment's running offsets.
+constexpr int kTotalsThreads = 1024;
+
+__global__ void __launch_bounds__(kTotalsThreads) k_scan_totals(double* __restrict__ totals, uint64_t nchunks) {
+    __shared__ double wsum[kTotalsThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t seg = (nchunks + kTotalsThreads - 1) / kTotalsThreads;
+    const uint64_t b0 = std::min<uint64_t>((uint64_t)threadIdx.x * seg, nchunks), b1 = std::min<uint64_t>(b0 + seg, nchunks);
+    double sum = 0.0;
+    for (uint64_t b = b0; b < b1; ++b) sum += totals[b];
+    double incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    double run = incl - sum;
+    for (int w = 0; w < wave; ++w) run += wsum[w];
+    for (uint64_t b = b0; b < b1; ++b) {
+        const double x = totals[b];
+        totals[b] = run;
+        run += x;
     }
 }
 
@@ -627,11 +684,22 @@ hipError_t launch_dm_collapse(void* rho, int nq, int target, int outcome, double
 
 size_t scan_scratch_doubles(uint64_t count) { return (size_t)((count + kScanChunk - 1) / kScanChunk); }
 
-hipError_t launch_scan(const double* probs, double* cdf, uint64_t count, double* scratch, hipStream_t st) {
+hipError_t launch_cdf(const void* psi, uint64_t count, int prec, double* cdf, double* scratch, hipStream_t st) {
     const uint64_t nchunks = (count + kScanChunk - 1) / kScanChunk;
-    hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)nchunks), dim3(kScanThreads), 0, st, probs, cdf, count, scratch);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, st, scratch, nchunks);
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nchunks), dim3(kScanThreads), 0, st, cdf, count, scratch);
+    if (nchunks > 0x7fffffffULL) return hipErrorInvalidValue;
+    if (prec == kC128)
+        hipLaunchKernelGGL(k_cdf_sums<double>, dim3((unsigned)nchunks), dim3(kScanThreads), 0, st, (const double2*)psi,
+                           count, scratch);
+    else
+        hipLaunchKernelGGL(k_cdf_sums<float>, dim3((unsigned)nchunks), dim3(kScanThreads), 0, st, (const float2*)psi,
+                           count, scratch);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kTotalsThreads), 0, st, scratch, nchunks);
+    if (prec == kC128)
+        hipLaunchKernelGGL(k_cdf_write<double>, dim3((unsigned)nchunks), dim3(kScanThreads), 0, st, (const double2*)psi,
+                           count, scratch, cdf);
+    else
+        hipLaunchKernelGGL(k_cdf_write<float>, dim3((unsigned)nchunks), dim3(kScanThreads), 0, st, (const float2*)psi,
+                           count, scratch, cdf);
     return hipGetLastError();
 }
 
