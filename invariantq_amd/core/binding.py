@@ -392,7 +392,8 @@ class NativeStateVector:
         lens = np.ascontiguousarray([len(o) for _, o, _ in terms], dtype=np.uint64)
         vals = np.empty(2 * max(1, len(terms)), dtype=np.float64)
         vr, vi = ctypes.c_double(0.0), ctypes.c_double(0.0)
-        # without per-term values the library may sum over tile passes
+        # without per-term values the library may sum over tile passes (same
+        # result to rounding)
         _NATIVE_LIB.iq_statevector_expval_pauli_sum(self._ptr, len(terms), _dp(coeffs), ops, qubits.ctypes.data_as(_s),
                                                     lens.ctypes.data_as(_s), ctypes.byref(vr), ctypes.byref(vi),
                                                     _dp(vals) if per_term else None)

@@ -144,12 +144,13 @@ __global__ void __launch_bounds__(256) k_cz_set(cv<R>* __restrict__ psi, cv<R>* 
 // ACCUM: the table's part of H psi is added to lambda; the energy partials
 // are this part's share either way.  SKIP (energy only): a term whose
 // right-hand amplitude psi[k ^ x] has |.|^2 < 1e-18 adds nothing, as in the
-// reference's <psi|P|psi> (core/src/statevector.cpp:273-275).
+// reference's <psi|P|psi> (core/src/statevector.cpp:273-275); the skip makes
+// the sum slightly non-real, so both parts are kept (Re, Im per block).
 template <typename R, bool WRITE, bool ACCUM = false, bool SKIP = false>
 __global__ void __launch_bounds__(256) k_apply_h(const cv<R>* __restrict__ psi, cv<R>* __restrict__ lam, uint64_t count,
                                                  ObsTable h, double* __restrict__ part) {
     static_assert(!(SKIP && WRITE), "the skip applies to the energy only");
-    double e[1] = {0.0};
+    double e[2] = {0.0, 0.0};
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
         double sr = 0.0, si = 0.0;
@@ -173,6 +174,7 @@ __global__ void __launch_bounds__(256) k_apply_h(const cv<R>* __restrict__ psi, 
         }
         const cv<R> p = psi[k];
         e[0] += (double)p.x * sr + (double)p.y * si;
+        if (SKIP) e[1] += (double)p.x * si - (double)p.y * sr;
         if (WRITE) {
             cv<R> o;
             if (ACCUM) {
@@ -186,7 +188,7 @@ __global__ void __launch_bounds__(256) k_apply_h(const cv<R>* __restrict__ psi, 
             lam[k] = o;
         }
     }
-    block_sum_store<1>(e, 1, part);
+    block_sum_store<2>(e, SKIP ? 2 : 1, part);
 }
 
 // ------------------------------------------------------------ tiled stages
@@ -756,7 +758,7 @@ __global__ void __launch_bounds__(512) k_obs_energy_tile(const cv<R>* __restrict
     const ObsGroupT* gp = groups + op.gfirst;
     const ObsTermT* tp = terms + op.tfirst;
     for (int k = tid; k < op.tcount; k += T) zls[k] = tp[k].zl;
-    double e = 0.0;
+    double e = 0.0, ei = 0.0;  // Re and (SKIP) Im of sum_i psi_i S_i
     for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
         tile_in<R>(d, t, psi, tile, TS);
         const uint64_t base = pass_global(d, t, 0);
@@ -836,18 +838,35 @@ __global__ void __launch_bounds__(512) k_obs_energy_tile(const cv<R>* __restrict
                 }
             }
 #pragma unroll
-            for (int q = 0; q < E; ++q) e += pr[q] * sr[q] - pi[q] * si[q];
+            for (int q = 0; q < E; ++q) {
+                e += pr[q] * sr[q] - pi[q] * si[q];
+                if (SKIP) ei += pr[q] * si[q] + pi[q] * sr[q];
+            }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
-    if ((tid & 63) == 0) wsum[tid >> 6] = e;
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_down(e, off, 64);
+        if (SKIP) ei += __shfl_down(ei, off, 64);
+    }
+    if ((tid & 63) == 0) {
+        wsum[tid >> 6] = e;
+        wsum[8 + (tid >> 6)] = ei;
+    }
     __syncthreads();
     if (tid == 0) {
-        double s2 = 0.0;
-        for (uint32_t w = 0; w < (T >> 6); ++w) s2 += wsum[w];
-        part[blockIdx.x] = s2;
+        double s2 = 0.0, s3 = 0.0;
+        for (uint32_t w = 0; w < (T >> 6); ++w) {
+            s2 += wsum[w];
+            s3 += wsum[8 + w];
+        }
+        if (SKIP) {
+            part[2 * blockIdx.x] = s2;
+            part[2 * blockIdx.x + 1] = s3;
+        } else {
+            part[blockIdx.x] = s2;
+        }
     }
 }
 
@@ -1222,11 +1241,13 @@ bool run_forward_segments(Circuit& c, const double* params) {
 
 // The planned Pauli sum over the state `amp` (dim amplitudes): Re <psi|H psi>
 // into energy_dev and, with `write`, H psi into lam.  `skip` (energy only)
-// applies the reference's |psi_i|^2 < 1e-18 skip.
+// applies the reference's |psi_i|^2 < 1e-18 skip and writes Re and Im into
+// energy_dev[0], energy_dev[1] (the plan holds two partials per block).
 template <typename R>
 bool launch_observable_plan(const ObsPlan& p, const void* amp, void* lam, uint64_t dim, hipStream_t st, bool write,
                             bool skip, double* energy_dev) {
     if (write && skip) return false;
+    const int m = skip ? 2 : 1;  // partials per block
     size_t off = 0;
     bool first = true;
     for (const ObsPass& op : p.opass) {
@@ -1236,7 +1257,7 @@ bool launch_observable_plan(const ObsPlan& p, const void* amp, void* lam, uint64
             auto kern = skip ? k_obs_energy_tile<R, true> : k_obs_energy_tile<R, false>;
             hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(512), lds, st, (const cv<R>*)amp, op, p.ogroups, p.oterms,
-                               p.opart + off);
+                               p.opart + m * off);
         } else {
             auto kern = first ? k_obs_tile<R, true, true> : k_obs_tile<R, true, false>;
             hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1251,11 +1272,11 @@ bool launch_observable_plan(const ObsPlan& p, const void* amp, void* lam, uint64
         const unsigned b = nblocks_for(dim);
         auto kern = skip ? k_apply_h<R, false, false, true>
                          : !write ? k_apply_h<R, false> : (first ? k_apply_h<R, true, false> : k_apply_h<R, true, true>);
-        hipLaunchKernelGGL(kern, dim3(b), dim3(256), 0, st, (const cv<R>*)amp, (cv<R>*)lam, dim, p.far, p.opart + off);
+        hipLaunchKernelGGL(kern, dim3(b), dim3(256), 0, st, (const cv<R>*)amp, (cv<R>*)lam, dim, p.far, p.opart + m * off);
         if (!hip_ok(hipGetLastError(), "observable")) return false;
         off += b;
     }
-    return hip_ok(launch_finalize(p.opart, (unsigned)off, 1, energy_dev, st), "finalize");
+    return hip_ok(launch_finalize(p.opart, (unsigned)off, m, energy_dev, st), "finalize");
 }
 
 template <typename R>
@@ -1709,7 +1730,9 @@ bool plan_observable_tiles(ObsPlan& p, int n, int prec, uint64_t dim, const std:
         p.far.ngroups = (int)F;
         parts += nblocks_for(dim);
     }
-    if (!hip_ok(hipMalloc((void**)&p.opart, std::max<size_t>(parts, 1) * sizeof(double)), "energy partials")) return false;
+    // two per block: a statevector Pauli sum keeps Re and Im
+    if (!hip_ok(hipMalloc((void**)&p.opart, 2 * std::max<size_t>(parts, 1) * sizeof(double)), "energy partials"))
+        return false;
     p.opart_len = parts;
     p.obs_tiled = true;
     return true;
@@ -1897,7 +1920,7 @@ bool prepare(Circuit& c, const double* params, size_t nparams) {
 
 }  // namespace
 
-// Sum_t coeff_t Re <psi|P_t|psi> over a statevector handle's state through
+// Sum_t coeff_t <psi|P_t|psi> over a statevector handle's state through
 // the tile passes, with the reference's per-index skip
 // (core/src/statevector.cpp:258-291 term by term).  One pass per flip mask
 // (up to eight strings each) costs a read of the state per mask; the tile
@@ -1905,6 +1928,7 @@ bool prepare(Circuit& c, const double* params, size_t nparams) {
 // every mask on the pass's tile bits.  Sets *used = false, touching nothing,
 // when the tile passes would not save a read (small states, few masks).
 bool pauli_sum_tiled(Engine& e, size_t nterms, const PauliTerm* terms, const double* coeffs, double* out, bool* used) {
+    // out[0], out[1]: Re and Im
     *used = false;
     if (e.n < 20 || e.n > 34 || nterms == 0) return true;
     std::vector<size_t> order(nterms);
@@ -1935,7 +1959,7 @@ bool pauli_sum_tiled(Engine& e, size_t nterms, const PauliTerm* terms, const dou
                         ? launch_observable_plan<double>(plan, e.amp, nullptr, e.dim, e.stream, false, true, e.red_out)
                         : launch_observable_plan<float>(plan, e.amp, nullptr, e.dim, e.stream, false, true, e.red_out);
     // the plan's tables are freed after the stream has finished with them
-    if (!ok || !e.fetch(1, out)) {
+    if (!ok || !e.fetch(2, out)) {
         e.sync();
         return false;
     }

@@ -20,6 +20,10 @@ template <typename R> using cv = typename cvec<R>::type;
 
 constexpr int kMaxK = 5;            // largest fused block: 5 qubits, 32x32
 constexpr int kMaxN = 36;           // guards 64-bit index arithmetic
+// A dispatch holds fewer than 2^32 work-items per grid dimension: a kernel
+// whose work can exceed that (states above 32 qubits) loops over a grid
+// capped at kMaxDispatchItems / block threads.
+constexpr uint64_t kMaxDispatchItems = 0xffffffffULL;
 constexpr int kRedBlocks = 1024;    // partial-sum slots for deterministic reductions
 constexpr int kRedThreads = 256;
 

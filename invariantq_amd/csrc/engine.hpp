@@ -110,8 +110,9 @@ struct Engine {
     bool pauli_terms(size_t nterms, const PauliTerm* terms, double* vals);
 };
 
-// Sum_t coeff_t Re <psi|P_t|psi> of the engine's state over tile passes
-// (adjoint.hip); *used = false when one pass per flip mask is as cheap.
+// Sum_t coeff_t <psi|P_t|psi> of the engine's state over tile passes
+// (adjoint.hip), Re and Im into out[0], out[1]; *used = false when one pass
+// per flip mask is as cheap.
 bool pauli_sum_tiled(Engine& e, size_t nterms, const PauliTerm* terms, const double* coeffs, double* out, bool* used);
 
 }  // namespace iqg

@@ -100,24 +100,26 @@ template <typename R, int K, bool M3, bool NT = false>
 __global__ void __launch_bounds__(256) k_gate_direct_ptr(cv<R>* __restrict__ psi, uint64_t ngroups, DirectArgs<K> a,
                                                          const R* __restrict__ mat) {
     constexpr int D = 1 << K;
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    const uint64_t base = insert_zero_bits<K>(g, a.pos);
-    R xr[D], xi[D];
+    // one group per thread; the grid is capped above 2^32 threads
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+        const uint64_t base = insert_zero_bits<K>(g, a.pos);
+        R xr[D], xi[D];
 #pragma unroll
-    for (int s = 0; s < D; ++s) {
-        const cv<R> v = ld_stream<R>(psi + base + a.off[s], NT);
-        xr[s] = v.x;
-        xi[s] = v.y;
+        for (int s = 0; s < D; ++s) {
+            const cv<R> v = ld_stream<R>(psi + base + a.off[s], NT);
+            xr[s] = v.x;
+            xi[s] = v.y;
+        }
+        matvec<R, D, M3>(
+            xr, xi, [&](int i) { return mat[i]; },
+            [&](int r, R yr, R yi) {
+                cv<R> o;
+                o.x = yr;
+                o.y = yi;
+                st_stream<R>(psi + base + a.off[r], o, NT);
+            });
     }
-    matvec<R, D, M3>(
-        xr, xi, [&](int i) { return mat[i]; },
-        [&](int r, R yr, R yi) {
-            cv<R> o;
-            o.x = yr;
-            o.y = yi;
-            st_stream<R>(psi + base + a.off[r], o, NT);
-        });
 }
 
 // NT as in k_gate_direct_ptr.
@@ -125,24 +127,26 @@ template <typename R, int K, bool M3, bool NT = false>
 __global__ void __launch_bounds__(256) k_gate_direct_val(cv<R>* __restrict__ psi, uint64_t ngroups, DirectArgs<K> a,
                                                          ValMat<R, (M3 ? 3 : 2) * (1 << (2 * K))> mat) {
     constexpr int D = 1 << K;
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    const uint64_t base = insert_zero_bits<K>(g, a.pos);
-    R xr[D], xi[D];
+    // one group per thread; the grid is capped above 2^32 threads
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+        const uint64_t base = insert_zero_bits<K>(g, a.pos);
+        R xr[D], xi[D];
 #pragma unroll
-    for (int s = 0; s < D; ++s) {
-        const cv<R> v = ld_stream<R>(psi + base + a.off[s], NT);
-        xr[s] = v.x;
-        xi[s] = v.y;
+        for (int s = 0; s < D; ++s) {
+            const cv<R> v = ld_stream<R>(psi + base + a.off[s], NT);
+            xr[s] = v.x;
+            xi[s] = v.y;
+        }
+        matvec<R, D, M3>(
+            xr, xi, [&](int i) { return mat.v[i]; },
+            [&](int r, R yr, R yi) {
+                cv<R> o;
+                o.x = yr;
+                o.y = yi;
+                st_stream<R>(psi + base + a.off[r], o, NT);
+            });
     }
-    matvec<R, D, M3>(
-        xr, xi, [&](int i) { return mat.v[i]; },
-        [&](int r, R yr, R yi) {
-            cv<R> o;
-            o.x = yr;
-            o.y = yi;
-            st_stream<R>(psi + base + a.off[r], o, NT);
-        });
 }
 
 template <typename R, int K, bool M3>
@@ -153,34 +157,37 @@ __global__ void __launch_bounds__(64) k_gate_tiled(cv<R>* __restrict__ psi, Tile
     const uint32_t lane = threadIdx.x;
     uint32_t pl, pg;
     tile_lane_positions<K>(lane, a, pl, pg);
-    const uint64_t t = blockIdx.x;
-    const uint64_t base = deposit_bits(t, a.rest) + lane;
-    {
-        cv<R> v[D];
+    // one tile per block; the grid is capped above 2^32 threads
+    for (uint64_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+        const uint64_t base = deposit_bits(t, a.rest) + lane;
+        {
+            cv<R> v[D];
 #pragma unroll
-        for (int m = 0; m < D; ++m) v[m] = psi[base + a.off[m]];
+            for (int m = 0; m < D; ++m) v[m] = psi[base + a.off[m]];
 #pragma unroll
-        for (int m = 0; m < D; ++m) tile[pl ^ a.pm[m]] = v[m];
+            for (int m = 0; m < D; ++m) tile[pl ^ a.pm[m]] = v[m];
+        }
+        __syncthreads();
+        R xr[D], xi[D];
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const cv<R> x = tile[pg ^ a.ts[s]];
+            xr[s] = x.x;
+            xi[s] = x.y;
+        }
+        matvec<R, D, M3>(
+            xr, xi, [&](int i) { return mat[i]; },
+            [&](int r, R yr, R yi) {
+                cv<R> o;
+                o.x = yr;
+                o.y = yi;
+                tile[pg ^ a.ts[r]] = o;
+            });
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < D; ++m) psi[base + a.off[m]] = tile[pl ^ a.pm[m]];
+        __syncthreads();  // the tile is rewritten by the next iteration
     }
-    __syncthreads();
-    R xr[D], xi[D];
-#pragma unroll
-    for (int s = 0; s < D; ++s) {
-        const cv<R> x = tile[pg ^ a.ts[s]];
-        xr[s] = x.x;
-        xi[s] = x.y;
-    }
-    matvec<R, D, M3>(
-        xr, xi, [&](int i) { return mat[i]; },
-        [&](int r, R yr, R yi) {
-            cv<R> o;
-            o.x = yr;
-            o.y = yi;
-            tile[pg ^ a.ts[r]] = o;
-        });
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < D; ++m) psi[base + a.off[m]] = tile[pl ^ a.pm[m]];
 }
 
 template <typename R>
@@ -315,8 +322,9 @@ __global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t 
     for (; i < count; i += stride) one(i, ld_stream<R>(psi + i, NT));
 }
 
+// blocks for `items` threads, capped below 2^32 threads (the kernels loop)
 inline unsigned grid_for(uint64_t items, unsigned block) {
-    const uint64_t g = (items + block - 1) / block;
+    const uint64_t g = std::min<uint64_t>((items + block - 1) / block, kMaxDispatchItems / block);
     return (unsigned)std::max<uint64_t>(1, g);
 }
 
@@ -359,8 +367,8 @@ template <typename R, int K, bool M3>
 hipError_t run_tiled(void* psi, int n, hipStream_t st, const GateDesc& g) {
     TiledArgs<K> a;
     fill_tiled_args<K>(a, n, g.q);
-    if (a.ntiles > 0x7fffffffULL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_gate_tiled<R, K, M3>), dim3((unsigned)a.ntiles), dim3(64), 0, st, (cv<R>*)psi, a,
+    const unsigned blocks = grid_for(a.ntiles, 64);
+    hipLaunchKernelGGL((k_gate_tiled<R, K, M3>), dim3(blocks), dim3(64), 0, st, (cv<R>*)psi, a,
                        (const R*)g.dmat);
     return hipGetLastError();
 }
@@ -503,7 +511,7 @@ hipError_t run_diag(void* psi, int n, hipStream_t st, const uint32_t* q, const d
     }
     const uint64_t count = 1ULL << n;
     // one thread per four amplitudes: every load of a thread issued at once
-    const unsigned blocks = (unsigned)std::min<uint64_t>(grid_for((count + 3) / 4, 256), 1ULL << 30);
+    const unsigned blocks = grid_for((count + 3) / 4, 256);
     if (count >= kStreamMinCount && g_direct_nt != 0)
         hipLaunchKernelGGL((k_diag<R, K, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d);
     else

@@ -14,9 +14,11 @@ namespace iqg {
 
 namespace {
 
+// blocks for `items` threads, at most `cap` and below 2^32 threads in all
+// (every kernel launched with it loops over its items)
 inline unsigned blocks_for(uint64_t items, unsigned block, uint64_t cap) {
     uint64_t g = (items + block - 1) / block;
-    g = std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+    g = std::max<uint64_t>(1, std::min<uint64_t>({g, cap, kMaxDispatchItems / block}));
     return (unsigned)g;
 }
 
@@ -431,16 +433,17 @@ __global__ void __launch_bounds__(kTotalsThreads) k_scan_totals(double* __restri
 // (reference core/src/statevector.cpp:248-254).
 __global__ void __launch_bounds__(256) k_search(const double* __restrict__ cdf, uint64_t count, const double* __restrict__ u,
                                                 uint64_t shots, uint64_t* __restrict__ out) {
-    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= shots) return;
-    const double x = u[s];
-    uint64_t lo = 0, hi = count;
-    while (lo < hi) {
-        const uint64_t mid = lo + ((hi - lo) >> 1);
-        if (cdf[mid] < x) lo = mid + 1;
-        else hi = mid;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < shots; s += stride) {
+        const double x = u[s];
+        uint64_t lo = 0, hi = count;
+        while (lo < hi) {
+            const uint64_t mid = lo + ((hi - lo) >> 1);
+            if (cdf[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
+        out[s] = lo < count ? lo : count - 1;
     }
-    out[s] = lo < count ? lo : count - 1;
 }
 
 // ---------------------------------------------------------- partial trace
@@ -686,7 +689,7 @@ size_t scan_scratch_doubles(uint64_t count) { return (size_t)((count + kScanChun
 
 hipError_t launch_cdf(const void* psi, uint64_t count, int prec, double* cdf, double* scratch, hipStream_t st) {
     const uint64_t nchunks = (count + kScanChunk - 1) / kScanChunk;
-    if (nchunks > 0x7fffffffULL) return hipErrorInvalidValue;
+    if (nchunks > kMaxDispatchItems / kScanThreads) return hipErrorInvalidValue;  // one block per chunk
     if (prec == kC128)
         hipLaunchKernelGGL(k_cdf_sums<double>, dim3((unsigned)nchunks), dim3(kScanThreads), 0, st, (const double2*)psi,
                            count, scratch);
