@@ -150,44 +150,42 @@ __global__ void __launch_bounds__(256) k_gate_direct_val(cv<R>* __restrict__ psi
 }
 
 template <typename R, int K, bool M3>
-__global__ void __launch_bounds__(64) k_gate_tiled(cv<R>* __restrict__ psi, TiledArgs<K> a, const R* __restrict__ mat) {
+__global__ void __launch_bounds__(64) k_gate_tiled(cv<R>* __restrict__ psi, TiledArgs<K> a, const R* __restrict__ mat,
+                                                   uint64_t t0) {
     constexpr int D = 1 << K;
     constexpr int T = 64 * D;
     __shared__ cv<R> tile[T];
     const uint32_t lane = threadIdx.x;
     uint32_t pl, pg;
     tile_lane_positions<K>(lane, a, pl, pg);
-    // one tile per block; the grid is capped above 2^32 threads
-    for (uint64_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-        const uint64_t base = deposit_bits(t, a.rest) + lane;
-        {
-            cv<R> v[D];
+    const uint64_t t = t0 + blockIdx.x;  // one tile per block
+    const uint64_t base = deposit_bits(t, a.rest) + lane;
+    {
+        cv<R> v[D];
 #pragma unroll
-            for (int m = 0; m < D; ++m) v[m] = psi[base + a.off[m]];
+        for (int m = 0; m < D; ++m) v[m] = psi[base + a.off[m]];
 #pragma unroll
-            for (int m = 0; m < D; ++m) tile[pl ^ a.pm[m]] = v[m];
-        }
-        __syncthreads();
-        R xr[D], xi[D];
-#pragma unroll
-        for (int s = 0; s < D; ++s) {
-            const cv<R> x = tile[pg ^ a.ts[s]];
-            xr[s] = x.x;
-            xi[s] = x.y;
-        }
-        matvec<R, D, M3>(
-            xr, xi, [&](int i) { return mat[i]; },
-            [&](int r, R yr, R yi) {
-                cv<R> o;
-                o.x = yr;
-                o.y = yi;
-                tile[pg ^ a.ts[r]] = o;
-            });
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < D; ++m) psi[base + a.off[m]] = tile[pl ^ a.pm[m]];
-        __syncthreads();  // the tile is rewritten by the next iteration
+        for (int m = 0; m < D; ++m) tile[pl ^ a.pm[m]] = v[m];
     }
+    __syncthreads();
+    R xr[D], xi[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        const cv<R> x = tile[pg ^ a.ts[s]];
+        xr[s] = x.x;
+        xi[s] = x.y;
+    }
+    matvec<R, D, M3>(
+        xr, xi, [&](int i) { return mat[i]; },
+        [&](int r, R yr, R yi) {
+            cv<R> o;
+            o.x = yr;
+            o.y = yi;
+            tile[pg ^ a.ts[r]] = o;
+        });
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < D; ++m) psi[base + a.off[m]] = tile[pl ^ a.pm[m]];
 }
 
 template <typename R>
@@ -367,9 +365,12 @@ template <typename R, int K, bool M3>
 hipError_t run_tiled(void* psi, int n, hipStream_t st, const GateDesc& g) {
     TiledArgs<K> a;
     fill_tiled_args<K>(a, n, g.q);
-    const unsigned blocks = grid_for(a.ntiles, 64);
-    hipLaunchKernelGGL((k_gate_tiled<R, K, M3>), dim3(blocks), dim3(64), 0, st, (cv<R>*)psi, a,
-                       (const R*)g.dmat);
+    // one tile per block; a dispatch holds fewer than 2^32 work-items, so
+    // more tiles than that take several launches (states above 34 qubits)
+    const uint64_t maxb = kMaxDispatchItems / 64;
+    for (uint64_t t0 = 0; t0 < a.ntiles; t0 += maxb)
+        hipLaunchKernelGGL((k_gate_tiled<R, K, M3>), dim3((unsigned)std::min<uint64_t>(a.ntiles - t0, maxb)), dim3(64), 0,
+                           st, (cv<R>*)psi, a, (const R*)g.dmat, t0);
     return hipGetLastError();
 }
 
