@@ -1913,6 +1913,10 @@ bool prepare(Circuit& c, const double* params, size_t nparams) {
         set_error(kErrInvalidArgument, "no observable set");
         return false;
     }
+    if (!params && nparams) {
+        set_error(kErrInvalidArgument, "null parameter array");
+        return false;
+    }
     if (!check_params(c, nparams)) return false;
     if (!c.compiled) compile(c);
     return true;

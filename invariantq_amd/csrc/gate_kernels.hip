@@ -308,16 +308,21 @@ __global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t 
         o.y = dr * v.y + di * v.x;
         st_stream<R>(psi + i, o, NT);
     };
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < count; i += 4 * stride) {
-        cv<R> v[4];
+    // a block takes spans of 4 * blockDim consecutive amplitudes, so a
+    // wave's four accesses fall in one span
+    const uint64_t span = 4ULL * blockDim.x;
+    for (uint64_t b = (uint64_t)blockIdx.x * span; b < count; b += (uint64_t)gridDim.x * span) {
+        const uint64_t i = b + threadIdx.x;
+        if (b + span <= count) {
+            cv<R> v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = ld_stream<R>(psi + i + u * stride, NT);
+            for (int u = 0; u < 4; ++u) v[u] = ld_stream<R>(psi + i + u * blockDim.x, NT);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) one(i + u * stride, v[u]);
+            for (int u = 0; u < 4; ++u) one(i + u * blockDim.x, v[u]);
+        } else {
+            for (uint64_t j = i; j < count; j += blockDim.x) one(j, ld_stream<R>(psi + j, NT));
+        }
     }
-    for (; i < count; i += stride) one(i, ld_stream<R>(psi + i, NT));
 }
 
 // blocks for `items` threads, capped below 2^32 threads (the kernels loop)

@@ -33,9 +33,26 @@ __device__ __forceinline__ void grid_stride(uint64_t count, Body&& body) {
     for (; i < count; i += stride) body(i, stride, 1);
 }
 
+// Read+write passes: a block takes spans of U * blockDim consecutive
+// elements (element span + u * blockDim + tid), so a wave's U accesses fall
+// in one span instead of U regions a grid apart; the body sees the same
+// (first element, stride, count) triple as with grid_stride.
+template <int U, typename Body>
+__device__ __forceinline__ void block_spans(uint64_t count, Body&& body) {
+    const uint64_t span = (uint64_t)U * blockDim.x;
+    for (uint64_t b = (uint64_t)blockIdx.x * span; b < count; b += (uint64_t)gridDim.x * span) {
+        const uint64_t i = b + threadIdx.x;
+        if (b + span <= count) {
+            body(i, (uint64_t)blockDim.x, U);
+        } else if (i < count) {
+            body(i, (uint64_t)blockDim.x, (int)std::min<uint64_t>(U, (count - i + blockDim.x - 1) / blockDim.x));
+        }
+    }
+}
+
 template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_init_basis(cv<R>* __restrict__ psi, uint64_t count, uint64_t index) {
-    grid_stride<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+    block_spans<4>(count, [&](uint64_t i, uint64_t stride, int n) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (u >= n) break;
@@ -55,7 +72,7 @@ template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_collapse(cv<R>* __restrict__ psi, uint64_t count, int target, int outcome,
                                                   double factor) {
     const R f = (R)factor;
-    grid_stride<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+    block_spans<4>(count, [&](uint64_t i, uint64_t stride, int n) {
         cv<R> v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -79,7 +96,7 @@ __global__ void __launch_bounds__(256) k_collapse(cv<R>* __restrict__ psi, uint6
 template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_scale(cv<R>* __restrict__ psi, uint64_t count, double factor) {
     const R f = (R)factor;
-    grid_stride<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+    block_spans<4>(count, [&](uint64_t i, uint64_t stride, int n) {
         cv<R> v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
