@@ -94,6 +94,56 @@ __device__ __forceinline__ void matvec(const R (&xr)[D], const R (&xi)[D], Get m
     }
 }
 
+// One- and two-qubit gates whose targets all lie below bit 6, inside the 64
+// consecutive amplitudes of a wave: lane l loads amplitude base + l, so each
+// load and store instruction covers whole 128-byte lines (the direct kernel's
+// lanes read strided halves of lines when a target is inside a line), takes
+// its group's other members from the lanes holding them, and stores its own
+// output row.  The product is matvec's on the same values, so the result
+// equals the direct kernel's bit for bit.
+template <int K>
+struct LaneArgs {
+    uint32_t q[K];           // targets, list order (local bit K-1-m <-> q[m])
+    uint32_t tmask;          // lane bits of the targets
+    uint32_t offl[1 << K];   // lane offset of local index c
+};
+
+template <typename R, int K, bool NT>
+__global__ void __launch_bounds__(256) k_gate_lanes(cv<R>* __restrict__ psi, uint64_t count, LaneArgs<K> a,
+                                                    ValMat<R, 2 * (1 << (2 * K))> mat) {
+    constexpr int D = 1 << K;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t l = 0;  // this lane's local index
+#pragma unroll
+    for (int m = 0; m < K; ++m) l |= ((lane >> a.q[m]) & 1u) << (K - 1 - m);
+    const uint32_t others = lane & ~a.tmask;
+    // count is a multiple of 64 and every wave starts 64-aligned, so the
+    // lanes of a wave run the loop together (the shuffles need all of them)
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const cv<R> v = ld_stream<R>(psi + i, NT);
+        R xr[D], xi[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const int src = (int)(others | a.offl[c]);
+            xr[c] = __shfl(v.x, src, 64);
+            xi[c] = __shfl(v.y, src, 64);
+        }
+        cv<R> o;
+        o.x = v.x;
+        o.y = v.y;
+        matvec<R, D, false>(
+            xr, xi, [&](int j) { return mat.v[j]; },
+            [&](int r, R yr, R yi) {
+                if ((uint32_t)r == l) {
+                    o.x = yr;
+                    o.y = yi;
+                }
+            });
+        st_stream<R>(psi + i, o, NT);
+    }
+}
+
 // NT: streaming loads and stores, for target sets with no target inside a
 // 128-byte line (each load instruction then covers whole lines).
 template <typename R, int K, bool M3, bool NT = false>
@@ -347,6 +397,38 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
     const uint32_t line_bits = std::is_same<R, double>::value ? 3u : 4u;
     bool whole = n >= 24 && g_direct_nt != 0;
     for (int m = 0; m < K; ++m) whole = whole && g.q[m] >= line_bits;
+    if constexpr (K <= 2) {
+        // every target below bit 6 and one inside a line: the lane-exchange form
+        bool lanes = n >= 6;
+        bool inside = false;
+        for (int m = 0; m < K; ++m) {
+            lanes = lanes && g.q[m] < 6;
+            inside = inside || g.q[m] < line_bits;
+        }
+        if (lanes && inside) {
+            LaneArgs<K> la;
+            la.tmask = 0;
+            for (int m = 0; m < K; ++m) {
+                la.q[m] = g.q[m];
+                la.tmask |= 1u << g.q[m];
+            }
+            for (int c = 0; c < D; ++c) {
+                uint32_t o = 0;
+                for (int m = 0; m < K; ++m)
+                    if ((c >> (K - 1 - m)) & 1) o |= 1u << g.q[m];
+                la.offl[c] = o;
+            }
+            ValMat<R, 2 * D * D> mv;
+            pack_gate_matrix(std::is_same<R, double>::value ? kC128 : kC64, K, false, g.re, g.im, mv.v);
+            const uint64_t count = 1ULL << n;
+            const unsigned lblocks = grid_for(count, 256);
+            if (n >= 24 && g_direct_nt != 0)
+                hipLaunchKernelGGL((k_gate_lanes<R, K, true>), dim3(lblocks), dim3(256), 0, st, (cv<R>*)psi, count, la, mv);
+            else
+                hipLaunchKernelGGL((k_gate_lanes<R, K, false>), dim3(lblocks), dim3(256), 0, st, (cv<R>*)psi, count, la, mv);
+            return hipGetLastError();
+        }
+    }
     if constexpr (K <= 3) {
         ValMat<R, (M3 ? 3 : 2) * D * D> mv;
         pack_gate_matrix(std::is_same<R, double>::value ? kC128 : kC64, K, M3, g.re, g.im, mv.v);
