@@ -407,9 +407,9 @@ def primitives_workload(binding, torch, n: int = 30, reps: int = 3):
                      "pct_hbm_peak": round(100 * nbytes / t / 8e12, 1), "timed": how})
 
     g1 = gates.u3(0.3, 1.1, 2.0)
-    for t in (0, 10, 29):
+    for t in (0, 4, 10, 29):
         row(f"a2 apply_1q target {t}", "statevector.cpp:26-47", 2 * N * s, dev(lambda: sv.apply_1q(t, g1)), "events")
-    for q0, q1 in ((0, 1), (5, 17), (28, 29)):
+    for q0, q1 in ((0, 1), (1, 20), (4, 20), (5, 17), (28, 29)):
         row(f"a3 apply_2q dense ({q0},{q1})", "statevector.cpp:49-87", 2 * N * s,
             dev(lambda: sv.apply_2q(q0, q1, u2)), "events")
     row("a3 apply_2q CX (3,20)", "statevector.cpp:49-87", 2 * N * s, dev(lambda: sv.apply_2q(3, 20, gates.CX)), "events")

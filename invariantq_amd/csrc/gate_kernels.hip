@@ -144,6 +144,65 @@ __global__ void __launch_bounds__(256) k_gate_lanes(cv<R>* __restrict__ psi, uin
     }
 }
 
+// Two-qubit operations with one target tl inside a 128-byte line and the
+// other, th, at bit 6 or above (a density matrix's one-qubit channels on
+// qubits 0-2: superoperators on qubits t and t + n): lane l holds the two
+// amplitudes i0 = base + l and i0 + 2^th (two runs of whole lines), takes the
+// tl-partners of both from lane l ^ 2^tl, and stores the two rows whose tl
+// bit is its own.  matvec's product on the same values: the direct kernel's
+// result bit for bit.
+template <typename R, bool NT>
+__global__ void __launch_bounds__(256) k_gate_lanes_mixed(cv<R>* __restrict__ psi, uint64_t npairs, uint32_t tl,
+                                                          uint32_t th, int low_first, ValMat<R, 32> mat) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lb = (lane >> tl) & 1u;
+    const int src = (int)(lane ^ (1u << tl));
+    const uint64_t lo = (1ULL << th) - 1;
+    // npairs = 2^(n-1) is a multiple of 64 and waves start 64-aligned: the
+    // lanes of a wave run the loop together
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < npairs; g += stride) {
+        const uint64_t i0 = ((g & ~lo) << 1) | (g & lo), i1 = i0 | (1ULL << th);
+        const cv<R> v0 = ld_stream<R>(psi + i0, NT), v1 = ld_stream<R>(psi + i1, NT);
+        cv<R> p0, p1;
+        p0.x = __shfl(v0.x, src, 64);
+        p0.y = __shfl(v0.y, src, 64);
+        p1.x = __shfl(v1.x, src, 64);
+        p1.y = __shfl(v1.y, src, 64);
+        // x[c], c the MSB-first local index over the targets in list order
+        R xr[4], xi[4];
+#pragma unroll
+        for (int low = 0; low < 2; ++low)
+#pragma unroll
+            for (int high = 0; high < 2; ++high) {
+                const cv<R>& mine = high ? v1 : v0;
+                const cv<R>& other = high ? p1 : p0;
+                const bool own = (uint32_t)low == lb;
+                const int c = low_first ? (low << 1) | high : (high << 1) | low;
+                xr[c] = own ? mine.x : other.x;
+                xi[c] = own ? mine.y : other.y;
+            }
+        cv<R> o0 = v0, o1 = v1;
+        matvec<R, 4, false>(
+            xr, xi, [&](int j) { return mat.v[j]; },
+            [&](int r, R yr, R yi) {
+                const uint32_t low = low_first ? ((uint32_t)r >> 1) : ((uint32_t)r & 1u);
+                const uint32_t high = low_first ? ((uint32_t)r & 1u) : ((uint32_t)r >> 1);
+                if (low == lb) {
+                    if (high) {
+                        o1.x = yr;
+                        o1.y = yi;
+                    } else {
+                        o0.x = yr;
+                        o0.y = yi;
+                    }
+                }
+            });
+        st_stream<R>(psi + i0, o0, NT);
+        st_stream<R>(psi + i1, o1, NT);
+    }
+}
+
 // NT: streaming loads and stores, for target sets with no target inside a
 // 128-byte line (each load instruction then covers whole lines).
 template <typename R, int K, bool M3, bool NT = false>
@@ -382,6 +441,7 @@ inline unsigned grid_for(uint64_t items, unsigned block) {
 }
 
 static int g_direct_nt = 1;  // streaming hint in the direct kernel: 1 where lines are whole, 0 never
+static uint32_t kLaneLow = 6;  // lane-exchange forms for targets below this bit
 
 template <typename R, int K, bool M3>
 hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
@@ -405,7 +465,26 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
             lanes = lanes && g.q[m] < 6;
             inside = inside || g.q[m] < line_bits;
         }
-        if (lanes && inside) {
+        if (K == 2 && n >= 7) {
+            // one target inside a line, the other at bit 6 or above
+            const uint32_t q0 = g.q[0], q1 = g.q[1];
+            const bool low0 = q0 < kLaneLow && q1 >= 6, low1 = q1 < kLaneLow && q0 >= 6;
+            if (low0 || low1) {
+                ValMat<R, 32> mv;
+                pack_gate_matrix(std::is_same<R, double>::value ? kC128 : kC64, 2, false, g.re, g.im, mv.v);
+                const uint64_t npairs = 1ULL << (n - 1);
+                const unsigned pblocks = grid_for(npairs, 256);
+                const uint32_t tl = low0 ? q0 : q1, th = low0 ? q1 : q0;
+                if (n >= 24 && g_direct_nt != 0)
+                    hipLaunchKernelGGL((k_gate_lanes_mixed<R, true>), dim3(pblocks), dim3(256), 0, st, (cv<R>*)psi, npairs,
+                                       tl, th, low0 ? 1 : 0, mv);
+                else
+                    hipLaunchKernelGGL((k_gate_lanes_mixed<R, false>), dim3(pblocks), dim3(256), 0, st, (cv<R>*)psi,
+                                       npairs, tl, th, low0 ? 1 : 0, mv);
+                return hipGetLastError();
+            }
+        }
+        if (lanes && (inside || kLaneLow > line_bits)) {
             LaneArgs<K> la;
             la.tmask = 0;
             for (int m = 0; m < K; ++m) {
