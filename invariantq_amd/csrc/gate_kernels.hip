@@ -419,15 +419,15 @@ __global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t 
     };
     // a block takes spans of 4 * blockDim consecutive amplitudes, so a
     // wave's four accesses fall in one span
-    const uint64_t span = 4ULL * blockDim.x;
+    const uint64_t span = (uint64_t)kPassSpan * blockDim.x;
     for (uint64_t b = (uint64_t)blockIdx.x * span; b < count; b += (uint64_t)gridDim.x * span) {
         const uint64_t i = b + threadIdx.x;
         if (b + span <= count) {
-            cv<R> v[4];
+            cv<R> v[kPassSpan];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = ld_stream<R>(psi + i + u * blockDim.x, NT);
+            for (int u = 0; u < kPassSpan; ++u) v[u] = ld_stream<R>(psi + i + u * blockDim.x, NT);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) one(i + u * blockDim.x, v[u]);
+            for (int u = 0; u < kPassSpan; ++u) one(i + u * blockDim.x, v[u]);
         } else {
             for (uint64_t j = i; j < count; j += blockDim.x) one(j, ld_stream<R>(psi + j, NT));
         }
@@ -678,7 +678,7 @@ hipError_t run_diag(void* psi, int n, hipStream_t st, const uint32_t* q, const d
     }
     const uint64_t count = 1ULL << n;
     // one thread per four amplitudes: every load of a thread issued at once
-    const unsigned blocks = grid_for((count + 3) / 4, 256);
+    const unsigned blocks = grid_for((count + kPassSpan - 1) / kPassSpan, 256);
     if (count >= kStreamMinCount && g_direct_nt != 0)
         hipLaunchKernelGGL((k_diag<R, K, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d);
     else

@@ -33,6 +33,10 @@ __device__ __forceinline__ void st_stream(cv<R>* p, cv<R> v, bool nt) {
     __builtin_nontemporal_store(w, reinterpret_cast<nv*>(p));
 }
 
+// Amplitudes per thread in the whole-state read+write passes and the
+// diagonal operator (a block takes spans of kPassSpan * blockDim).
+constexpr int kPassSpan = 1;
+
 // States from this many amplitudes up are far beyond L2 and the Infinity
 // Cache, where whole-state streaming passes use the hint.
 constexpr uint64_t kStreamMinCount = 1ULL << 26;

@@ -52,7 +52,7 @@ __device__ __forceinline__ void block_spans(uint64_t count, Body&& body) {
 
 template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_init_basis(cv<R>* __restrict__ psi, uint64_t count, uint64_t index) {
-    block_spans<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+    block_spans<kPassSpan>(count, [&](uint64_t i, uint64_t stride, int n) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (u >= n) break;
@@ -72,7 +72,7 @@ template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_collapse(cv<R>* __restrict__ psi, uint64_t count, int target, int outcome,
                                                   double factor) {
     const R f = (R)factor;
-    block_spans<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+    block_spans<kPassSpan>(count, [&](uint64_t i, uint64_t stride, int n) {
         cv<R> v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) k_collapse(cv<R>* __restrict__ psi, uint6
 template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_scale(cv<R>* __restrict__ psi, uint64_t count, double factor) {
     const R f = (R)factor;
-    block_spans<4>(count, [&](uint64_t i, uint64_t stride, int n) {
+    block_spans<kPassSpan>(count, [&](uint64_t i, uint64_t stride, int n) {
         cv<R> v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -575,7 +575,7 @@ using camp_ptr = const cv<R>*;
 
 // element-wise passes: one thread per four amplitudes (every load of a thread
 // in flight at once)
-inline unsigned pass_blocks(uint64_t count) { return blocks_for((count + 3) / 4, 256, 1ULL << 30); }
+inline unsigned pass_blocks(uint64_t count) { return blocks_for((count + kPassSpan - 1) / kPassSpan, 256, 1ULL << 30); }
 
 hipError_t launch_init_basis(void* psi, int n, int prec, uint64_t index, hipStream_t st) {
     const uint64_t count = 1ULL << n;
