@@ -441,7 +441,7 @@ inline unsigned grid_for(uint64_t items, unsigned block) {
 }
 
 static int g_direct_nt = 1;  // streaming hint in the direct kernel: 1 where lines are whole, 0 never
-static uint32_t kLaneLow = 6;  // lane-exchange forms for targets below this bit
+constexpr uint32_t kLaneLow = 6;  // lane-exchange forms for targets below this bit (a wave's 64 lanes)
 
 template <typename R, int K, bool M3>
 hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
@@ -458,17 +458,17 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
     bool whole = n >= 24 && g_direct_nt != 0;
     for (int m = 0; m < K; ++m) whole = whole && g.q[m] >= line_bits;
     if constexpr (K <= 2) {
-        // every target below bit 6 and one inside a line: the lane-exchange form
-        bool lanes = n >= 6;
-        bool inside = false;
-        for (int m = 0; m < K; ++m) {
-            lanes = lanes && g.q[m] < 6;
-            inside = inside || g.q[m] < line_bits;
-        }
-        if (K == 2 && n >= 7) {
-            // one target inside a line, the other at bit 6 or above
+        // Lane-exchange forms for targets below bit 6, on states beyond the
+        // caches (from 24 qubits; measured at 14-qubit density matrices and 30
+        // qubits).  On a 20-qubit state, which stays in the Infinity Cache, the
+        // shuffles' latency made them slower (12.3 against 7.9 us per pass).
+        const bool big = n >= 24;
+        bool lanes = big;  // every target below bit 6
+        for (int m = 0; m < K; ++m) lanes = lanes && g.q[m] < kLaneLow;
+        if (K == 2 && big) {
+            // one target below bit 6, the other at bit 6 or above
             const uint32_t q0 = g.q[0], q1 = g.q[1];
-            const bool low0 = q0 < kLaneLow && q1 >= 6, low1 = q1 < kLaneLow && q0 >= 6;
+            const bool low0 = q0 < kLaneLow && q1 >= kLaneLow, low1 = q1 < kLaneLow && q0 >= kLaneLow;
             if (low0 || low1) {
                 ValMat<R, 32> mv;
                 pack_gate_matrix(std::is_same<R, double>::value ? kC128 : kC64, 2, false, g.re, g.im, mv.v);
@@ -484,7 +484,7 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
                 return hipGetLastError();
             }
         }
-        if (lanes && (inside || kLaneLow > line_bits)) {
+        if (lanes) {
             LaneArgs<K> la;
             la.tmask = 0;
             for (int m = 0; m < K; ++m) {
