@@ -107,7 +107,8 @@ const char* iq_backend_name(void);
 void        iq_set_gate_path(int path);
 /* Matrix-core kernel options. Defaults are per target set; 0 / -1 restores them.
  * - batches_per_iteration: groups-of-16 batches per wave iteration (1, 2, 4). Default:
- *   2 for complex128 sets whose instructions cover whole 128-byte lines, else 4.
+ *   2 for complex128 sets whose instructions cover whole 128-byte lines and for
+ *   complex64 sets with a target at qubit 10 or above, else 4.
  * - matrix_in_lds: matrix fragments in LDS (1) or VGPRs (0; default).
  * - nontemporal: streaming loads and stores (1) or cached (0). Default: on when every
  *   instruction covers whole 128-byte lines. 0 also keeps the direct, diagonal and
