@@ -228,7 +228,7 @@ bool Engine::apply(int K, const uint32_t* q, const double* re, const double* im,
     GatePlan p = plan_gate(K, q, re, im, path);
     const void* dmat = nullptr;
     if (p.kind == GatePlan::kDevice) {
-        dmat = upload_matrix(K, p.m3, re, im, p.mfma ? p.g.q : nullptr);
+        dmat = upload_matrix(p.g.K, p.m3, p.g.re, p.g.im, p.mfma ? p.g.q : nullptr);
         if (!dmat) return false;
     }
     return launch_plan(p, dmat);
