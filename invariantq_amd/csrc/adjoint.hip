@@ -603,6 +603,21 @@ struct ObsGroupT {
     uint32_t xl;  // flip mask on tile bits
     int first, count, nre;  // terms, relative to the pass's first term
     int nre0, nim0;
+    // After the real run's nre0 tile-constant terms come nch terms that the
+    // energy kernel takes as ccount chain classes (table index cfirst); the
+    // lambda kernel takes them term by term.
+    int nch, cfirst, ccount;
+};
+
+// A chain class: real-part terms with the same coefficient whose sign bits on
+// the tile are one bit a (shift 0) or two bits a, a + shift, and none fixed
+// by the tile number.  mask holds every such a (distinct), so for element j
+// the class adds coeff * (count - 2 popcount(f(j) & mask)), f(j) = j for
+// shift 0 and j ^ (j >> shift) otherwise: a uniform ZZ chain in one popcount.
+struct ObsChainT {
+    uint32_t mask;
+    int shift, count, pad;
+    double coeff;
 };
 
 struct ObsTermT {
@@ -746,7 +761,8 @@ __global__ void __launch_bounds__(512) k_obs_tile(const cv<R>* __restrict__ psi,
 template <typename R, bool SKIP>
 __global__ void __launch_bounds__(512) k_obs_energy_tile(const cv<R>* __restrict__ psi, ObsPass op,
                                                          const ObsGroupT* __restrict__ groups,
-                                                         const ObsTermT* __restrict__ terms, double* __restrict__ part) {
+                                                         const ObsTermT* __restrict__ terms,
+                                                         const ObsChainT* __restrict__ chains, double* __restrict__ part) {
     constexpr int E = 4;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double wsum[16];
@@ -816,6 +832,19 @@ __global__ void __launch_bounds__(512) k_obs_energy_tile(const cv<R>* __restrict
                     fr[q] = fr0;
                     fi[q] = fi0;
                 }
+                // chain classes: coeff * (count - 2 popcount(f(j) & mask))
+                const int c0 = gp[g].cfirst, c1 = c0 + gp[g].ccount;
+                for (int ci = c0; ci < c1; ++ci) {
+                    const int c = __builtin_amdgcn_readfirstlane(ci);
+                    const uint32_t mask = chains[c].mask;
+                    const int sh = chains[c].shift, cnt = chains[c].count;
+                    const double cc = chains[c].coeff;
+#pragma unroll
+                    for (int q = 0; q < E; ++q) {
+                        const uint32_t f = sh ? (jj[q] ^ (jj[q] >> sh)) : jj[q];
+                        fr[q] += cc * (double)(cnt - 2 * (int)__popc(f & mask));
+                    }
+                }
                 auto run = [&](int ka, int kb, double(&acc)[E]) __attribute__((always_inline)) {
                     for (int k = ka; k < kb; ++k) {
                         const double c = csg[k];
@@ -829,7 +858,7 @@ __global__ void __launch_bounds__(512) k_obs_energy_tile(const cv<R>* __restrict
                         }
                     }
                 };
-                run(kr, km, fr);
+                run(kr + gp[g].nch, km, fr);
                 run(ki, k1, fi);
 #pragma unroll
                 for (int q = 0; q < E; ++q) {
@@ -943,6 +972,7 @@ struct ObsPlan {
     void* otab_dev = nullptr;
     const ObsGroupT* ogroups = nullptr;
     const ObsTermT* oterms = nullptr;
+    const ObsChainT* ochains = nullptr;
     ObsTable far{};
     void* far_dev = nullptr;
     double* opart = nullptr;
@@ -960,6 +990,7 @@ struct ObsPlan {
         opart = nullptr;
         ogroups = nullptr;
         oterms = nullptr;
+        ochains = nullptr;
         far = ObsTable{};
         opass.clear();
         opart_len = 0;
@@ -1257,7 +1288,7 @@ bool launch_observable_plan(const ObsPlan& p, const void* amp, void* lam, uint64
             auto kern = skip ? k_obs_energy_tile<R, true> : k_obs_energy_tile<R, false>;
             hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(512), lds, st, (const cv<R>*)amp, op, p.ogroups, p.oterms,
-                               p.opart + m * off);
+                               p.ochains, p.opart + m * off);
         } else {
             auto kern = first ? k_obs_tile<R, true, true> : k_obs_tile<R, true, false>;
             hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1634,6 +1665,7 @@ bool plan_observable_tiles(ObsPlan& p, int n, int prec, uint64_t dim, const std:
     }
     std::vector<ObsGroupT> G;
     std::vector<ObsTermT> Tm;
+    std::vector<ObsChainT> Ch;
     p.opass.clear();
     for (int k = 0; k < npass; ++k) {
         ObsPass op{};
@@ -1653,22 +1685,62 @@ bool plan_observable_tiles(ObsPlan& p, int n, int prec, uint64_t dim, const std:
             split(pd[k], xm[gi], gg.xl, fx);
             gg.first = op.tcount;
             gg.count = nt;
-            gg.nre = gg.nre0 = gg.nim0 = 0;
+            gg.nre = gg.nre0 = gg.nim0 = gg.nch = 0;
+            gg.cfirst = (int)Ch.size();
+            gg.ccount = 0;
             // the real-part run, then the imaginary-part run; in each, the
-            // terms without sign bits on the tile first
-            for (int imag = 0; imag < 2; ++imag)
-                for (int on_tile = 0; on_tile < 2; ++on_tile)
-                    for (int t = start[gi]; t < start[gi + 1]; ++t) {
-                        const int cls = ny[t] & 3;
-                        if ((cls & 1) != imag) continue;
-                        ObsTermT tt{};
-                        split(pd[k], zy[t], tt.zl, tt.zfix);
-                        if ((tt.zl != 0) != (on_tile != 0)) continue;
-                        tt.coeff = cls >= 2 ? -co[t] : co[t];
-                        Tm.push_back(tt);
-                        if (!imag) ++gg.nre;
-                        if (!on_tile) ++(imag ? gg.nim0 : gg.nre0);
+            // terms without sign bits on the tile first; in the real run the
+            // chain-class terms next, class by class
+            std::vector<ObsTermT> re0, rech, reot, im0, imot;
+            std::vector<int> chain_of;  // class (relative) of each chain term, in re order
+            for (int t = start[gi]; t < start[gi + 1]; ++t) {
+                const int cls = ny[t] & 3;
+                ObsTermT tt{};
+                split(pd[k], zy[t], tt.zl, tt.zfix);
+                tt.coeff = cls >= 2 ? -co[t] : co[t];
+                if (cls & 1) {
+                    (tt.zl ? imot : im0).push_back(tt);
+                    continue;
+                }
+                if (!tt.zl) {
+                    re0.push_back(tt);
+                    continue;
+                }
+                // chain eligibility: one or two tile bits, none fixed by the tile
+                const int nb = __builtin_popcount(tt.zl);
+                int placed = -1;
+                if (tt.zfix == 0 && nb <= 2) {
+                    const uint32_t a = (uint32_t)__builtin_ctz(tt.zl);
+                    const int sh = nb == 2 ? 31 - __builtin_clz(tt.zl) - (int)a : 0;
+                    for (int c = gg.cfirst; c < (int)Ch.size() && placed < 0; ++c)
+                        if (Ch[c].shift == sh && Ch[c].coeff == tt.coeff && !((Ch[c].mask >> a) & 1u)) placed = c;
+                    if (placed < 0) {
+                        ObsChainT nc{};
+                        nc.shift = sh;
+                        nc.coeff = tt.coeff;
+                        Ch.push_back(nc);
+                        placed = (int)Ch.size() - 1;
                     }
+                    Ch[placed].mask |= 1u << a;
+                    Ch[placed].count += 1;
+                    rech.push_back(tt);
+                    chain_of.push_back(placed);
+                } else {
+                    reot.push_back(tt);
+                }
+            }
+            gg.ccount = (int)Ch.size() - gg.cfirst;
+            for (const ObsTermT& t : re0) Tm.push_back(t);
+            for (int c = gg.cfirst; c < (int)Ch.size(); ++c)  // chain terms class by class
+                for (size_t i = 0; i < rech.size(); ++i)
+                    if (chain_of[i] == c) Tm.push_back(rech[i]);
+            for (const ObsTermT& t : reot) Tm.push_back(t);
+            for (const ObsTermT& t : im0) Tm.push_back(t);
+            for (const ObsTermT& t : imot) Tm.push_back(t);
+            gg.nre0 = (int)re0.size();
+            gg.nch = (int)rech.size();
+            gg.nre = (int)(re0.size() + rech.size() + reot.size());
+            gg.nim0 = (int)im0.size();
             G.push_back(gg);
             ++op.gcount;
             op.tcount += nt;
@@ -1698,14 +1770,17 @@ bool plan_observable_tiles(ObsPlan& p, int n, int prec, uint64_t dim, const std:
     p.obs_tiled = false;
     const size_t gbytes = std::max<size_t>(16, G.size() * sizeof(ObsGroupT));
     const size_t tbytes = std::max<size_t>(16, Tm.size() * sizeof(ObsTermT));
-    const size_t toff = (gbytes + 255) & ~size_t(255);
-    if (!hip_ok(hipMalloc(&p.otab_dev, toff + tbytes), "observable tiles")) return false;
+    const size_t cbytes = std::max<size_t>(16, Ch.size() * sizeof(ObsChainT));
+    const size_t toff = (gbytes + 255) & ~size_t(255), coff = (toff + tbytes + 255) & ~size_t(255);
+    if (!hip_ok(hipMalloc(&p.otab_dev, coff + cbytes), "observable tiles")) return false;
     char* od = static_cast<char*>(p.otab_dev);
     if ((!G.empty() && !hip_ok(hipMemcpy(od, G.data(), G.size() * sizeof(ObsGroupT), hipMemcpyHostToDevice), "upload")) ||
-        (!Tm.empty() && !hip_ok(hipMemcpy(od + toff, Tm.data(), Tm.size() * sizeof(ObsTermT), hipMemcpyHostToDevice), "upload")))
+        (!Tm.empty() && !hip_ok(hipMemcpy(od + toff, Tm.data(), Tm.size() * sizeof(ObsTermT), hipMemcpyHostToDevice), "upload")) ||
+        (!Ch.empty() && !hip_ok(hipMemcpy(od + coff, Ch.data(), Ch.size() * sizeof(ObsChainT), hipMemcpyHostToDevice), "upload")))
         return false;
     p.ogroups = reinterpret_cast<const ObsGroupT*>(od);
     p.oterms = reinterpret_cast<const ObsTermT*>(od + toff);
+    p.ochains = reinterpret_cast<const ObsChainT*>(od + coff);
     size_t parts = 0;
     for (const ObsPass& op : p.opass) parts += std::min<uint64_t>(op.d.ntiles, 2048);
     if (!far.empty()) {
