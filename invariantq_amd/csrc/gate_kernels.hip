@@ -96,10 +96,10 @@ __device__ __forceinline__ void matvec(const R (&xr)[D], const R (&xi)[D], Get m
 
 // One- and two-qubit gates whose targets all lie below bit 6, inside the 64
 // consecutive amplitudes of a wave: lane l loads amplitude base + l, so each
-// load and store instruction covers whole 128-byte lines (the direct kernel's
-// lanes read strided halves of lines when a target is inside a line), takes
-// its group's other members from the lanes holding them, and stores its own
-// output row.  The product is matvec's on the same values, so the result
+// wave moves one contiguous run per instruction (the direct kernel's lanes
+// read strided parts of lines when a target is inside a line, and two or four
+// runs otherwise), takes its group's other members from the lanes holding
+// them, and stores its own output row.  The product is matvec's on the same values, so the result
 // equals the direct kernel's bit for bit.
 template <int K>
 struct LaneArgs {
@@ -144,13 +144,13 @@ __global__ void __launch_bounds__(256) k_gate_lanes(cv<R>* __restrict__ psi, uin
     }
 }
 
-// Two-qubit operations with one target tl inside a 128-byte line and the
-// other, th, at bit 6 or above (a density matrix's one-qubit channels on
-// qubits 0-2: superoperators on qubits t and t + n): lane l holds the two
-// amplitudes i0 = base + l and i0 + 2^th (two runs of whole lines), takes the
-// tl-partners of both from lane l ^ 2^tl, and stores the two rows whose tl
-// bit is its own.  matvec's product on the same values: the direct kernel's
-// result bit for bit.
+// Two-qubit operations with one target tl below bit 6 and the other, th, at
+// bit 6 or above (a density matrix's one-qubit channels on qubits 0-5:
+// superoperators on qubits t and t + n): lane l holds the two amplitudes
+// i0 = base + l and i0 + 2^th (two runs of whole lines, where the direct
+// kernel's four group members make four), takes the tl-partners of both from
+// lane l ^ 2^tl, and stores the two rows whose tl bit is its own.  matvec's
+// product on the same values: the direct kernel's result bit for bit.
 template <typename R, bool NT>
 __global__ void __launch_bounds__(256) k_gate_lanes_mixed(cv<R>* __restrict__ psi, uint64_t npairs, uint32_t tl,
                                                           uint32_t th, int low_first, ValMat<R, 32> mat) {
