@@ -430,8 +430,9 @@ def primitives_workload(binding, torch, n: int = 30, reps: int = 3):
     row(f"a10 TFIM <H>, {len(terms)} strings in one call (bytes of one read)", "statevector.cpp:258-291", N * s,
         wall(lambda: sv.expectation_value_pauli_sum(terms)), "wall")
     shots = 10000
-    row(f"a9 sample {shots} shots", "statevector.cpp:235-256", N * s + 8 * N + 16 * shots,
-        wall(lambda: sv.sample(shots, 42)), "wall")
+    # bytes: one read of the state, then one 4096-amplitude chunk and 16 B per shot
+    row(f"a9 sample {shots} shots (one read + a chunk per shot)", "statevector.cpp:235-256",
+        N * s + shots * (4096 * s + 16), wall(lambda: sv.sample(shots, 42)), "wall")
     host = np.empty(N, dtype=np.float64)
     host.fill(0.0)
     row("a8 get_probabilities (kernel + device-to-host copy)", "api.cpp:85-90, statevector.cpp:226-233",

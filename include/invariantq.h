@@ -128,9 +128,12 @@ double iq_statevector_norm_squared(void* sv);     /* StateVector::norm_squared, 
 void   iq_statevector_normalize(void* sv);        /* StateVector::normalize, statevector.cpp:215-224 */
 void   iq_statevector_apply_diagonal(void* sv, const size_t* qubits, size_t k, const double* d_real,
                                      const double* d_imag);
-/* Sampling with caller-supplied uniforms in [0,1) (e.g. drawn from a seed tree). */
+/* Sampling with caller-supplied host uniforms in [0,1) (e.g. drawn from a seed
+ * tree): each index is the first whose iq_statevector_get_cdf value reaches
+ * the uniform, clamped to dim - 1, as for iq_statevector_sample. */
 void   iq_statevector_sample_uniforms(void* sv, size_t num_shots, const double* uniforms, uint64_t* results_out);
-/* Inclusive FP64 cumulative distribution used by sampling (dim doubles). */
+/* Inclusive FP64 cumulative distribution used by sampling (dim doubles,
+ * non-decreasing; within rounding of the serial running sum). */
 void   iq_statevector_get_cdf(void* sv, double* cdf_out);
 /* sum_t coeffs[t] <P_t>; term t uses lens[t] consecutive characters of ops and
  * entries of qubits.  per_term (optional) receives Re/Im <P_t> pairs. */

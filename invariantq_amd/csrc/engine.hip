@@ -677,6 +677,42 @@ void iq_statevector_normalize(void* sv) {
     hip_ok(launch_scale(e->amp, e->dim, e->prec, 1.0 / std::sqrt(v), e->stream), "scale");
 }
 
+// Sampling buffers in the work area: the CDF (when it is written), the scan
+// scratch, the uniforms and the results.
+namespace {
+struct SampleBufs {
+    bool direct;  // per-chunk searches, no CDF written
+    double* cdf;
+    double* scratch;
+    double* du;
+    uint64_t* dout;
+};
+
+bool sample_bufs(Engine* e, size_t shots, SampleBufs& b) {
+    b.direct = sample_without_cdf(e->dim, shots);
+    const size_t ncdf = b.direct ? 0 : e->dim, nscr = scan_scratch_doubles(e->dim), ns = std::max<size_t>(shots, 1);
+    if (!e->ensure_work(ncdf + nscr + 2 * ns)) return false;
+    b.cdf = e->work;
+    b.scratch = e->work + ncdf;
+    b.du = b.scratch + nscr;
+    b.dout = reinterpret_cast<uint64_t*>(b.du + ns);
+    return true;
+}
+
+// the passes over the state that do not need the uniforms
+bool sample_prepare(Engine* e, const SampleBufs& b) {
+    return b.direct ? hip_ok(launch_cdf_chunks(e->amp, e->dim, e->prec, b.scratch, e->stream), "cdf")
+                    : hip_ok(launch_cdf(e->amp, e->dim, e->prec, b.cdf, b.scratch, e->stream), "cdf");
+}
+
+// uniforms in b.du -> indices in b.dout
+bool sample_search(Engine* e, const SampleBufs& b, size_t shots) {
+    return b.direct
+               ? hip_ok(launch_sample_chunks(e->amp, e->dim, e->prec, b.scratch, b.du, shots, b.dout, e->stream), "search")
+               : hip_ok(launch_search(b.cdf, e->dim, b.du, shots, b.dout, e->stream), "search");
+}
+}  // namespace
+
 // reference core/src/statevector.cpp:235-256: probabilities, inclusive CDF,
 // uniform_real_distribution<double>(0,1) over mt19937_64(seed), lower_bound,
 // clamp to dim-1.  The uniforms are drawn here on the host with the same
@@ -685,24 +721,20 @@ void iq_statevector_sample(void* sv, size_t num_shots, uint64_t seed, uint64_t* 
     if (!sv || !results_out) return;
     Engine* e = sv_of(sv);
     DeviceGuard g(e->dev);
-    const size_t nchunks = scan_scratch_doubles(e->dim);
-    if (!e->ensure_work(e->dim + nchunks + 2 * std::max<size_t>(num_shots, 1))) return;
-    double* cdf = e->work;
-    double* scratch = cdf + e->dim;
-    double* du = scratch + nchunks;
-    uint64_t* dout = reinterpret_cast<uint64_t*>(du + std::max<size_t>(num_shots, 1));
+    SampleBufs b;
+    if (!sample_bufs(e, num_shots, b)) return;
     std::vector<double> u(num_shots);
     std::mt19937_64 gen(seed);
     std::uniform_real_distribution<double> uni(0.0, 1.0);
     for (size_t s = 0; s < num_shots; ++s) u[s] = uni(gen);
-    if (!hip_ok(launch_cdf(e->amp, e->dim, e->prec, cdf, scratch, e->stream), "cdf")) return;
+    if (!sample_prepare(e, b)) return;
     if (num_shots == 0) {
         e->sync();
         return;
     }
-    if (!hip_ok(hipMemcpyAsync(du, u.data(), num_shots * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
-        !hip_ok(launch_search(cdf, e->dim, du, num_shots, dout, e->stream), "search") ||
-        !hip_ok(hipMemcpyAsync(results_out, dout, num_shots * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
+    if (!hip_ok(hipMemcpyAsync(b.du, u.data(), num_shots * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
+        !sample_search(e, b, num_shots) ||
+        !hip_ok(hipMemcpyAsync(results_out, b.dout, num_shots * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
         return;
     e->sync();
 }
@@ -712,8 +744,9 @@ void iq_statevector_sample(void* sv, size_t num_shots, uint64_t seed, uint64_t* 
 // serves by re-simulating the circuit, invariantq/distributed/worker.py:52-95):
 // task t draws shots[t] indices with std::mt19937_64(seeds[t]) exactly as
 // iq_statevector_sample(sv, shots[t], seeds[t], ...) does; results are
-// concatenated in task order.  The CDF is built once and one search launch
-// serves every shot; the tasks' uniforms are drawn on host threads.
+// concatenated in task order.  The passes over the state run once and one
+// search launch serves every shot; the tasks' uniforms are drawn on host
+// threads meanwhile.
 void iq_statevector_sample_tasks(void* sv, size_t ntasks, const uint64_t* seeds, const uint64_t* shots,
                                  uint64_t* results_out) {
     if (!sv || !results_out || !seeds || !shots || ntasks == 0) return;
@@ -728,13 +761,8 @@ void iq_statevector_sample_tasks(void* sv, size_t ntasks, const uint64_t* seeds,
         off[t + 1] = off[t] + shots[t];
     }
     const size_t total = off[ntasks];
-    const size_t nchunks = scan_scratch_doubles(e->dim);
-    if (!e->ensure_work(e->dim + nchunks + 2 * std::max<size_t>(total, 1))) return;
-    double* cdf = e->work;
-    double* scratch = cdf + e->dim;
-    double* du = scratch + nchunks;
-    uint64_t* dout = reinterpret_cast<uint64_t*>(du + std::max<size_t>(total, 1));
-    if (!hip_ok(launch_cdf(e->amp, e->dim, e->prec, cdf, scratch, e->stream), "cdf")) return;
+    SampleBufs b;
+    if (!sample_bufs(e, total, b) || !sample_prepare(e, b)) return;
     std::vector<double> u(total);
     auto draw = [&](size_t t0, size_t t1) {
         for (size_t t = t0; t < t1; ++t) {
@@ -755,9 +783,9 @@ void iq_statevector_sample_tasks(void* sv, size_t ntasks, const uint64_t* seeds,
         e->sync();
         return;
     }
-    if (!hip_ok(hipMemcpyAsync(du, u.data(), total * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
-        !hip_ok(launch_search(cdf, e->dim, du, total, dout, e->stream), "search") ||
-        !hip_ok(hipMemcpyAsync(results_out, dout, total * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
+    if (!hip_ok(hipMemcpyAsync(b.du, u.data(), total * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
+        !sample_search(e, b, total) ||
+        !hip_ok(hipMemcpyAsync(results_out, b.dout, total * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
         return;
     e->sync();
 }
@@ -768,29 +796,25 @@ void iq_statevector_sample_uniforms(void* sv, size_t num_shots, const double* un
     if (!sv || !results_out || (!uniforms && num_shots)) return;
     Engine* e = sv_of(sv);
     DeviceGuard g(e->dev);
-    const size_t nchunks = scan_scratch_doubles(e->dim);
-    if (!e->ensure_work(e->dim + nchunks + 2 * std::max<size_t>(num_shots, 1))) return;
-    double* cdf = e->work;
-    double* scratch = cdf + e->dim;
-    double* du = scratch + nchunks;
-    uint64_t* dout = reinterpret_cast<uint64_t*>(du + std::max<size_t>(num_shots, 1));
-    if (!hip_ok(launch_cdf(e->amp, e->dim, e->prec, cdf, scratch, e->stream), "cdf")) return;
+    SampleBufs b;
+    if (!sample_bufs(e, num_shots, b) || !sample_prepare(e, b)) return;
     if (num_shots) {
-        if (!hip_ok(hipMemcpyAsync(du, uniforms, num_shots * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
-            !hip_ok(launch_search(cdf, e->dim, du, num_shots, dout, e->stream), "search") ||
-            !hip_ok(hipMemcpyAsync(results_out, dout, num_shots * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
+        if (!hip_ok(hipMemcpyAsync(b.du, uniforms, num_shots * 8, hipMemcpyHostToDevice, e->stream), "H2D") ||
+            !sample_search(e, b, num_shots) ||
+            !hip_ok(hipMemcpyAsync(results_out, b.dout, num_shots * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
             return;
     }
     e->sync();
 }
 
-// Inclusive FP64 CDF of the probabilities, exported (for boundary-margin checks).
+// Inclusive FP64 CDF of the probabilities (the running maximum of the chunked
+// prefix sums, so monotone), exported for boundary-margin checks; sampling
+// returns the first index whose value here reaches the uniform.
 void iq_statevector_get_cdf(void* sv, double* cdf_out) {
     if (!sv || !cdf_out) return;
     Engine* e = sv_of(sv);
     DeviceGuard g(e->dev);
-    const size_t nchunks = scan_scratch_doubles(e->dim);
-    if (!e->ensure_work(e->dim + nchunks + 2)) return;
+    if (!e->ensure_work(e->dim + scan_scratch_doubles(e->dim) + 2)) return;
     if (!hip_ok(launch_cdf(e->amp, e->dim, e->prec, e->work, e->work + e->dim, e->stream), "cdf") ||
         !hip_ok(hipMemcpyAsync(cdf_out, e->work, e->dim * 8, hipMemcpyDeviceToHost, e->stream), "D2H"))
         return;

@@ -69,10 +69,18 @@ hipError_t reduce_pauli_group(const void* psi, int n, int prec, uint64_t xmask, 
 hipError_t reduce_dm_diag(const void* rho, int nq, int mode, int target, double* part, double* out, hipStream_t st);
 hipError_t launch_dm_collapse(void* rho, int nq, int target, int outcome, double factor, hipStream_t st);
 
-// Sampling: inclusive FP64 prefix sum of probs into cdf, then lower_bound per uniform.
+// Sampling: the first index whose FP64 CDF value of |psi_i|^2 reaches each
+// uniform, clamped to count - 1.  scratch holds scan_scratch_doubles(count)
+// doubles (chunk offsets and running maxima).
 size_t scan_scratch_doubles(uint64_t count);
-// Inclusive FP64 CDF of |psi_i|^2 into cdf (count doubles); scratch holds
-// scan_scratch_doubles(count) chunk sums.
+// true when per-chunk searches (launch_cdf_chunks + launch_sample_chunks)
+// cost less than writing the CDF (launch_cdf + launch_search): up to twice
+// as many shots as 4096-amplitude chunks.  Both give the same indices.
+bool sample_without_cdf(uint64_t count, uint64_t shots);
+hipError_t launch_cdf_chunks(const void* psi, uint64_t count, int prec, double* scratch, hipStream_t st);
+hipError_t launch_sample_chunks(const void* psi, uint64_t count, int prec, const double* scratch, const double* u,
+                                uint64_t shots, uint64_t* out, hipStream_t st);
+// Inclusive FP64 CDF (monotone) into cdf (count doubles).
 hipError_t launch_cdf(const void* psi, uint64_t count, int prec, double* cdf, double* scratch, hipStream_t st);
 hipError_t launch_search(const double* cdf, uint64_t count, const double* u, uint64_t shots, uint64_t* out, hipStream_t st);
 
