@@ -401,10 +401,12 @@ def primitives_workload(binding, torch, n: int = 30, reps: int = 3):
 
     rows = []
 
-    def row(name, ref, nbytes, t, how):
+    def row(name, ref, nbytes, t, how, host_bytes=0):
         rows.append({"row": name, "reference": ref, "algorithmic_GB": round(nbytes / 1e9, 3),
                      "ms": round(t * 1e3, 3), "TBps": round(nbytes / t / 1e12, 3),
                      "pct_hbm_peak": round(100 * nbytes / t / 8e12, 1), "timed": how})
+        if host_bytes:  # bytes delivered into the caller's (pageable) host buffers
+            rows[-1]["host_GBps"] = round(host_bytes / t / 1e9, 1)
 
     g1 = gates.u3(0.3, 1.1, 2.0)
     for t in (0, 4, 10, 29):
@@ -436,11 +438,11 @@ def primitives_workload(binding, torch, n: int = 30, reps: int = 3):
     host = np.empty(N, dtype=np.float64)
     host.fill(0.0)
     row("a8 get_probabilities (kernel + device-to-host copy)", "api.cpp:85-90, statevector.cpp:226-233",
-        N * s + 8 * N, wall(lambda: lib.iq_statevector_get_probabilities(sv._ptr, dp(host)), 1), "wall")
+        N * s + 8 * N, wall(lambda: lib.iq_statevector_get_probabilities(sv._ptr, dp(host)), 1), "wall", 8 * N)
     host2 = np.empty(N, dtype=np.float64)
     host2.fill(0.0)
     row("a11 get_amplitudes (kernel + device-to-host copy)", "api.cpp:74-83", N * s + 16 * N,
-        wall(lambda: lib.iq_statevector_get_amplitudes(sv._ptr, dp(host), dp(host2)), 1), "wall")
+        wall(lambda: lib.iq_statevector_get_amplitudes(sv._ptr, dp(host), dp(host2)), 1), "wall", 16 * N)
     del host, host2
     row("a6 measure target 29 (probability + collapse)", "statevector.cpp:171-190", 2.5 * N * s,
         wall(lambda: sv.measure(29, 0.3)), "wall")

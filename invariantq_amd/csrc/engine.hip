@@ -119,6 +119,9 @@ Engine::~Engine() {
     if (host_out) hipHostFree(host_out);
     if (work) hipFree(work);
     if (stage) hipFree(stage);
+    if (pinned) hipHostFree(pinned);
+    for (hipEvent_t ev : pinned_ev)
+        if (ev) hipEventDestroy(ev);
     if (stream) hipStreamDestroy(stream);
 }
 
@@ -311,41 +314,107 @@ bool Engine::ensure_stage() {
     return true;
 }
 
-bool Engine::export_split(double* re, double* im) {
-    if (!ensure_stage()) return false;
-    for (uint64_t first = 0; first < dim; first += kStage) {
-        const uint64_t cnt = std::min<uint64_t>(kStage, dim - first);
-        if (!hip_ok(launch_split(amp, first, cnt, prec, stage, stage + kStage, stream), "split")) return false;
-        if (!hip_ok(hipMemcpyAsync(re + first, stage, cnt * 8, hipMemcpyDeviceToHost, stream), "D2H")) return false;
-        if (!hip_ok(hipMemcpyAsync(im + first, stage + kStage, cnt * 8, hipMemcpyDeviceToHost, stream), "D2H")) return false;
-        if (!sync()) return false;
+bool Engine::ensure_pinned(int arrays) {
+    const size_t need = 2 * (size_t)arrays * std::min<uint64_t>(kStage, dim);
+    if (!pinned_ev[0])
+        for (hipEvent_t& ev : pinned_ev)
+            if (!hip_ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event")) return false;
+    if (pinned_doubles >= need) return true;
+    if (pinned) hipHostFree(pinned);
+    pinned = nullptr;
+    pinned_doubles = 0;
+    if (hipHostMalloc((void**)&pinned, need * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        set_error(kErrOutOfMemory, "cannot allocate %zu pinned staging doubles", need);
+        pinned = nullptr;
+        return false;
     }
+    pinned_doubles = need;
     return true;
 }
 
-bool Engine::import_split(const double* re, const double* im) {
-    if (!ensure_stage() || !sync()) return false;
-    for (uint64_t first = 0; first < dim; first += kStage) {
-        const uint64_t cnt = std::min<uint64_t>(kStage, dim - first);
-        if (!hip_ok(hipMemcpyAsync(stage, re + first, cnt * 8, hipMemcpyHostToDevice, stream), "H2D")) return false;
-        if (!hip_ok(hipMemcpyAsync(stage + kStage, im + first, cnt * 8, hipMemcpyHostToDevice, stream), "H2D")) return false;
-        if (!hip_ok(launch_join(amp, first, cnt, prec, stage, stage + kStage, stream), "join")) return false;
-        if (!sync()) return false;
+namespace {
+// memcpy split over host threads (page-aligned parts) for large copies
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kPart = 2u << 20;
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>({16, hw, (bytes + kPart - 1) / kPart});
+    if (nt <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
     }
-    return true;
+    const size_t part = ((bytes / nt) + 4095) & ~size_t(4095);
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt && t * part < bytes; ++t)
+        pool.emplace_back([=] {
+            std::memcpy(static_cast<char*>(dst) + t * part, static_cast<const char*>(src) + t * part,
+                        std::min(part, bytes - t * part));
+        });
+    std::memcpy(dst, src, std::min(part, bytes));
+    for (auto& th : pool) th.join();
+}
+}  // namespace
+
+// Chunk c of kStage amplitudes: `produce` writes `arrays` device arrays at
+// stage + a * kStage, a DMA moves them into pinned slot c % 2, and the host
+// copies them out to dst[a] while the next chunk is produced and moved.
+bool Engine::export_chunks(int arrays, double* const* dst, bool (*produce)(Engine*, uint64_t, uint64_t)) {
+    if (!ensure_stage() || !ensure_pinned(arrays)) return false;
+    const uint64_t per = std::min<uint64_t>(kStage, dim), nch = (dim + kStage - 1) / kStage;
+    auto slot = [&](uint64_t c, int a) { return pinned + ((c & 1) * arrays + a) * per; };
+    auto enqueue = [&](uint64_t c) {
+        const uint64_t first = c * kStage, cnt = std::min<uint64_t>(kStage, dim - first);
+        if (!produce(this, first, cnt)) return false;
+        for (int a = 0; a < arrays; ++a)
+            if (!hip_ok(hipMemcpyAsync(slot(c, a), stage + a * kStage, cnt * 8, hipMemcpyDeviceToHost, stream), "D2H"))
+                return false;
+        return hip_ok(hipEventRecord(pinned_ev[c & 1], stream), "event");
+    };
+    if (!enqueue(0)) return false;
+    for (uint64_t c = 0; c < nch; ++c) {
+        // slot (c + 1) % 2 was last read by chunk c - 1's host copy, which has returned
+        if (c + 1 < nch && !enqueue(c + 1)) return false;
+        if (!hip_ok(hipEventSynchronize(pinned_ev[c & 1]), "event")) return false;
+        const uint64_t first = c * kStage, cnt = std::min<uint64_t>(kStage, dim - first);
+        for (int a = 0; a < arrays; ++a) parallel_copy(dst[a] + first, slot(c, a), cnt * 8);
+    }
+    return sync();
+}
+
+bool Engine::export_split(double* re, double* im) {
+    double* const dst[2] = {re, im};
+    return export_chunks(2, dst, [](Engine* e, uint64_t first, uint64_t cnt) {
+        return hip_ok(launch_split(e->amp, first, cnt, e->prec, e->stage, e->stage + kStage, e->stream), "split");
+    });
 }
 
 bool Engine::export_probs(double* out) {
-    if (!ensure_stage()) return false;
-    const size_t esz = elem;
-    for (uint64_t first = 0; first < dim; first += kStage) {
-        const uint64_t cnt = std::min<uint64_t>(kStage, dim - first);
-        const void* src = static_cast<const char*>(amp) + first * esz;
-        if (!hip_ok(launch_probs(src, cnt, prec, stage, stream), "probs")) return false;
-        if (!hip_ok(hipMemcpyAsync(out + first, stage, cnt * 8, hipMemcpyDeviceToHost, stream), "D2H")) return false;
-        if (!sync()) return false;
+    double* const dst[1] = {out};
+    return export_chunks(1, dst, [](Engine* e, uint64_t first, uint64_t cnt) {
+        const void* src = static_cast<const char*>(e->amp) + first * e->elem;
+        return hip_ok(launch_probs(src, cnt, e->prec, e->stage, e->stream), "probs");
+    });
+}
+
+// Chunk c: the host copies re/im into pinned slot c % 2 (after that slot's
+// previous DMA has finished), a DMA moves it to the device stage, and the join
+// kernel writes the amplitudes, overlapping the next chunk's host copy.
+bool Engine::import_split(const double* re, const double* im) {
+    if (!ensure_stage() || !ensure_pinned(2) || !sync()) return false;
+    const uint64_t per = std::min<uint64_t>(kStage, dim), nch = (dim + kStage - 1) / kStage;
+    for (uint64_t c = 0; c < nch; ++c) {
+        const uint64_t first = c * kStage, cnt = std::min<uint64_t>(kStage, dim - first);
+        double* pr = pinned + (c & 1) * 2 * per;
+        double* pi = pr + per;
+        if (c >= 2 && !hip_ok(hipEventSynchronize(pinned_ev[c & 1]), "event")) return false;
+        parallel_copy(pr, re + first, cnt * 8);
+        parallel_copy(pi, im + first, cnt * 8);
+        if (!hip_ok(hipMemcpyAsync(stage, pr, cnt * 8, hipMemcpyHostToDevice, stream), "H2D") ||
+            !hip_ok(hipMemcpyAsync(stage + kStage, pi, cnt * 8, hipMemcpyHostToDevice, stream), "H2D") ||
+            !hip_ok(launch_join(amp, first, cnt, prec, stage, stage + kStage, stream), "join") ||
+            !hip_ok(hipEventRecord(pinned_ev[c & 1], stream), "event"))
+            return false;
     }
-    return true;
+    return sync();
 }
 
 bool Engine::fetch(int m, double* out) {

@@ -63,6 +63,12 @@ struct Engine {
     double* work = nullptr;
     size_t work_doubles = 0;
     double* stage = nullptr;
+    // host copies of exports and imports: two pinned slots of `arrays` x
+    // kStage doubles, so that one chunk's DMA overlaps the previous chunk's
+    // host copy (pageable copies through the runtime run at about 16 GB/s)
+    double* pinned = nullptr;
+    size_t pinned_doubles = 0;
+    hipEvent_t pinned_ev[2] = {nullptr, nullptr};
 
     // matrices of one apply_sequence call: pinned (coherent) + device copy
     struct SeqBuf {
@@ -101,6 +107,8 @@ struct Engine {
     bool sync();
     bool ensure_work(size_t doubles);
     bool ensure_stage();
+    bool ensure_pinned(int arrays);
+    bool export_chunks(int arrays, double* const* dst, bool (*produce)(Engine*, uint64_t, uint64_t));
     bool export_split(double* re, double* im);
     bool import_split(const double* re, const double* im);
     bool export_probs(double* out);
