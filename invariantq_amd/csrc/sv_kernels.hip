@@ -511,9 +511,12 @@ __global__ void __launch_bounds__(kTotalsThreads) k_scan_small(double* __restric
     }
 }
 
+// single: one group, whose offset and exclusive running maximum are 0 (the
+// scans of k_totals_sums and k_scan_small would give exactly that); bmax[0]
+// then receives that 0 directly.
 __global__ void __launch_bounds__(kScanThreads) k_totals_apply(double* __restrict__ totals, double* __restrict__ tops,
                                                                uint64_t nchunks, const double* __restrict__ boff,
-                                                               double* __restrict__ bmax) {
+                                                               double* __restrict__ bmax, int single) {
     const uint64_t first = (uint64_t)blockIdx.x * kGroupChunks + (uint64_t)threadIdx.x * kScanItems;
     double sums[kScanItems];
     double run = 0.0;
@@ -523,7 +526,7 @@ __global__ void __launch_bounds__(kScanThreads) k_totals_apply(double* __restric
         run += sums[k];
     }
     const double off = block_exclusive_scan(run, nullptr);
-    const double group_off = boff[blockIdx.x];
+    const double group_off = single ? 0.0 : boff[blockIdx.x];
     double part = 0.0, mx = 0.0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
@@ -542,7 +545,7 @@ __global__ void __launch_bounds__(kScanThreads) k_totals_apply(double* __restric
             totals[first + k] = sums[k];
             tops[first + k] = r;
         }
-    if (threadIdx.x == 0) bmax[blockIdx.x] = all;
+    if (threadIdx.x == 0) bmax[blockIdx.x] = single ? 0.0 : all;
 }
 
 // Pass 3 (the written CDF): each chunk's prefixes plus its offset, raised
@@ -897,10 +900,15 @@ template <typename R>
 static hipError_t cdf_chunks(const void* psi, uint64_t count, const ScanScratch& z, hipStream_t st) {
     hipLaunchKernelGGL(k_cdf_sums<R>, dim3((unsigned)z.nchunks), dim3(kScanThreads), 0, st, (const cv<R>*)psi, count,
                        z.offsets, z.tops);
+    if (z.ngroups == 1) {  // up to 4096 chunks (2^24 amplitudes): one launch
+        hipLaunchKernelGGL(k_totals_apply, dim3(1), dim3(kScanThreads), 0, st, z.offsets, z.tops, z.nchunks, z.boff,
+                           z.bpre, 1);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_totals_sums, dim3((unsigned)z.ngroups), dim3(kScanThreads), 0, st, z.offsets, z.nchunks, z.boff);
     hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kTotalsThreads), 0, st, z.boff, z.ngroups, 0);
     hipLaunchKernelGGL(k_totals_apply, dim3((unsigned)z.ngroups), dim3(kScanThreads), 0, st, z.offsets, z.tops, z.nchunks,
-                       z.boff, z.bpre);
+                       z.boff, z.bpre, 0);
     hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kTotalsThreads), 0, st, z.bpre, z.ngroups, 1);
     return hipGetLastError();
 }
