@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <mutex>
 #include <new>
 #include <random>
@@ -343,13 +344,18 @@ void parallel_copy(void* dst, const void* src, size_t bytes) {
         return;
     }
     const size_t part = ((bytes / nt) + 4095) & ~size_t(4095);
+    auto copy_part = [=](size_t t) {
+        std::memcpy(static_cast<char*>(dst) + t * part, static_cast<const char*>(src) + t * part,
+                    std::min(part, bytes - t * part));
+    };
     std::vector<std::thread> pool;
-    for (size_t t = 1; t < nt && t * part < bytes; ++t)
-        pool.emplace_back([=] {
-            std::memcpy(static_cast<char*>(dst) + t * part, static_cast<const char*>(src) + t * part,
-                        std::min(part, bytes - t * part));
-        });
-    std::memcpy(dst, src, std::min(part, bytes));
+    size_t t = 1;
+    try {
+        for (; t < nt && t * part < bytes; ++t) pool.emplace_back(copy_part, t);
+    } catch (const std::exception&) {  // no thread to be had: copy the rest here
+        for (; t < nt && t * part < bytes; ++t) copy_part(t);
+    }
+    copy_part(0);
     for (auto& th : pool) th.join();
 }
 }  // namespace
