@@ -415,6 +415,9 @@ def primitives_workload(binding, torch, n: int = 30, reps: int = 3):
         row(f"a3 apply_2q dense ({q0},{q1})", "statevector.cpp:49-87", 2 * N * s,
             dev(lambda: sv.apply_2q(q0, q1, u2)), "events")
     row("a3 apply_2q CX (3,20)", "statevector.cpp:49-87", 2 * N * s, dev(lambda: sv.apply_2q(3, 20, gates.CX)), "events")
+    # control at bit 20: only the half with the control set is read and written
+    row("a3 apply_2q CX (20,3), controlled: half the state", "statevector.cpp:49-87", N * s,
+        dev(lambda: sv.apply_2q(20, 3, gates.CX)), "events")
     row("a3 apply_2q CZ (7,25), diagonal", "statevector.cpp:49-87", 2 * N * s,
         dev(lambda: sv.apply_2q(7, 25, gates.CZ)), "events")
     row("a4 apply_fused k=3 (4,12,27)", "statevector.cpp:89-152", 2 * N * s,

@@ -158,6 +158,31 @@ static bool is_diagonal(int K, const double* re, const double* im) {
     return true;
 }
 
+// A two-qubit matrix (MSB-first local index: l = 2 b(q[0]) + b(q[1])) of the
+// form blockdiag(I, U) over one of its qubits: returns that control's list
+// position (0 or 1) and U, or -1.
+static int controlled_form(const double* re, const double* im, double* ure, double* uim) {
+    for (int cpos = 0; cpos < 2; ++cpos) {
+        const int cb = cpos == 0 ? 2 : 1, tb = 3 - cb;  // local bit values of control and target
+        auto is = [&](int r, int c, double v) { return re[r * 4 + c] == v && im[r * 4 + c] == 0.0; };
+        bool ok = true;
+        for (int r = 0; r < 4 && ok; ++r)
+            for (int c = 0; c < 4 && ok; ++c) {
+                const bool rc = r & cb, cc = c & cb;
+                if (!rc || !cc) ok = is(r, c, (!rc && !cc && r == c) ? 1.0 : 0.0);
+            }
+        if (!ok) continue;
+        for (int r = 0; r < 2; ++r)
+            for (int c = 0; c < 2; ++c) {
+                const int R = cb | (r ? tb : 0), C = cb | (c ? tb : 0);
+                ure[r * 2 + c] = re[R * 4 + C];
+                uim[r * 2 + c] = im[R * 4 + C];
+            }
+        return cpos;
+    }
+    return -1;
+}
+
 bool Engine::check_gate(int K, const uint32_t* q) const {
     if (K < 1 || K > kMaxK || K > n) {
         set_error(kErrInvalidArgument, "gate on %d qubits is not supported (1..5, <= n)", K);
@@ -191,6 +216,18 @@ Engine::GatePlan Engine::plan_gate(int K, const uint32_t* q, const double* re, c
         p.kind = GatePlan::kDiag;
         return p;
     }
+    // controlled one-qubit gates: half the state, once the control bit
+    // separates whole 256-byte spans (below that the skipped half shares
+    // cache lines with the rest)
+    if (path == kPathAuto && K == 2) {
+        const int cpos = controlled_form(re, im, p.ure, p.uim);
+        if (cpos >= 0 && q[cpos] >= 4) {
+            p.kind = GatePlan::kControlled;
+            p.ctrl = q[cpos];
+            p.targ = q[1 - cpos];
+            return p;
+        }
+    }
     int chosen = choose_path(n, prec, p.g);
     if (chosen == kPathMfma && (K != 5 || n < 9)) chosen = kPathDirect;
     p.g.path = chosen;
@@ -223,6 +260,8 @@ bool Engine::launch_plan(GatePlan& p, const void* dmat) {
         }
         return hip_ok(launch_diag(amp, n, prec, stream, p.g.K, p.g.q, dre, dimv), "diagonal gate launch");
     }
+    if (p.kind == GatePlan::kControlled)
+        return hip_ok(launch_controlled(amp, n, prec, stream, p.ctrl, p.targ, p.ure, p.uim), "controlled gate launch");
     p.g.dmat = dmat;
     return hip_ok(launch_gate(amp, n, prec, stream, p.g), "gate launch");
 }

@@ -81,12 +81,14 @@ struct Engine {
     } seq;
 
     struct GatePlan {
-        enum Kind { kDiag, kValue, kDevice };
+        enum Kind { kDiag, kControlled, kValue, kDevice };
         Kind kind = kValue;
         GateDesc g;
         size_t bytes = 0;
         bool mfma = false;
         bool m3 = false;
+        uint32_t ctrl = 0, targ = 0;  // kControlled: U (row-major 2x2) on targ where ctrl is set
+        double ure[4] = {}, uim[4] = {};
     };
 
     Engine() = default;

@@ -397,6 +397,34 @@ __global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, Mfma
 // Diagonal operator: each amplitude times the entry its target bits select.
 // Four amplitudes in flight per thread; NT streams them (large states: each
 // amplitude is touched once).
+// Controlled one-qubit gate, a two-qubit matrix blockdiag(I, U) over the
+// control bit: the identity block leaves its amplitudes unchanged, so only
+// the half of the state with the control bit set is read and written (pairs
+// base, base + toff with the control bit set and the target bit clear).
+template <typename R, bool NT>
+__global__ void __launch_bounds__(256) k_gate_controlled(cv<R>* __restrict__ psi, uint64_t npairs, DirectArgs<2> a,
+                                                         uint64_t cbit, uint64_t toff, ValMat<R, 8> mat) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < npairs; g += stride) {
+        const uint64_t base = insert_zero_bits<2>(g, a.pos) | cbit;
+        R xr[2], xi[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const cv<R> v = ld_stream<R>(psi + base + (s ? toff : 0), NT);
+            xr[s] = v.x;
+            xi[s] = v.y;
+        }
+        matvec<R, 2, false>(
+            xr, xi, [&](int i) { return mat.v[i]; },
+            [&](int r, R yr, R yi) {
+                cv<R> o;
+                o.x = yr;
+                o.y = yi;
+                st_stream<R>(psi + base + (r ? toff : 0), o, NT);
+            });
+    }
+}
+
 template <typename R, int K, bool NT = false>
 __global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t count, DirectArgs<K> a,
                                               ValMat<double, 2 * (1 << K)> d) {
@@ -770,6 +798,38 @@ hipError_t launch_gate(void* psi, int n, int prec, hipStream_t st, const GateDes
     if (g.K >= 4 && !g.dmat) return hipErrorInvalidValue;
     if (path == kPathTiled && !g.dmat) path = kPathDirect;
     return prec == kC128 ? dispatch<double>(psi, n, st, g, path) : dispatch<float>(psi, n, st, g, path);
+}
+
+namespace {
+template <typename R>
+hipError_t run_controlled(void* psi, int n, hipStream_t st, uint32_t c, uint32_t t, const double* ure,
+                          const double* uim) {
+    DirectArgs<2> a;
+    a.pos[0] = std::min(c, t);
+    a.pos[1] = std::max(c, t);
+    ValMat<R, 8> m;  // row-major 2x2: real parts, then imaginary parts, per row
+    for (int r = 0; r < 2; ++r)
+        for (int k = 0; k < 2; ++k) {
+            m.v[r * 4 + k] = (R)ure[r * 2 + k];
+            m.v[r * 4 + 2 + k] = (R)uim[r * 2 + k];
+        }
+    const uint64_t count = 1ULL << n, npairs = count >> 2;
+    const unsigned blocks = grid_for(npairs, 256);
+    if (count >= kStreamMinCount && g_direct_nt != 0)
+        hipLaunchKernelGGL((k_gate_controlled<R, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, npairs, a,
+                           1ULL << c, 1ULL << t, m);
+    else
+        hipLaunchKernelGGL((k_gate_controlled<R, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, npairs, a,
+                           1ULL << c, 1ULL << t, m);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_controlled(void* psi, int n, int prec, hipStream_t st, uint32_t c, uint32_t t, const double* ure,
+                             const double* uim) {
+    if (n < 2 || c == t || (int)c >= n || (int)t >= n) return hipErrorInvalidValue;
+    return prec == kC128 ? run_controlled<double>(psi, n, st, c, t, ure, uim)
+                         : run_controlled<float>(psi, n, st, c, t, ure, uim);
 }
 
 hipError_t launch_diag(void* psi, int n, int prec, hipStream_t st, int K, const uint32_t* q, const double* dre,
