@@ -104,9 +104,10 @@ const char* iq_last_error_message(void);
 void        iq_clear_error(void);
 const char* iq_backend_name(void);
 /* Gate kernel selection: 0 automatic, 1 direct (group per lane), 2 LDS-tiled, 3 matrix-core (5-qubit blocks).
- * Automatic also sends diagonal matrices to a diagonal pass and two-qubit
- * blockdiag(I, U) matrices with the control at bit 4 or above to a pass over
- * the control-set half; the forced paths apply every matrix densely. */
+ * Automatic also sends diagonal matrices to a diagonal pass and two- or
+ * three-qubit blockdiag(I, U) matrices (U on one qubit, every control at bit 4
+ * or above) to a pass over the controls-set part of the state; the forced
+ * paths apply every matrix densely. */
 void        iq_set_gate_path(int path);
 /* Matrix-core kernel options. Defaults are per target set; 0 / -1 restores them.
  * - batches_per_iteration: groups-of-16 batches per wave iteration (1, 2, 4). Default:

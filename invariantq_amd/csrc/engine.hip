@@ -158,27 +158,26 @@ static bool is_diagonal(int K, const double* re, const double* im) {
     return true;
 }
 
-// A two-qubit matrix (MSB-first local index: l = 2 b(q[0]) + b(q[1])) of the
-// form blockdiag(I, U) over one of its qubits: returns that control's list
-// position (0 or 1) and U, or -1.
-static int controlled_form(const double* re, const double* im, double* ure, double* uim) {
-    for (int cpos = 0; cpos < 2; ++cpos) {
-        const int cb = cpos == 0 ? 2 : 1, tb = 3 - cb;  // local bit values of control and target
-        auto is = [&](int r, int c, double v) { return re[r * 4 + c] == v && im[r * 4 + c] == 0.0; };
+// A K-qubit matrix (MSB-first local index: bit K-1-m of l is b(q[m])) of the
+// form blockdiag(I, U), U acting on one qubit where all the others (the
+// controls) are 1: returns that target's list position and U, or -1.
+static int controlled_form(int K, const double* re, const double* im, double* ure, double* uim) {
+    const int d = 1 << K;
+    for (int tpos = 0; tpos < K; ++tpos) {
+        const int tb = 1 << (K - 1 - tpos), cb = (d - 1) & ~tb;  // local bits of target and controls
         bool ok = true;
-        for (int r = 0; r < 4 && ok; ++r)
-            for (int c = 0; c < 4 && ok; ++c) {
-                const bool rc = r & cb, cc = c & cb;
-                if (!rc || !cc) ok = is(r, c, (!rc && !cc && r == c) ? 1.0 : 0.0);
-            }
+        for (int r = 0; r < d && ok; ++r)
+            for (int c = 0; c < d && ok; ++c)
+                if ((r & cb) != cb || (c & cb) != cb)
+                    ok = re[r * d + c] == (r == c ? 1.0 : 0.0) && im[r * d + c] == 0.0;
         if (!ok) continue;
         for (int r = 0; r < 2; ++r)
             for (int c = 0; c < 2; ++c) {
                 const int R = cb | (r ? tb : 0), C = cb | (c ? tb : 0);
-                ure[r * 2 + c] = re[R * 4 + C];
-                uim[r * 2 + c] = im[R * 4 + C];
+                ure[r * 2 + c] = re[R * d + C];
+                uim[r * 2 + c] = im[R * d + C];
             }
-        return cpos;
+        return tpos;
     }
     return -1;
 }
@@ -216,17 +215,23 @@ Engine::GatePlan Engine::plan_gate(int K, const uint32_t* q, const double* re, c
         p.kind = GatePlan::kDiag;
         return p;
     }
-    // controlled one-qubit gates: half the state, once the control bit
-    // separates whole 256-byte spans (below that the skipped half shares
-    // cache lines with the rest)
-    if (path == kPathAuto && K == 2) {
-        const int cpos = controlled_form(re, im, p.ure, p.uim);
-        if (cpos >= 0 && q[cpos] >= 4) {
+    // controlled one-qubit gates (one or two controls): half or a quarter of
+    // the state, once every control bit separates whole 256-byte spans (below
+    // that the skipped amplitudes share cache lines with the rest)
+    if (path == kPathAuto && (K == 2 || K == 3)) {
+        const int tpos = controlled_form(K, re, im, p.ure, p.uim);
+        bool high = tpos >= 0;
+        for (int m = 0; m < K && high; ++m)
+            if (m != tpos) {
+                high = q[m] >= 4;
+                p.ctrl[p.nctrl++] = q[m];
+            }
+        if (high) {
             p.kind = GatePlan::kControlled;
-            p.ctrl = q[cpos];
-            p.targ = q[1 - cpos];
+            p.targ = q[tpos];
             return p;
         }
+        p.nctrl = 0;
     }
     int chosen = choose_path(n, prec, p.g);
     if (chosen == kPathMfma && (K != 5 || n < 9)) chosen = kPathDirect;
@@ -261,7 +266,8 @@ bool Engine::launch_plan(GatePlan& p, const void* dmat) {
         return hip_ok(launch_diag(amp, n, prec, stream, p.g.K, p.g.q, dre, dimv), "diagonal gate launch");
     }
     if (p.kind == GatePlan::kControlled)
-        return hip_ok(launch_controlled(amp, n, prec, stream, p.ctrl, p.targ, p.ure, p.uim), "controlled gate launch");
+        return hip_ok(launch_controlled(amp, n, prec, stream, p.ctrl, p.nctrl, p.targ, p.ure, p.uim),
+                      "controlled gate launch");
     p.g.dmat = dmat;
     return hip_ok(launch_gate(amp, n, prec, stream, p.g), "gate launch");
 }

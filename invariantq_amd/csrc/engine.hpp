@@ -87,7 +87,8 @@ struct Engine {
         size_t bytes = 0;
         bool mfma = false;
         bool m3 = false;
-        uint32_t ctrl = 0, targ = 0;  // kControlled: U (row-major 2x2) on targ where ctrl is set
+        uint32_t ctrl[2] = {}, targ = 0;  // kControlled: U (row-major 2x2) on targ where every ctrl is set
+        int nctrl = 0;
         double ure[4] = {}, uim[4] = {};
     };
 

@@ -397,16 +397,16 @@ __global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, Mfma
 // Diagonal operator: each amplitude times the entry its target bits select.
 // Four amplitudes in flight per thread; NT streams them (large states: each
 // amplitude is touched once).
-// Controlled one-qubit gate, a two-qubit matrix blockdiag(I, U) over the
-// control bit: the identity block leaves its amplitudes unchanged, so only
-// the half of the state with the control bit set is read and written (pairs
-// base, base + toff with the control bit set and the target bit clear).
-template <typename R, bool NT>
-__global__ void __launch_bounds__(256) k_gate_controlled(cv<R>* __restrict__ psi, uint64_t npairs, DirectArgs<2> a,
-                                                         uint64_t cbit, uint64_t toff, ValMat<R, 8> mat) {
+// Controlled one-qubit gate, a K-qubit matrix blockdiag(I, U) over K-1
+// control bits: the identity block leaves its amplitudes unchanged, so only
+// the 2^-(K-1) of the state with every control bit set is read and written
+// (pairs base, base + toff with the control bits set and the target bit clear).
+template <typename R, int K, bool NT>
+__global__ void __launch_bounds__(256) k_gate_controlled(cv<R>* __restrict__ psi, uint64_t npairs, DirectArgs<K> a,
+                                                         uint64_t cbits, uint64_t toff, ValMat<R, 8> mat) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < npairs; g += stride) {
-        const uint64_t base = insert_zero_bits<2>(g, a.pos) | cbit;
+        const uint64_t base = insert_zero_bits<K>(g, a.pos) | cbits;
         R xr[2], xi[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -801,35 +801,46 @@ hipError_t launch_gate(void* psi, int n, int prec, hipStream_t st, const GateDes
 }
 
 namespace {
-template <typename R>
-hipError_t run_controlled(void* psi, int n, hipStream_t st, uint32_t c, uint32_t t, const double* ure,
+template <typename R, int K>
+hipError_t run_controlled(void* psi, int n, hipStream_t st, const uint32_t* c, uint32_t t, const double* ure,
                           const double* uim) {
-    DirectArgs<2> a;
-    a.pos[0] = std::min(c, t);
-    a.pos[1] = std::max(c, t);
+    DirectArgs<K> a;
+    uint64_t cbits = 0;
+    for (int m = 0; m + 1 < K; ++m) {
+        a.pos[m] = c[m];
+        cbits |= 1ULL << c[m];
+    }
+    a.pos[K - 1] = t;
+    std::sort(a.pos, a.pos + K);
+    for (int l = 0; l < (1 << K); ++l) a.off[l] = 0;
     ValMat<R, 8> m;  // row-major 2x2: real parts, then imaginary parts, per row
     for (int r = 0; r < 2; ++r)
         for (int k = 0; k < 2; ++k) {
             m.v[r * 4 + k] = (R)ure[r * 2 + k];
             m.v[r * 4 + 2 + k] = (R)uim[r * 2 + k];
         }
-    const uint64_t count = 1ULL << n, npairs = count >> 2;
+    const uint64_t count = 1ULL << n, npairs = count >> K;
     const unsigned blocks = grid_for(npairs, 256);
     if (count >= kStreamMinCount && g_direct_nt != 0)
-        hipLaunchKernelGGL((k_gate_controlled<R, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, npairs, a,
-                           1ULL << c, 1ULL << t, m);
+        hipLaunchKernelGGL((k_gate_controlled<R, K, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, npairs, a, cbits,
+                           1ULL << t, m);
     else
-        hipLaunchKernelGGL((k_gate_controlled<R, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, npairs, a,
-                           1ULL << c, 1ULL << t, m);
+        hipLaunchKernelGGL((k_gate_controlled<R, K, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, npairs, a,
+                           cbits, 1ULL << t, m);
     return hipGetLastError();
 }
 }  // namespace
 
-hipError_t launch_controlled(void* psi, int n, int prec, hipStream_t st, uint32_t c, uint32_t t, const double* ure,
-                             const double* uim) {
-    if (n < 2 || c == t || (int)c >= n || (int)t >= n) return hipErrorInvalidValue;
-    return prec == kC128 ? run_controlled<double>(psi, n, st, c, t, ure, uim)
-                         : run_controlled<float>(psi, n, st, c, t, ure, uim);
+hipError_t launch_controlled(void* psi, int n, int prec, hipStream_t st, const uint32_t* c, int nc, uint32_t t,
+                             const double* ure, const double* uim) {
+    if (nc < 1 || nc > 2 || nc + 1 > n || (int)t >= n) return hipErrorInvalidValue;
+    for (int m = 0; m < nc; ++m)
+        if ((int)c[m] >= n || c[m] == t || (m == 1 && c[1] == c[0])) return hipErrorInvalidValue;
+    if (nc == 1)
+        return prec == kC128 ? run_controlled<double, 2>(psi, n, st, c, t, ure, uim)
+                             : run_controlled<float, 2>(psi, n, st, c, t, ure, uim);
+    return prec == kC128 ? run_controlled<double, 3>(psi, n, st, c, t, ure, uim)
+                         : run_controlled<float, 3>(psi, n, st, c, t, ure, uim);
 }
 
 hipError_t launch_diag(void* psi, int n, int prec, hipStream_t st, int K, const uint32_t* q, const double* dre,

@@ -38,10 +38,10 @@ hipError_t launch_gate(void* psi, int n, int prec, hipStream_t st, const GateDes
 // Diagonal gate: psi[i] *= d[local(i)] (d host, 2^K entries, MSB-first local index).
 hipError_t launch_diag(void* psi, int n, int prec, hipStream_t st, int K, const uint32_t* q,
                        const double* dre, const double* dim);
-// Controlled one-qubit gate on target t (control c): psi <- U psi on the
-// amplitudes with bit c set; ure/uim row-major 2x2 (host).
-hipError_t launch_controlled(void* psi, int n, int prec, hipStream_t st, uint32_t c, uint32_t t, const double* ure,
-                             const double* uim);
+// Controlled one-qubit gate on target t (nc = 1 or 2 controls c): psi <- U psi
+// on the amplitudes with every control bit set; ure/uim row-major 2x2 (host).
+hipError_t launch_controlled(void* psi, int n, int prec, hipStream_t st, const uint32_t* c, int nc, uint32_t t,
+                             const double* ure, const double* uim);
 int choose_path(int n, int prec, const GateDesc& g);
 // Streaming (non-temporal) hints in whole-state passes: on unless turned off
 // through the matrix-core options (nontemporal = 0).
