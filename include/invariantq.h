@@ -40,6 +40,7 @@ void   iq_statevector_get_amplitudes(void* sv, double* real_out, double* imag_ou
 void   iq_statevector_get_probabilities(void* sv, double* probs_out);                                /* c_api.h:21 */
 void   iq_statevector_sample(void* sv, size_t num_shots, uint64_t seed, uint64_t* results_out);      /* c_api.h:22 */
 double iq_statevector_entropy(void* sv, const size_t* qubits, size_t num_subsys_qubits);             /* c_api.h:23 */
+/*     entropy: up to 15 subsystem qubits; NaN (error set) when it cannot be computed. */
 void   iq_statevector_expval_pauli(void* sv, const char* pauli_str, const size_t* qubits, size_t len,
                                    double* val_real, double* val_imag);                              /* c_api.h:24 */
 
@@ -156,7 +157,7 @@ void   iq_statevector_apply_fused_sequence(void* sv, size_t count, const size_t*
  * iq_statevector_sample(sv, shots[t], seeds[t], ...); results concatenated in task order. */
 void   iq_statevector_sample_tasks(void* sv, size_t ntasks, const uint64_t* seeds, const uint64_t* shots,
                                    uint64_t* results_out);
-/* Reduced density matrix of `qubits` (k = 1..12; bit b of a row index = qubits[b]), 2^k x 2^k
+/* Reduced density matrix of `qubits` (k = 1..15; bit b of a row index = qubits[b]), 2^k x 2^k
  * row-major (the reference's StateVector::partial_trace, core/src/statevector.cpp:293-330, a
  * class method with no C export). */
 void   iq_statevector_reduced_density_matrix(void* sv, const size_t* qubits, size_t k, double* re_out,

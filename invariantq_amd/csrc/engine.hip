@@ -29,6 +29,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <limits>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <random>
@@ -350,6 +352,14 @@ bool Engine::ensure_work(size_t doubles) {
     return true;
 }
 
+void Engine::trim_work(size_t keep_doubles) {
+    if (work_doubles <= keep_doubles) return;
+    sync();
+    hipFree(work);
+    work = nullptr;
+    work_doubles = 0;
+}
+
 bool Engine::ensure_stage() {
     if (stage) return true;
     if (hipMalloc(&stage, 2 * kStage * sizeof(double)) != hipSuccess) {
@@ -360,8 +370,9 @@ bool Engine::ensure_stage() {
     return true;
 }
 
-bool Engine::ensure_pinned(int arrays) {
-    const size_t need = 2 * (size_t)arrays * std::min<uint64_t>(kStage, dim);
+bool Engine::ensure_pinned(int arrays) { return ensure_pinned_doubles(2 * (size_t)arrays * std::min<uint64_t>(kStage, dim)); }
+
+bool Engine::ensure_pinned_doubles(size_t need) {
     if (!pinned_ev[0])
         for (hipEvent_t& ev : pinned_ev)
             if (!hip_ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event")) return false;
@@ -427,6 +438,28 @@ bool Engine::export_chunks(int arrays, double* const* dst, bool (*produce)(Engin
         if (!hip_ok(hipEventSynchronize(pinned_ev[c & 1]), "event")) return false;
         const uint64_t first = c * kStage, cnt = std::min<uint64_t>(kStage, dim - first);
         for (int a = 0; a < arrays; ++a) parallel_copy(dst[a] + first, slot(c, a), cnt * 8);
+    }
+    return sync();
+}
+
+bool Engine::download(const double* dsrc, double* hdst, uint64_t count) {
+    if (count <= kStage) {
+        return hip_ok(hipMemcpyAsync(hdst, dsrc, count * 8, hipMemcpyDeviceToHost, stream), "D2H") && sync();
+    }
+    if (!ensure_pinned_doubles(2 * kStage)) return false;
+    const uint64_t per = pinned_doubles / 2, nch = (count + per - 1) / per;
+    auto enqueue = [&](uint64_t c) {
+        const uint64_t first = c * per, cnt = std::min<uint64_t>(per, count - first);
+        return hip_ok(hipMemcpyAsync(pinned + (c & 1) * per, dsrc + first, cnt * 8, hipMemcpyDeviceToHost, stream),
+                      "D2H") &&
+               hip_ok(hipEventRecord(pinned_ev[c & 1], stream), "event");
+    };
+    if (!enqueue(0)) return false;
+    for (uint64_t c = 0; c < nch; ++c) {
+        if (c + 1 < nch && !enqueue(c + 1)) return false;
+        if (!hip_ok(hipEventSynchronize(pinned_ev[c & 1]), "event")) return false;
+        const uint64_t first = c * per, cnt = std::min<uint64_t>(per, count - first);
+        parallel_copy(hdst + first, pinned + (c & 1) * per, cnt * 8);
     }
     return sync();
 }
@@ -808,7 +841,7 @@ struct SampleBufs {
     uint64_t* dout;
 };
 
-bool sample_bufs(Engine* e, size_t shots, SampleBufs& b) {
+static bool sample_bufs(Engine* e, size_t shots, SampleBufs& b) {
     b.direct = sample_without_cdf(e->dim, shots);
     const size_t ncdf = b.direct ? 0 : e->dim, nscr = scan_scratch_doubles(e->dim), ns = std::max<size_t>(shots, 1);
     if (!e->ensure_work(ncdf + nscr + 2 * ns)) return false;
@@ -820,13 +853,13 @@ bool sample_bufs(Engine* e, size_t shots, SampleBufs& b) {
 }
 
 // the passes over the state that do not need the uniforms
-bool sample_prepare(Engine* e, const SampleBufs& b) {
+static bool sample_prepare(Engine* e, const SampleBufs& b) {
     return b.direct ? hip_ok(launch_cdf_chunks(e->amp, e->dim, e->prec, b.scratch, e->stream), "cdf")
                     : hip_ok(launch_cdf(e->amp, e->dim, e->prec, b.cdf, b.scratch, e->stream), "cdf");
 }
 
 // uniforms in b.du -> indices in b.dout
-bool sample_search(Engine* e, const SampleBufs& b, size_t shots) {
+static bool sample_search(Engine* e, const SampleBufs& b, size_t shots) {
     return b.direct
                ? hip_ok(launch_sample_chunks(e->amp, e->dim, e->prec, b.scratch, b.du, shots, b.dout, e->stream), "search")
                : hip_ok(launch_search(b.cdf, e->dim, b.du, shots, b.dout, e->stream), "search");
@@ -941,15 +974,18 @@ void iq_statevector_get_cdf(void* sv, double* cdf_out) {
     e->sync();
 }
 
-// reference core/src/statevector.cpp:293-330 (partial trace, LSB-first local
-// index over `qubits`) and :395-410 (entropy from capped Jacobi eigenvalues).
-// Reduced density matrix of `qubits` (bit b of a row index = qubits[b]) into the
-// work buffer: the partial trace of reference core/src/statevector.cpp:293-330,
-// on the matrix cores for 4 or more kept and 2 or more traced qubits.
-static bool reduced_matrix(Engine* e, const size_t* qubits, size_t k, std::vector<std::complex<double>>& red) {
-    if (k < 1 || k > 12 || k > (size_t)e->n) {
-        set_error(k > 12 ? kErrUnsupported : kErrInvalidArgument,
-                  "reduced matrices of 1..12 qubits (at most the state's) are supported");
+// Reduced density matrices of up to 15 qubits: 2^30 complex doubles (16 GiB)
+// on the device and again on the host.
+constexpr size_t kMaxReducedQubits = 15;
+
+// Reduced density matrix of `qubits` (bit b of a row index = qubits[b]) in a
+// host buffer of 2 * 4^k doubles (re, im interleaved; not zero-filled first):
+// the partial trace of reference core/src/statevector.cpp:293-330, on the
+// matrix cores for 4 or more kept and 2 or more traced qubits.
+static bool reduced_matrix(Engine* e, const size_t* qubits, size_t k, std::unique_ptr<double[]>& red) {
+    if (k < 1 || k > kMaxReducedQubits || k > (size_t)e->n) {
+        set_error(k > kMaxReducedQubits ? kErrUnsupported : kErrInvalidArgument,
+                  "reduced matrices of 1..15 qubits (at most the state's) are supported");
         return false;
     }
     uint32_t q[16];
@@ -968,23 +1004,38 @@ static bool reduced_matrix(Engine* e, const size_t* qubits, size_t k, std::vecto
     const size_t kd = (size_t)1 << k;
     // reduced matrix, then (matrix-core path) the state reordered to (environment, kept)
     if (!e->ensure_work(2 * kd * kd + 2 * e->dim)) return false;
-    red.resize(kd * kd);
+    red.reset(new (std::nothrow) double[2 * kd * kd]);
+    if (!red) {
+        set_error(kErrOutOfMemory, "cannot allocate %zu host doubles for the reduced matrix", 2 * kd * kd);
+        return false;
+    }
     double2* out = reinterpret_cast<double2*>(e->work);
     return hip_ok(launch_partial_trace(e->amp, e->n, e->prec, q, (int)k, out, out + kd * kd, e->stream),
                   "partial trace") &&
-           hip_ok(hipMemcpyAsync(red.data(), e->work, kd * kd * 16, hipMemcpyDeviceToHost, e->stream), "D2H") &&
-           e->sync();
+           e->download(e->work, red.get(), 2 * kd * kd);
 }
 
+// the reduced-matrix scratch is given back once it passes 2 GiB (13 or more
+// kept qubits, or large gathered states), so one entropy call does not hold
+// tens of GiB of device memory for the rest of the handle's life
+constexpr size_t kKeepWorkDoubles = (size_t)1 << 28;
+
+// reference core/src/statevector.cpp:395-410: entropy from the capped Jacobi
+// eigenvalues of the reduced matrix.  0 for an empty or full subsystem, as
+// the reference; NaN (with the error set) when the entropy cannot be computed
+// (invalid qubits, more than 15 kept, no memory), never a plausible value.
 double iq_statevector_entropy(void* sv, const size_t* qubits, size_t num_subsys_qubits) {
     if (!sv || !qubits) return 0.0;
     Engine* e = sv_of(sv);
     DeviceGuard g(e->dev);
     const size_t k = num_subsys_qubits;
     if (k == 0 || k == (size_t)e->n) return 0.0;
-    std::vector<std::complex<double>> red;
-    if (!reduced_matrix(e, qubits, k, red)) return 0.0;
-    return entropy_bits(jacobi_eigenvalues(std::move(red), (size_t)1 << k, 100, 1e-12));
+    std::unique_ptr<double[]> red;
+    const bool ok = reduced_matrix(e, qubits, k, red);
+    e->trim_work(kKeepWorkDoubles);
+    if (!ok) return std::numeric_limits<double>::quiet_NaN();
+    auto* A = reinterpret_cast<std::complex<double>*>(red.get());
+    return entropy_bits(jacobi_eigenvalues_inplace(A, (size_t)1 << k, 100, 1e-12));
 }
 
 // Reduced density matrix export (no C export in the reference: partial_trace
@@ -994,11 +1045,14 @@ void iq_statevector_reduced_density_matrix(void* sv, const size_t* qubits, size_
     if (!sv || !qubits || !re_out || !im_out) return;
     Engine* e = sv_of(sv);
     DeviceGuard g(e->dev);
-    std::vector<std::complex<double>> red;
-    if (!reduced_matrix(e, qubits, k, red)) return;
-    for (size_t i = 0; i < red.size(); ++i) {
-        re_out[i] = red[i].real();
-        im_out[i] = red[i].imag();
+    std::unique_ptr<double[]> red;
+    const bool ok = reduced_matrix(e, qubits, k, red);
+    e->trim_work(kKeepWorkDoubles);
+    if (!ok) return;
+    const size_t total = (size_t)1 << (2 * k);
+    for (size_t i = 0; i < total; ++i) {
+        re_out[i] = red[2 * i];
+        im_out[i] = red[2 * i + 1];
     }
 }
 
@@ -1234,7 +1288,7 @@ double iq_density_matrix_entropy(void* dm) {
         return v;
     }
     std::vector<double> re(d * d), im(d * d);
-    if (!h->eng.export_split(re.data(), im.data())) return 0.0;
+    if (!h->eng.export_split(re.data(), im.data())) return std::numeric_limits<double>::quiet_NaN();
     std::vector<std::complex<double>> A(d * d);
     for (size_t i = 0; i < d * d; ++i) A[i] = {re[i], im[i]};
     return entropy_bits(jacobi_eigenvalues(std::move(A), d, 120, 1e-13));

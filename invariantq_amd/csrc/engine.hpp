@@ -109,9 +109,14 @@ struct Engine {
                         int path);
     bool sync();
     bool ensure_work(size_t doubles);
+    void trim_work(size_t keep_doubles);  // frees the work buffer when it holds more
     bool ensure_stage();
     bool ensure_pinned(int arrays);
+    bool ensure_pinned_doubles(size_t need);
     bool export_chunks(int arrays, double* const* dst, bool (*produce)(Engine*, uint64_t, uint64_t));
+    // `count` doubles of device memory into pageable host memory through the
+    // pinned slots (large copies), waiting for the handle's stream first
+    bool download(const double* dsrc, double* hdst, uint64_t count);
     bool export_split(double* re, double* im);
     bool import_split(const double* re, const double* im);
     bool export_probs(double* out);

@@ -394,9 +394,6 @@ __global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, Mfma
     }
 }
 
-// Diagonal operator: each amplitude times the entry its target bits select.
-// Four amplitudes in flight per thread; NT streams them (large states: each
-// amplitude is touched once).
 // Controlled one-qubit gate, a K-qubit matrix blockdiag(I, U) over K-1
 // control bits: the identity block leaves its amplitudes unchanged, so only
 // the 2^-(K-1) of the state with every control bit set is read and written
@@ -425,6 +422,9 @@ __global__ void __launch_bounds__(256) k_gate_controlled(cv<R>* __restrict__ psi
     }
 }
 
+// Diagonal operator: each amplitude times the entry its target bits select.
+// Four amplitudes in flight per thread; NT streams them (large states: each
+// amplitude is touched once).
 template <typename R, int K, bool NT = false>
 __global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t count, DirectArgs<K> a,
                                               ValMat<double, 2 * (1 << K)> d) {

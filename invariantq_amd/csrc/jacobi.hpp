@@ -21,18 +21,25 @@
 // lose precision to underflow, are scanned with std::abs throughout.  The
 // pivots, and therefore the arithmetic and the result, are those of the full
 // scan, bit for bit.
+//
+// The first scan of every row (the O(d^2) part: 2^29 entries for a 15-qubit
+// reduced matrix) is split over host threads, rows dealt round-robin so each
+// thread gets a similar share of the triangle; rows are independent, so the
+// result does not depend on the split.
 #pragma once
 
 #include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstddef>
+#include <thread>
 #include <vector>
 
 namespace iqg {
 
-inline std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> A, size_t d, int max_iter,
-                                              double tol) {
+// Eigenvalue estimates of the d x d Hermitian matrix A (row-major), which is
+// overwritten.
+inline std::vector<double> jacobi_eigenvalues_inplace(std::complex<double>* A, size_t d, int max_iter, double tol) {
     using cplx = std::complex<double>;
     auto at = [&](size_t i, size_t j) -> cplx& { return A[i * d + j]; };
     constexpr double kSlack = 1.0 - 1e-14;  // squared-modulus filter, relative
@@ -80,7 +87,22 @@ inline std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> 
         offer(j1);
         if (two) offer(j2);
     };
-    for (size_t i = 0; i + 1 < d; ++i) rescan(i);
+    {
+        const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nt = d >= 512 ? std::min<size_t>(16, hw) : 1;
+        auto scan_rows = [&](size_t t) {
+            for (size_t i = t; i + 1 < d; i += nt) rescan(i);
+        };
+        std::vector<std::thread> pool;
+        size_t t = 1;
+        try {
+            for (; t < nt; ++t) pool.emplace_back(scan_rows, t);
+        } catch (const std::exception&) {  // no thread to be had: scan the rest here
+            for (; t < nt; ++t) scan_rows(t);
+        }
+        scan_rows(0);
+        for (auto& th : pool) th.join();
+    }
     for (int it = 0; it < max_iter; ++it) {
         double best = 0.0;
         size_t p = 0, q = 1;
@@ -124,6 +146,11 @@ inline std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> 
     std::vector<double> ev(d);
     for (size_t i = 0; i < d; ++i) ev[i] = at(i, i).real();
     return ev;
+}
+
+inline std::vector<double> jacobi_eigenvalues(std::vector<std::complex<double>> A, size_t d, int max_iter,
+                                              double tol) {
+    return jacobi_eigenvalues_inplace(A.data(), d, max_iter, tol);
 }
 
 inline double entropy_bits(const std::vector<double>& ev) {

@@ -3,7 +3,8 @@
 Gate definitions restate invariantq/ir/qir.py:67-208 (RX/RY/RZ/U3/CX/CZ ...):
 row-major, and for two-qubit gates the local index is 2*bit(q0) + bit(q1), so
 CX's control is the first wire (qir.py:116-122).  Kraus sets restate
-invariantq/noise/channels.py:18-44.
+invariantq/noise/channels.py:11-102, including the identity returned for an
+unknown channel name.
 """
 
 from __future__ import annotations
@@ -98,15 +99,51 @@ def phase_flip(p: float) -> List[np.ndarray]:
     return [math.sqrt(1 - p) * I2, math.sqrt(p) * Z]
 
 
-CHANNELS = {"depolarizing": depolarizing, "depol": depolarizing,
-            "amplitude_damping": amplitude_damping, "amp_damping": amplitude_damping, "t1": amplitude_damping,
-            "phase_damping": phase_damping, "dephasing": phase_damping, "t2": phase_damping,
-            "bit_flip": bit_flip, "bitflip": bit_flip, "phase_flip": phase_flip, "phaseflip": phase_flip}
+def thermal_relaxation(t1: float = 50e-6, t2: float = 70e-6, t_gate: float = 50e-9) -> List[np.ndarray]:
+    """channels.py:61-80: T2 clipped to 2 T1; amplitude damping with
+    gamma = 1 - exp(-t_gate / T1) followed by phase damping with
+    lambda = 1 - exp(-t_gate * max(0, 1/T2 - 1/(2 T1))); the four products
+    K_phase @ K_amp in phase-major order."""
+    t2 = min(t2, 2.0 * t1)
+    gamma = 1.0 - math.exp(-t_gate / t1) if t1 > 0 else 0.0
+    inv_tphi = (1.0 / t2) - (1.0 / (2.0 * t1)) if (t1 > 0 and t2 > 0) else 0.0
+    lam = 1.0 - math.exp(-t_gate * max(0.0, inv_tphi))
+    return [kp @ ka for kp in phase_damping(lam) for ka in amplitude_damping(gamma)]
+
+
+def depolarizing_2q(p: float) -> List[np.ndarray]:
+    """channels.py:82-99: the 16 two-qubit Paulis P_i (x) P_j, sqrt(1 - 15p/16)
+    on I (x) I and sqrt(p/16) on the rest (4x4, local index 2*bit(q0) + bit(q1))."""
+    p = min(1.0, max(0.0, p))
+    paulis = [I2, X, Y, Z]
+    out = []
+    for i in range(4):
+        for j in range(4):
+            w = math.sqrt(max(0.0, 1.0 - (15.0 / 16.0) * p)) if i == 0 and j == 0 else math.sqrt(p / 16.0)
+            out.append(w * np.kron(paulis[i], paulis[j]))
+    return out
+
+
+_ONE_PARAM = {"depolarizing": depolarizing, "depol": depolarizing,
+              "amplitude_damping": amplitude_damping, "amp_damping": amplitude_damping, "t1": amplitude_damping,
+              "phase_damping": phase_damping, "dephasing": phase_damping, "t2": phase_damping,
+              "bit_flip": bit_flip, "bitflip": bit_flip, "phase_flip": phase_flip, "phaseflip": phase_flip,
+              "depolarizing_2q": depolarizing_2q, "depol_2q": depolarizing_2q}
+CHANNELS = dict(_ONE_PARAM, thermal_relaxation=thermal_relaxation, thermal=thermal_relaxation)
 
 
 def build_kraus_channel(name: str, params: Sequence[float]) -> List[np.ndarray]:
-    """Same names and clamping as invariantq/noise/channels.py:11-60."""
-    fn = CHANNELS.get(name.lower())
+    """Same names, defaults and clamping as invariantq/noise/channels.py:11-102.
+    Thermal relaxation reads (T1, T2, t_gate) with defaults 50 us, 70 us,
+    50 ns; every other named channel reads params[0] (0 when absent); an
+    unknown name gives the identity channel [I] (channels.py:101-102)."""
+    key = name.lower()
+    if key in ("thermal_relaxation", "thermal"):
+        t1 = params[0] if len(params) > 0 else 50e-6
+        t2 = params[1] if len(params) > 1 else 70e-6
+        tg = params[2] if len(params) > 2 else 50e-9
+        return thermal_relaxation(t1, t2, tg)
+    fn = _ONE_PARAM.get(key)
     if fn is None:
-        raise ValueError(f"unknown noise channel '{name}'")
+        return [I2.copy()]
     return fn(params[0] if params else 0.0)

@@ -173,7 +173,7 @@ class StateVectorSimulator:
         probs_vec = sv.get_probabilities()
         ent = 0.0
         if entropy and n >= 2:
-            ent = sv.von_neumann_entropy(list(range(n // 2))) if n // 2 <= 12 else None
+            ent = sv.von_neumann_entropy(list(range(n // 2))) if n // 2 <= 15 else None
         if allow_non_unitary:
             total = float(np.sum(probs_vec))
             if not np.isfinite(total) or total <= 0.0:
@@ -253,8 +253,17 @@ class DensityMatrixSimulator:
             elif kind == "reset":
                 dm.reset_qubit(qubits[0])
             else:
-                for q in qubits:
-                    dm.apply_1q_kraus(q, matrix)
+                # a channel listed on two qubits whose Kraus operators are 4x4
+                # (depolarizing_2q) acts on the pair; 2x2 operators act on each
+                # listed qubit (density_matrix.py:54-58)
+                d = np.asarray(matrix[0]).shape[0]
+                if len(qubits) == 2 and d == 4:
+                    dm.apply_2q_kraus(qubits[0], qubits[1], matrix)
+                elif d == 2:
+                    for q in qubits:
+                        dm.apply_1q_kraus(q, matrix)
+                else:
+                    raise ValueError(f"Kraus operators of size {d} do not fit qubits {qubits}")
         mat = dm.get_matrix()
         trace, purity, ent = dm.trace(), dm.purity(), dm.von_neumann_entropy()
         diag = np.maximum(0.0, np.real(np.diag(mat)))

@@ -165,6 +165,17 @@ static double sv_norm2(const SV& s) {
     return acc;
 }
 
+// reference core/src/statevector.cpp:215-224: scale by 1/sqrt(norm^2) when it is positive
+static void sv_normalize(SV& s) {
+    const double n2 = sv_norm2(s);
+    if (n2 > 0.0) {
+        const double f = 1.0 / std::sqrt(n2);
+        cplx* A = s.amp.data();
+#pragma omp parallel for schedule(static)
+        for (uint64_t i = 0; i < s.dim; ++i) A[i] *= f;
+    }
+}
+
 // reference core/src/statevector.cpp:226-233
 static void sv_probs(const SV& s, double* out) {
     const cplx* A = s.amp.data();
@@ -573,6 +584,9 @@ void iqo_statevector_set_amplitudes(void* p, const double* re, const double* im)
     for (uint64_t i = 0; i < s.dim; ++i) s.amp[i] = cplx(re[i], im[i]);
 }
 double iqo_statevector_norm_squared(void* p) { return p ? sv_norm2(*static_cast<SV*>(p)) : 0.0; }
+void iqo_statevector_normalize(void* p) {
+    if (p) sv_normalize(*static_cast<SV*>(p));
+}
 // Raw std::mt19937_64 words and the uniform_real_distribution<double>(0,1)
 // draws that reference statevector.cpp:244-250 consumes, for known-answer tests.
 void iqo_mt19937_64_raw(uint64_t seed, size_t count, uint64_t* out) {

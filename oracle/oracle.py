@@ -102,6 +102,8 @@ class CoreLib:
             L.iqo_statevector_set_amplitudes.restype = None
             L.iqo_statevector_norm_squared.argtypes = [vp]
             L.iqo_statevector_norm_squared.restype = ctypes.c_double
+            L.iqo_statevector_normalize.argtypes = [vp]
+            L.iqo_statevector_normalize.restype = None
             L.iqo_density_matrix_expval_z.argtypes = [vp, _dp]
             L.iqo_density_matrix_expval_z.restype = None
             L.iqo_density_matrix_set_elements.argtypes = [vp, _dp, _dp]
@@ -215,6 +217,10 @@ class CoreStateVector:
 
     def norm_squared(self) -> float:
         return self.L.iqo_statevector_norm_squared(self._ptr)
+
+    def normalize(self):
+        """StateVector::normalize (statevector.cpp:215-224); oracle library only."""
+        self.L.iqo_statevector_normalize(self._ptr)
 
 
 class CoreDensityMatrix:

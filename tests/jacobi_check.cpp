@@ -163,6 +163,34 @@ int main() {
         bad += check(R, d, 100, 1e-12, "half-cut-1024");
         ++cases;
     }
+    {
+        // rows scanned by several host threads (d >= 512): exact ties over a
+        // 600 x 600 matrix, and a rank-8 reduced state at d = 2048 (an 11-qubit
+        // cut), against the single-threaded full scan
+        std::uniform_int_distribution<int> U(0, 3);
+        const size_t d = 600;
+        std::vector<std::complex<double>> T(d * d, 0.0);
+        for (size_t i = 0; i < d; ++i)
+            for (size_t j = i; j < d; ++j) {
+                const double mag = 0.25 * (1 + U(g) % 2);
+                std::complex<double> v = (U(g) & 1) ? std::complex<double>(0, mag) : std::complex<double>(-mag, 0);
+                if (i == j) v = {0.5 * U(g), 0.0};
+                T[i * d + j] = v;
+                T[j * d + i] = std::conj(v);
+            }
+        bad += check(T, d, 100, 1e-13, "ties-600");
+        const size_t d2 = 2048, r = 8;
+        std::vector<std::complex<double>> V(d2 * r), R(d2 * d2);
+        for (auto& x : V) x = {N(g), N(g)};
+        for (size_t i = 0; i < d2; ++i)
+            for (size_t j = 0; j < d2; ++j) {
+                std::complex<double> a = 0.0;
+                for (size_t c = 0; c < r; ++c) a += V[i * r + c] * std::conj(V[j * r + c]);
+                R[i * d2 + j] = a * (1.0 / (d2 * r));
+            }
+        bad += check(R, d2, 100, 1e-12, "rank8-2048");
+        cases += 2;
+    }
     if (bad) {
         printf("%d of %d cases differ\n", bad, cases);
         return 1;

@@ -89,3 +89,29 @@ def test_autotuner_table(M, tmp_path):
     assert again["cached"] and again["tile_bits"] == t["tile_bits"]
     e, g, _ = r.run(specs, tile_bits=t["tile_bits"], block_threads=t["block_threads"])
     assert np.all(np.isfinite(e))
+
+
+def test_batch_1024_circuits_every_size_stratified(M):
+    """A launch of 1024 circuits with n in 10..20 and up to 16 layers (the
+    batched workload's distribution), autotuned; two circuits per qubit count
+    (the deepest and one at random) against the single-circuit complex128
+    adjoint at 1e-4 relative."""
+    adjoint, batch = M
+    specs = workloads.batch_specs(1024, seed=11)
+    ns = sorted({s[0] for s in specs})
+    assert ns == list(range(10, 21)) and max(s[1] for s in specs) == 16
+    r = batch.BatchRunner("complex64")
+    tune = batch.autotune(r, specs, cache=None, per_n=16)
+    print(f"1024-circuit batch, tiles {tune['tile_bits']}, {tune['block_threads']} threads", flush=True)
+    e, g, ms = r.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"])
+    assert ms > 0 and np.all(np.isfinite(e))
+    rng = np.random.default_rng(12)
+    for n in ns:
+        idx = [i for i, s in enumerate(specs) if s[0] == n]
+        deepest = max(idx, key=lambda i: specs[i][1])
+        other = int(rng.choice([i for i in idx if i != deepest]))
+        for i in (deepest, other):
+            _, l, p = specs[i]
+            e_ref, g_ref = _single(adjoint, n, l, p)
+            assert abs(e[i] - e_ref) < 1e-4 * max(1.0, abs(e_ref)), (n, l)
+            assert np.linalg.norm(g[i] - g_ref) / np.linalg.norm(g_ref) < 1e-4, (n, l)

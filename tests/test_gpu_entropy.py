@@ -59,3 +59,67 @@ def test_reduced_matrix_errors(oracle_lib):
         sv.reduced_density_matrix([1, 1])
     with pytest.raises(ValueError):
         sv.reduced_density_matrix([6])
+
+
+def _sparse_state(n, m, rng):
+    """m random basis states with random complex weights (normalised)."""
+    idx = rng.choice(1 << n, size=m, replace=False)
+    w = rng.normal(size=m) + 1j * rng.normal(size=m)
+    psi = np.zeros(1 << n, dtype=complex)
+    psi[idx] = w / np.linalg.norm(w)
+    return psi, idx
+
+
+def _exact_entropy(psi, n, keep):
+    """Entropy of rho_A by dense diagonalisation of its non-zero block."""
+    nz = np.nonzero(psi)[0]
+    kept = [sum(((int(i) >> q) & 1) << b for b, q in enumerate(keep)) for i in nz]
+    env = [int(i) & ~sum(1 << q for q in keep) for i in nz]
+    rows = sorted(set(kept))
+    envs = sorted(set(env))
+    m = np.zeros((len(rows), len(envs)), dtype=complex)
+    for i, r, e in zip(nz, kept, env):
+        m[rows.index(r), envs.index(e)] = psi[i]
+    lam = np.linalg.eigvalsh(m @ m.conj().T)
+    lam = lam[lam > 1e-14]
+    return float(-np.sum(lam * np.log2(lam)))
+
+
+@pytest.mark.parametrize("n,k", [(14, 13), (17, 15)])
+def test_entropy_beyond_12_kept_qubits(n, k):
+    """13 and 15 kept qubits (rho_A of 2^26 and 2^30 entries): a state on four
+    basis states has a 4x4 non-zero block, which the capped Jacobi
+    diagonalises well inside its 100 rotations, so the entropy is the exact one."""
+    from invariantq_amd.core.binding import NativeStateVector
+    rng = np.random.default_rng(n * 100 + k)
+    psi, _ = _sparse_state(n, 4, rng)
+    keep = [int(q) for q in rng.permutation(n)[:k]]
+    print(f"entropy n={n} k={k}", flush=True)
+    sv = NativeStateVector(n)
+    sv.set_amplitudes(psi)
+    h = sv.von_neumann_entropy(keep)
+    assert abs(h - _exact_entropy(psi, n, keep)) < 1e-9
+
+
+def test_reduced_matrix_13_qubits_matches_oracle(oracle_lib):
+    n, keep = 14, [13, 0, 12, 1, 11, 2, 10, 3, 9, 4, 8, 5, 7]
+    sv, ref = states(oracle_lib, n, seed=21)
+    print(f"reduced matrix n={n} k={len(keep)}", flush=True)
+    got = sv.reduced_density_matrix(keep)
+    want = ref.partial_trace(keep)
+    assert np.max(np.abs(got - want)) < 1e-13
+
+
+def test_entropy_above_15_is_nan_not_a_value():
+    import ctypes
+    from invariantq_amd.core import binding
+    lib = binding.lib()
+    sv = binding.NativeStateVector(17)
+    q = np.arange(16, dtype=np.uint64)
+    lib.iq_clear_error()
+    h = lib.iq_statevector_entropy(sv._ptr, q.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)), 16)
+    assert np.isnan(h)
+    assert lib.iq_last_error() == 5  # IQ_ERR_UNSUPPORTED
+    lib.iq_clear_error()
+    # an empty or full subsystem is 0, as in the reference
+    assert lib.iq_statevector_entropy(sv._ptr, q.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)), 0) == 0.0

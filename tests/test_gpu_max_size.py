@@ -71,12 +71,26 @@ def test_largest_states(n, precision):
         assert abs(sv.measure_prob(q, 0) - p0[q]) < tol
     assert abs(sv.expectation_value_pauli_sum(terms) - tiled) < tol
 
-    if precision == "complex128":  # the CDF (64 GiB) beside the state
+    if precision == "complex128":
+        # few shots: per-chunk searches after one read of the state
         shots = 2000
         idx = sv.sample(shots, seed=5)
         assert idx.max() < (1 << n)
         for q in probe:
             freq = np.mean((idx >> np.uint64(q)) & np.uint64(1))
             sigma = np.sqrt(p0[q] * (1 - p0[q]) / shots)
+            assert abs(freq - (1 - p0[q])) < 5 * sigma + 1e-3
+        # more uniforms than twice the 2^21 chunks: the written CDF (64 GiB)
+        # beside the state and the binary search over indices past 2^32;
+        # the first 4096 agree with the per-chunk search on the same uniforms
+        u = np.random.default_rng(35).random((1 << 22) + 4096)
+        print("written-CDF sampling at 33 qubits", flush=True)
+        many = sv.sample_uniforms(u)
+        few = sv.sample_uniforms(u[:4096])
+        assert np.array_equal(many[:4096], few)
+        assert many.max() < (1 << n) and many.max() >= (1 << 32)
+        for q in probe:
+            freq = np.mean((many >> np.uint64(q)) & np.uint64(1))
+            sigma = np.sqrt(p0[q] * (1 - p0[q]) / many.size)
             assert abs(freq - (1 - p0[q])) < 5 * sigma + 1e-3
     del sv

@@ -301,3 +301,31 @@ def test_small_and_empty(B, oracle_lib):
     ref = oracle_lib.statevector(5)
     ref.apply_fused([4, 2, 0, 1, 3], U)
     assert np.max(np.abs(five.get_amplitudes() - ref.get_amplitudes())) < TOL128
+
+
+@pytest.mark.parametrize("n,precision", [(14, "complex128"), (14, "complex64"), (27, "complex128")])
+def test_normalize_unnormalised_state(oracle_lib, n, precision):
+    """StateVector::normalize (statevector.cpp:215-224) on a state of norm 3.7:
+    the result must equal the oracle's scaled state, not the input."""
+    from invariantq_amd.core import binding
+    rng = np.random.default_rng(n)
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi *= 3.7 / np.linalg.norm(psi)
+    print(f"normalize n={n} {precision}", flush=True)
+    sv = binding.NativeStateVector(n, precision=precision)
+    sv.set_amplitudes(psi)
+    assert abs(sv.norm_squared() - 3.7 ** 2) < 1e-12 * 3.7 ** 2 * (1 if precision == "complex128" else 1e6)
+    sv.normalize()
+    ref = oracle_lib.statevector(n)
+    ref.set_amplitudes(psi)
+    ref.normalize()
+    want = ref.get_amplitudes()
+    got = sv.get_amplitudes()
+    scale = np.max(np.abs(want))
+    if precision == "complex128":
+        assert np.max(np.abs(got - want)) < 1e-15
+        assert abs(sv.norm_squared() - 1.0) < 1e-13
+    else:
+        assert np.max(np.abs(got - want)) / scale < 1e-6
+        assert abs(sv.norm_squared() - 1.0) < 1e-6
+    assert np.max(np.abs(got - psi)) > 0.5 * scale  # not left as it was

@@ -16,7 +16,7 @@ def test_row_max_jacobi_matches_full_scan(tmp_path):
         orc.build(ref=False)
     exe = tmp_path / "jacobi_check"
     subprocess.run(["g++", "-O3", "-march=x86-64-v3", "-std=c++17", f"-I{ROOT / 'invariantq_amd' / 'csrc'}",
-                    str(ROOT / "tests" / "jacobi_check.cpp"), str(orc.ORACLE_LIB), "-o", str(exe),
+                    "-pthread", str(ROOT / "tests" / "jacobi_check.cpp"), str(orc.ORACLE_LIB), "-o", str(exe),
                     f"-Wl,-rpath,{orc.ORACLE_LIB.parent}"], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr

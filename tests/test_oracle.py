@@ -345,3 +345,69 @@ def test_gate_builders_match_matrix_exponentials():
     th, ph, la = 0.7, 1.9, 2.6
     decomposed = np.exp(0.5j * (ph + la)) * gates.rz(ph) @ gates.ry(th) @ gates.rz(la)
     assert np.allclose(gates.u3(th, ph, la), decomposed, atol=1e-14)
+
+
+# ------------------------------------------------ partial trace and normalize
+
+def _reduced_by_tensor_axes(psi, n, keep):
+    """rho_A by NumPy tensor reshaping, independent of the oracle's index loops:
+    axis a of psi.reshape([2]*n) is qubit n-1-a; the row index r of rho_A has
+    bit b = qubit keep[b] (statevector.cpp:306-323), so its C-order axes are
+    keep[k-1], ..., keep[0]."""
+    t = psi.reshape([2] * n)
+    kept_axes = [n - 1 - q for q in reversed(keep)]
+    env_axes = [a for a in range(n) if a not in kept_axes]
+    m = t.transpose(kept_axes + env_axes).reshape(1 << len(keep), -1)
+    return m @ m.conj().T
+
+
+def test_partial_trace_two_qubit_closed_form(oracle_lib):
+    # psi = a|00> + b|10> (index 2: qubit 1 set); keeping qubit 1 gives
+    # [[|a|^2, a conj(b)], [b conj(a), |b|^2]]: entry (0, 1) is a * conj(b),
+    # so a transposed or conjugated result fails
+    a, b = 0.6 + 0.0j, 0.48 + 0.64j
+    sv = oracle_lib.statevector(2)
+    sv.set_amplitudes(np.array([a, 0, b, 0]))
+    rho = sv.partial_trace([1])
+    expect = np.array([[abs(a) ** 2, a * np.conj(b)], [b * np.conj(a), abs(b) ** 2]])
+    assert np.max(np.abs(rho - expect)) < 1e-15
+    # keeping qubit 0 instead: everything sits in the 0 row
+    rho0 = sv.partial_trace([0])
+    assert np.max(np.abs(rho0 - np.diag([1.0, 0.0]))) < 1e-15
+
+
+@pytest.mark.parametrize("n,keep", [(7, [5, 0, 3]), (8, [2, 7]), (6, [0, 1, 2, 3]), (8, [6, 1, 4, 3, 0]),
+                                    (5, [4])])
+def test_partial_trace_elementwise_vs_tensor_reshape(oracle_lib, n, keep):
+    rng = np.random.default_rng(100 + n + len(keep))
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    sv = oracle_lib.statevector(n)
+    sv.set_amplitudes(psi)
+    rho = sv.partial_trace(keep)
+    ref = _reduced_by_tensor_axes(psi, n, keep)
+    assert np.max(np.abs(rho - ref)) < 1e-14
+    # the input is asymmetric enough that the mistakes the entropy cannot see
+    # (a transposed matrix, the kept bits permuted) fail the element-wise check
+    assert np.max(np.abs(rho.T - ref)) > 1e-3
+    if len(keep) > 1:
+        swapped = _reduced_by_tensor_axes(psi, n, [keep[1], keep[0]] + keep[2:])
+        assert np.max(np.abs(swapped - ref)) > 1e-3
+
+
+def test_normalize_scaled_state(oracle_lib):
+    # statevector.cpp:215-224: psi * (1 / sqrt(sum |psi_i|^2))
+    n = 9
+    rng = np.random.default_rng(37)
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    sv = oracle_lib.statevector(n)
+    sv.set_amplitudes(3.7 * psi)
+    assert sv.norm_squared() == pytest.approx(3.7 ** 2, rel=1e-14)
+    sv.normalize()
+    assert np.max(np.abs(sv.get_amplitudes() - psi)) < 1e-15
+    assert abs(sv.norm_squared() - 1.0) < 1e-14
+    # the zero vector is left as it is (the n2 > 0 guard)
+    sv.set_amplitudes(np.zeros(1 << n))
+    sv.normalize()
+    assert not np.any(sv.get_amplitudes())
