@@ -137,6 +137,16 @@ void   iq_statevector_apply_diagonal(void* sv, const size_t* qubits, size_t k, c
  * tree): each index is the first whose iq_statevector_get_cdf value reaches
  * the uniform, clamped to dim - 1, as for iq_statevector_sample. */
 void   iq_statevector_sample_uniforms(void* sv, size_t num_shots, const double* uniforms, uint64_t* results_out);
+/* Sampling that allocates nothing.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same indices as iq_statevector_sample_uniforms for the same uniforms, but
+ * `uniforms` and `results_out` are device-accessible memory and the call only enqueues work on
+ * the handle's stream (iq_statevector_stream): results are valid once that stream reaches them.
+ * Its scan scratch is one static 8 MiB device array per GPU, kept until the process ends, so
+ * states of at most 2^31 amplitudes are taken (larger: IQ_ERR_UNSUPPORTED); calls that overlap
+ * on one GPU (other handles, other threads) share that array and race. */
+void   iq_statevector_sample_uniforms_noalloc(void* sv, size_t num_shots, const double* uniforms,
+                                              uint64_t* results_out);
 /* Inclusive FP64 cumulative distribution used by sampling (dim doubles,
  * non-decreasing; within rounding of the serial running sum). */
 void   iq_statevector_get_cdf(void* sv, double* cdf_out);

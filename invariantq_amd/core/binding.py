@@ -135,6 +135,7 @@ SIGNATURES = {
     "iq_statevector_normalize": ([_vp], None),
     "iq_statevector_apply_diagonal": ([_vp, _s, _sz, _d, _d], None),
     "iq_statevector_sample_uniforms": ([_vp, _sz, _d, _u64], None),
+    "iq_statevector_sample_uniforms_noalloc": ([_vp, _sz, _vp, _vp], None),
     "iq_statevector_get_cdf": ([_vp, _d], None),
     "iq_statevector_expval_pauli_sum": ([_vp, _sz, _d, ctypes.c_char_p, _s, _s, _d, _d, _d], None),
     "iq_statevector_apply_fused_sequence": ([_vp, _sz, _s, _s, _d, _d], None),
@@ -368,6 +369,13 @@ class NativeStateVector:
         _NATIVE_LIB.iq_statevector_sample_uniforms(self._ptr, u.size, _dp(u), out.ctypes.data_as(_u64))
         check_error()
         return out
+
+    def sample_uniforms_noalloc(self, uniforms_ptr: int, results_ptr: int, num_shots: int) -> None:
+        """Sampling that allocates nothing: `uniforms_ptr` (float64) and
+        `results_ptr` (uint64) are device addresses of num_shots entries; the
+        call only enqueues on this handle's stream (wait with synchronize())."""
+        _NATIVE_LIB.iq_statevector_sample_uniforms_noalloc(self._ptr, num_shots, uniforms_ptr, results_ptr)
+        check_error()
 
     def von_neumann_entropy(self, subsystem_qubits: Sequence[int]) -> float:
         q = np.ascontiguousarray(subsystem_qubits, dtype=np.uint64)

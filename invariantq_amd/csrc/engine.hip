@@ -960,6 +960,82 @@ void iq_statevector_sample_uniforms(void* sv, size_t num_shots, const double* un
     e->sync();
 }
 
+// ------------------------------------------------ sampling without allocation
+// Scan scratch of the per-chunk sampling path (chunk offsets and running
+// maxima, group offsets and maxima) for states of up to 2^31 amplitudes:
+// 2 * 2^19 + 2 * 2^7 doubles, 8 MiB per GPU, held until the process ends.
+// Every entry is written by the scan before it is read, so nothing needs
+// clearing between calls.
+constexpr uint64_t kNoAllocMaxAmps = 1ULL << 31;
+constexpr size_t kNoAllocScratchDoubles = 2 * (kNoAllocMaxAmps / 4096) + 2 * (kNoAllocMaxAmps / 4096 / 4096);
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+// (static: these helpers sit inside the extern "C" block)
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's `scratch` (the host-side name is not one),
+// looked up once per device with that device current.
+static double* noalloc_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error(kErrUnsupported, "device index %d beyond %d", dev, kMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch symbol address")) return nullptr;
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+static bool device_accessible(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // an unregistered host pointer leaves an error behind
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged ||
+           (a.type == hipMemoryTypeHost && a.devicePointer != nullptr);
+}
+
+// As iq_statevector_sample_uniforms (the same indices for the same uniforms),
+// with uniforms and results in device memory, nothing allocated and nothing
+// waited for: the scan and the searches are enqueued on the handle's stream,
+// whose completion makes the results valid.  Always the per-chunk searches
+// (no CDF written).  States above 2^31 amplitudes are rejected.  Every call
+// on one GPU shares that GPU's scratch: calls that overlap on one GPU (other
+// handles' streams, other host threads) race.
+void iq_statevector_sample_uniforms_noalloc(void* sv, size_t num_shots, const double* uniforms,
+                                            uint64_t* results_out) {
+    if (!sv) return;
+    if (num_shots == 0) return;
+    if (!uniforms || !results_out) {
+        set_error(kErrInvalidArgument, "null uniforms or results");
+        return;
+    }
+    Engine* e = sv_of(sv);
+    DeviceGuard g(e->dev);
+    if (e->dim > kNoAllocMaxAmps) {
+        set_error(kErrUnsupported, "sampling without allocation takes states of at most 2^31 amplitudes");
+        return;
+    }
+    if (!device_accessible(uniforms) || !device_accessible(results_out)) {
+        set_error(kErrInvalidArgument, "uniforms and results must be device-accessible memory");
+        return;
+    }
+    if (scan_scratch_doubles(e->dim) > kNoAllocScratchDoubles) {
+        set_error(kErrUnsupported, "scan scratch beyond the static buffer");
+        return;
+    }
+    double* sc = noalloc_scratch(e->dev);
+    if (!sc) return;
+    if (!hip_ok(launch_cdf_chunks(e->amp, e->dim, e->prec, sc, e->stream), "cdf")) return;
+    hip_ok(launch_sample_chunks(e->amp, e->dim, e->prec, sc, uniforms, num_shots, results_out, e->stream), "search");
+}
+
 // Inclusive FP64 CDF of the probabilities (the running maximum of the chunked
 // prefix sums, so monotone), exported for boundary-margin checks; sampling
 // returns the first index whose value here reaches the uniform.
