@@ -1,15 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark of the gfx950 statevector hot path.
+"""Benchmark of the gfx950 statevector hot path, every workload in one line.
 
-Default (N=1): the fused-gate workload — a 30-qubit complex128 state, 200
-seeded Haar-random 32x32 unitaries on random 5-qubit subsets (even blocks on
-qubits 0-9, odd blocks unrestricted), applied through the C ABI
-(iq_statevector_apply_fused).  One step = one pass of all 200 blocks.  Value =
-achieved HBM TB/s over the timed region with algorithmic bytes 2 * 2^30 * 16 B
-per block.  The line also carries the per-kernel roofline, the complex64 pass,
-and the CPU oracle timed on a bounded sample of the same blocks.
+Default (N=1): the headline is the fused-gate workload -- a 30-qubit
+complex128 state, 200 seeded Haar-random 32x32 unitaries on random 5-qubit
+subsets (even blocks on qubits 0-9, odd blocks unrestricted), applied through
+the C ABI (iq_statevector_apply_fused).  One step = one pass of all 200
+blocks.  value = achieved HBM TB/s over the timed region with algorithmic
+bytes 2 * 2^30 * 16 B per block.  The same JSON line carries, each with its
+own roofline (dominant kernel, algorithmic bytes per launch, HIP-event
+durations on the stream the kernel runs on), error statement and CPU
+baseline (the oracle timed on a bounded sample on the host cores):
+  complex64   the same fused pass with complex64 amplitudes
+  circuit20   20 qubits, 20 layers of U3 + brick CX, 10^4 seed-tree shots
+  adjoint24   24-qubit, 576-parameter ansatz: adjoint vs GPU parameter shift
+  batch       4096 ansatz circuits (n 10..20, 4..16 layers) in one launch
+  dm14        14-qubit noisy density matrix (Kraus-apply TB/s)
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--gpus 1]
+    python bench.py --workload circuit20|dm14|batch4096|adjoint24|bandwidth|primitives
 """
 
 from __future__ import annotations
@@ -33,6 +41,44 @@ FP32_PEAK_TFLOPS = 157.3
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def host_info() -> dict:
+    """The host the CPU baselines run on: model, NUMA nodes, threads used."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        numa = len([d for d in Path("/sys/devices/system/node").iterdir() if d.name.startswith("node")])
+    except OSError:
+        numa = None
+    return {"cpu_model": model, "numa_nodes": numa, "host_cpus": os.cpu_count()}
+
+
+def cpu_leg(value, unit, kind, sample) -> dict:
+    out = {"value": value, "unit": unit, "cores": cpu_threads(), "kind": kind, "sample": sample}
+    out.update(host_info())
+    return out
+
+
+def roofline(kernel: str, bytes_per_launch: float, avg_s: float, launches: int, traffic=None, bound="hbm",
+             peak_gbps: float = None) -> dict:
+    """Roofline of one kernel: algorithmic bytes per launch over its average
+    HIP-event duration, against MI355X's 8 TB/s HBM peak."""
+    peak = peak_gbps if peak_gbps is not None else HBM_PEAK_TBPS * 1000
+    ach = bytes_per_launch / avg_s / 1e9
+    return {"bound": bound, "achieved": round(ach, 1), "peak": peak, "unit": "GB/s", "frac": round(ach / peak, 4),
+            "traffic": traffic, "kernel": kernel, "algorithmic_bytes_per_launch": int(bytes_per_launch),
+            "avg_ms": round(avg_s * 1e3, 4), "launches_timed": launches}
 
 
 def fused_workload(n: int, blocks: int, seed: int = 20260):
@@ -87,32 +133,98 @@ def measured_traffic(n: int):
         return None
 
 
-def cpu_fused_baseline(n: int, blocks_to_time: int):
-    """The oracle's fused loop (reference statevector.cpp:89-152, k<=4 guard lifted)
-    on the first blocks of the same workload, on the host cores."""
+def _chunked_diff(re_a, im_a, re_b, im_b, chunk=1 << 26):
+    """max |a_i - b_i| and |<a|b>|^2 / (|a|^2 |b|^2) over split re/im arrays."""
+    mx, dot, na, nb = 0.0, 0.0 + 0.0j, 0.0, 0.0
+    for i in range(0, re_a.size, chunk):
+        ar, ai, br, bi = re_a[i:i + chunk], im_a[i:i + chunk], re_b[i:i + chunk], im_b[i:i + chunk]
+        mx = max(mx, float(np.max(np.hypot(ar - br, ai - bi))))
+        dot += complex(np.dot(ar, br) + np.dot(ai, bi), np.dot(ar, bi) - np.dot(ai, br))
+        na += float(np.dot(ar, ar) + np.dot(ai, ai))
+        nb += float(np.dot(br, br) + np.dot(bi, bi))
+    return mx, abs(dot) ** 2 / (na * nb)
+
+
+def cpu_fused_baseline(binding, n: int, blocks_to_time: int):
+    """The oracle's fused loop (reference statevector.cpp:89-152, k<=4 guard
+    lifted) timed on the first blocks of the same workload on the host cores,
+    and the error statement: the GPU's complex128 state after the same U3
+    layer and blocks against the oracle's, element by element."""
     from oracle import oracle as orc
-    _, ops = fused_workload(n, 200)
+    init, ops = fused_workload(n, 200)
     core = orc.oracle()
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    sv = core.statevector(n)
+    ref = core.statevector(n)
+    for op in init:
+        ref.apply_1q(op.qubits[0], op.matrix)
     t = 0.0
     for op in ops[:blocks_to_time]:
         t0 = time.perf_counter()
-        sv.apply_fused(op.qubits, op.matrix)
+        ref.apply_fused(op.qubits, op.matrix)
         t += time.perf_counter() - t0
-    del sv
     bytes_per_block = 2 * (1 << n) * 16
-    return {
-        "value": round(bytes_per_block * blocks_to_time / t / 1e12, 5),
-        "unit": "TB/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"oracle C++/OpenMP fused loop on the first {blocks_to_time} of the 200 blocks at {n} qubits "
-                  f"complex128 ({t:.1f} s); per-block time {t / blocks_to_time:.2f} s",
-    }
+    out = cpu_leg(round(bytes_per_block * blocks_to_time / t / 1e12, 5), "TB/s", "port",
+                  f"oracle C++/OpenMP fused loop on the first {blocks_to_time} of the 200 blocks at {n} qubits "
+                  f"complex128 ({t:.1f} s); per-block time {t / blocks_to_time:.2f} s")
+    sv = binding.NativeStateVector(n)
+    for op in init:
+        sv.apply_1q(op.qubits[0], op.matrix)
+    for op in ops[:blocks_to_time]:
+        sv.apply_fused(op.qubits, op.matrix)
+    N = 1 << n
+    rg, ig, ro, io = (np.empty(N) for _ in range(4))
+    binding.lib().iq_statevector_get_amplitudes(sv._ptr, binding._dp(rg), binding._dp(ig))
+    binding.check_error()
+    del sv
+    core.lib.iq_statevector_get_amplitudes(ref._ptr, binding._dp(ro), binding._dp(io))
+    del ref
+    mx, fid = _chunked_diff(rg, ig, ro, io)
+    err = {"what": f"GPU complex128 state vs the oracle after the U3 layer and the first {blocks_to_time} blocks "
+                   f"at {n} qubits", "max_abs_amp_diff": mx, "one_minus_fidelity": 1.0 - fid,
+           "bar": "max |d amp| <= 1e-12"}
+    return out, err
 
 
-def adjoint_vs_shift(torch, n: int, layers: int, steps: int, shift_params: int):
+class _DeviceArray:
+    """A raw device buffer described by __cuda_array_interface__, for torch.as_tensor."""
+
+    def __init__(self, ptr: int, count: int, typestr: str):
+        self.__cuda_array_interface__ = {"shape": (count,), "typestr": typestr, "data": (ptr, False),
+                                         "version": 2, "strides": None}
+
+
+def c64_fidelity(binding, torch, n: int, blocks: int):
+    """One pass of the U3 layer and the blocks in complex64 and in complex128;
+    the overlap of the two states on the device (FP64 sums)."""
+    init, ops = fused_workload(n, blocks)
+    states = {}
+    for prec in ("complex128", "complex64"):
+        sv = binding.NativeStateVector(n, precision=prec)
+        for op in init:
+            sv.apply_1q(op.qubits[0], op.matrix)
+        for op in ops:
+            sv.apply_fused(op.qubits, op.matrix)
+        sv.synchronize()
+        states[prec] = sv
+    N = 1 << n
+    a = torch.as_tensor(_DeviceArray(states["complex64"].device_ptr, N, "<c8"), device="cuda")
+    b = torch.as_tensor(_DeviceArray(states["complex128"].device_ptr, N, "<c16"), device="cuda")
+    dot, na, nb, mx = 0.0 + 0.0j, 0.0, 0.0, 0.0
+    chunk = 1 << 26
+    for i in range(0, N, chunk):
+        x = a[i:i + chunk].to(torch.complex128)
+        y = b[i:i + chunk]
+        dot += complex(torch.vdot(x, y).item())
+        na += float(torch.vdot(x, x).real.item())
+        nb += float(torch.vdot(y, y).real.item())
+        mx = max(mx, float((x - y).abs().max().item()))
+    torch.cuda.synchronize()
+    del a, b, states
+    return {"what": f"complex64 vs complex128 after one pass (U3 layer + {blocks} blocks) at {n} qubits",
+            "one_minus_fidelity": 1.0 - abs(dot) ** 2 / (na * nb), "max_abs_amp_diff": mx,
+            "bar": "fidelity >= 1 - 1e-6, max |d amp| <= 2e-5"}
+
+
+def adjoint_vs_shift(torch, n: int, layers: int, steps: int, shift_params: int, cpu: bool = False):
     """Adjoint gradient (complex64 state, FP64 accumulation) vs parameter-shift on the same GPU."""
     from invariantq_amd import adjoint, workloads
     ans = workloads.Ansatz(n, layers)
@@ -138,6 +250,12 @@ def adjoint_vs_shift(torch, n: int, layers: int, steps: int, shift_params: int):
     rel_adj = float(np.linalg.norm(g64 - g128) / np.linalg.norm(g128))
     rel_adj_vs_ps = float(np.linalg.norm(g64[which] - ps128) / np.linalg.norm(ps128))
     rel_ps = float(np.linalg.norm(ps64 - ps128) / np.linalg.norm(ps128))
+    # one more gradient with every launch bracketed by events on the circuit's
+    # stream: the reverse tile passes are the dominant kernel
+    fast.set_timing(True)
+    fast.gradient(params)
+    times = fast.launch_times()
+    fast.set_timing(False)
     n_amp = 1 << n
     segs = fast.num_segments
     # HBM traffic of the tiled sweeps: a forward pass reads and writes the
@@ -147,6 +265,44 @@ def adjoint_vs_shift(torch, n: int, layers: int, steps: int, shift_params: int):
     # the same gradient gate by gate: each gate a forward pass, each gate a
     # reverse pass over psi and lambda, one observable pass
     gate_bytes = fast.num_gates * 2 * n_amp * 8 + 2 * n_amp * 8 + fast.num_gates * 4 * n_amp * 8
+    by_kind = {}
+    for k, ms in times:
+        by_kind.setdefault(k, []).append(ms * 1e-3)
+    rev, fwd = by_kind.get("reverse", []), by_kind.get("forward", [])
+    launch_stats = {k: {"launches": len(v), "total_ms": round(sum(v) * 1e3, 4), "avg_ms": round(np.mean(v) * 1e3, 4)}
+                    for k, v in by_kind.items()}
+    if fwd:
+        launch_stats["forward"]["achieved_TBps"] = round(2 * n_amp * 8 / np.mean(fwd) / 1e12, 3)
+    out_roof = None
+    if rev:
+        # a reverse pass reads the stored psi and reads and writes lambda: 3 N 8 B
+        out_roof = roofline("k_tile_rev<float, true> (reverse tile pass)", 3 * n_amp * 8, float(np.mean(rev)), len(rev))
+        out_roof["per_pass_in_layer_TBps"] = [round(3 * n_amp * 8 / float(np.mean(rev[i::3])) / 1e12, 3)
+                                              for i in range(min(3, len(rev)))]
+    cpu_out = None
+    if cpu:
+        # the oracle's FP64 parameter shift: two full evaluations per parameter;
+        # timed on one layer of the circuit and the observable, extrapolated
+        from oracle import oracle as orc
+        core = orc.oracle()
+        ops = orc.hea_ops(n, layers, params)
+        per_layer = len(ops) // layers
+        ref = core.statevector(n)
+        t0 = time.perf_counter()
+        for op in ops[:per_layer]:
+            qs, m = op
+            (ref.apply_1q(qs[0], m) if len(qs) == 1 else ref.apply_2q(qs[0], qs[1], m))
+        t_layer = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for c_, o_, q_ in orc.tfim_terms(n):
+            ref.expectation_value_pauli(o_, q_)
+        t_obs = time.perf_counter() - t0
+        del ref
+        t_eval = layers * t_layer + t_obs
+        cpu_out = cpu_leg(round(1.0 / (2.0 * t_eval), 5), "gradients/s", "port",
+                          f"oracle FP64 parameter shift (2 evaluations per parameter); one evaluation "
+                          f"= {layers} x one timed layer ({t_layer:.2f} s) + the {len(orc.tfim_terms(n))}-term "
+                          f"observable ({t_obs:.2f} s) at {n} qubits")
     return {
         "workload": f"{n} qubits, {layers} layers RY+RZ+CZ ring ({fast.num_gates} gates, {ans.num_params} params), "
                     f"TFIM open chain, complex64 state, FP64 accumulation",
@@ -165,6 +321,13 @@ def adjoint_vs_shift(torch, n: int, layers: int, steps: int, shift_params: int):
         "adjoint_effective_TBps": round(adj_bytes / t_adj / 1e12, 3),
         "gate_by_gate_GB_per_gradient": round(gate_bytes / 1e9, 1),
         "gate_by_gate_equivalent_TBps": round(gate_bytes / t_adj / 1e12, 2),
+        "launches": launch_stats,
+        "roofline": out_roof,
+        "error": {"what": "complex64 adjoint gradient vs complex128 parameter shift (timed parameters) and "
+                          "vs the complex128 adjoint (all)",
+                  "rel_l2_vs_param_shift_c128": rel_adj_vs_ps, "rel_l2_vs_adjoint_c128": rel_adj,
+                  "bar": "relative L2 <= 1e-4"},
+        "cpu_baseline": cpu_out,
     }
 
 
@@ -206,13 +369,36 @@ def circuit_workload(binding, torch, steps: int, warmup: int, cpu: bool):
         t_total += t2 - t0
         t_gpu += ev[0].elapsed_time(ev[1]) * 1e-3
     t_gates, t_total, t_gpu = t_gates / steps, t_total / steps, t_gpu / steps
+    # one more run with every gate bracketed by events on the handle's stream:
+    # per-kind launch durations (the one-qubit U3 launches dominate)
+    pairs = []
+    sv.reset()
+    for op in ops:
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        if len(op.qubits) == 1:
+            sv.apply_1q(op.qubits[0], op.matrix)
+        else:
+            sv.apply_2q(op.qubits[0], op.qubits[1], op.matrix)
+        b.record(stream)
+        pairs.append((len(op.qubits), a, b))
+    sv.synchronize()
+    d1 = [a.elapsed_time(b) * 1e-3 for k, a, b in pairs if k == 1]
+    d2 = [a.elapsed_time(b) * 1e-3 for k, a, b in pairs if k == 2]
+    gate_bytes = 2 * (1 << n) * 16
     out = {"workload": "20 qubits complex128, 20 layers U3 + brick CX (590 gates), 10^4 seed-tree shots",
            "gate_applications_per_s": round(len(ops) / t_gates, 1),
            "gate_applications_per_s_gpu_time": round(len(ops) / t_gpu, 1),
            "circuit_ms": round(t_gates * 1e3, 3), "circuit_gpu_ms": round(t_gpu * 1e3, 3),
            "circuit_plus_shots_ms": round(t_total * 1e3, 3),
            "effective_TBps_gpu_time": round(len(ops) * 2 * (1 << n) * 16 / t_gpu / 1e12, 3),
-           "first_samples": [int(x) for x in samples[:4]]}
+           "first_samples": [int(x) for x in samples[:4]],
+           # the 16 MiB state stays in the 256 MiB Infinity Cache: the HBM roof is
+           # a reference line here, launch overhead and cache bandwidth bound it
+           "roofline": roofline("U3 one-qubit gate launches (k_gate_direct_val / k_gate_lanes)", gate_bytes,
+                                float(np.mean(d1)), len(d1)),
+           "cx_launch_avg_ms": round(float(np.mean(d2)) * 1e3, 4),
+           "cx_launch_TBps": round(gate_bytes / float(np.mean(d2)) / 1e12, 3)}
     # the same circuit fused into blocks of up to 5 qubits (fusion.py)
     from invariantq_amd import fusion
     t0 = time.perf_counter()
@@ -300,10 +486,12 @@ def circuit_workload(binding, torch, steps: int, warmup: int, cpu: bool):
         out["entropy"]["k8_cpu_oracle_ms"] = round((time.perf_counter() - te) * 1e3, 1)
         out["entropy"]["k8_abs_diff_vs_oracle"] = abs(rh8 - h8)
         out["samples_bit_exact_vs_oracle"] = bool(np.array_equal(samples, rs))
-        out["cpu_baseline"] = {"value": round(len(ops) / (t1 - t0), 1), "unit": "gate-applications/s",
-                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
-                               "sample": f"oracle C++/OpenMP, the full 590-gate circuit ({(t1 - t0) * 1e3:.1f} ms) "
-                                         f"and 10^4 shots ({(t2 - t1) * 1e3:.1f} ms)"}
+        out["error"] = {"what": "final 20-qubit state vs the oracle; 10^4 shots vs the oracle's",
+                        "max_abs_amp_diff": out["max_abs_amp_err_vs_oracle"],
+                        "samples_bit_exact": out["samples_bit_exact_vs_oracle"], "bar": "max |d amp| <= 1e-12"}
+        out["cpu_baseline"] = cpu_leg(round(len(ops) / (t1 - t0), 1), "gate-applications/s", "port",
+                                      f"oracle C++/OpenMP, the full 590-gate circuit ({(t1 - t0) * 1e3:.1f} ms) "
+                                      f"and 10^4 shots ({(t2 - t1) * 1e3:.1f} ms)")
     return out
 
 
@@ -497,6 +685,9 @@ def density_matrix_workload(binding, torch, cpu: bool):
                       "TBps": round(byts / float(np.mean(v)) / 1e12, 4)}
     out["kraus_apply_TBps"] = out["kraus"]["TBps"]
     out["kraus_apply_pct_hbm_peak"] = round(100 * out["kraus"]["TBps"] / HBM_PEAK_TBPS, 2)
+    # a one-qubit Kraus channel is one superoperator pass over the 2n-qubit vector
+    out["roofline"] = roofline("Kraus apply: superoperator pass on rho (k_gate_direct_val / k_gate_lanes)", byts,
+                               float(np.mean(kinds["kraus"])), len(kinds["kraus"]))
     # the same circuit as fused superoperators on up to 2 qubits (fusion.py)
     from invariantq_amd import fusion
     t0 = time.perf_counter()
@@ -519,6 +710,11 @@ def density_matrix_workload(binding, torch, cpu: bool):
                     "speedup_wall": round(wall / fwall, 2),
                     "trace_diff": abs(ft - trace), "purity_diff": abs(fp - purity),
                     "max_expval_z_diff": float(np.max(np.abs(np.asarray(fz) - np.asarray(z))))}
+    out["error"] = {"what": "trace and purity, op by op vs fused superoperators; <Z_i> op by op vs fused",
+                    "trace_minus_one": trace - 1.0, "purity": purity,
+                    "trace_diff_fused": out["fused"]["trace_diff"], "purity_diff_fused": out["fused"]["purity_diff"],
+                    "max_expval_z_diff_fused": out["fused"]["max_expval_z_diff"],
+                    "bar": "|trace - 1| <= 1e-12; fused vs op by op <= 1e-12"}
     if cpu:
         from invariantq_amd import gates as G
         from oracle import oracle as orc
@@ -526,10 +722,9 @@ def density_matrix_workload(binding, torch, cpu: bool):
         t0 = time.perf_counter()
         ref.apply_1q_kraus(3, G.depolarizing(1e-3))
         t_k = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(byts / t_k / 1e12, 5), "unit": "TB/s",
-                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
-                               "sample": f"oracle restatement of density_matrix.cpp:147-163 as written (4 copies + "
-                                         f"left/right multiplies), one depolarizing Kraus apply at 14 qubits: {t_k:.2f} s"}
+        out["cpu_baseline"] = cpu_leg(round(byts / t_k / 1e12, 5), "TB/s", "port",
+                                      f"oracle restatement of density_matrix.cpp:147-163 as written (4 copies + "
+                                      f"left/right multiplies), one depolarizing Kraus apply at 14 qubits: {t_k:.2f} s")
     return out
 
 
@@ -591,6 +786,22 @@ def batch_workload(torch, count: int, cpu: bool):
                            "circuits_per_s": round(len(cls) / (cms * 1e-3), 1), "bound": bound,
                            "achieved_TBps": round(ach / 1e12, 3), "frac": round(ach / peak, 3)}
     out["size_classes"] = classes
+    # The launch's roofline: every class at its own bound (HBM for circuits
+    # wider than the tile, LDS for those resident in it); the floor is the sum
+    # of each class's bytes over its peak, and frac = floor / launch time.
+    # achieved is the floor time's HBM-equivalent bytes over the launch time.
+    floor_s = sum(c["achieved_TBps"] * 1e12 * c["ms"] * 1e-3 / (lds_peak if c["bound"] == "lds" else
+                                                              HBM_PEAK_TBPS * 1e12) for c in classes.values())
+    t_launch = ms * 1e-3
+    out["roofline"] = {"bound": "hbm", "kernel": "k_batch_hea (one launch, 4096 circuits)",
+                       "achieved": round(floor_s * HBM_PEAK_TBPS * 1e12 / t_launch / 1e9, 1),
+                       "peak": HBM_PEAK_TBPS * 1000, "unit": "GB/s", "frac": round(floor_s / t_launch, 4),
+                       "traffic": None, "avg_ms": round(ms, 3), "launches_timed": 1,
+                       "note": "classes wider than the tile at the HBM roof, resident ones at the LDS roof "
+                               "(256 CUs x 128 B/clk x 2.4 GHz); bytes per class in size_classes"}
+    out["error"] = {"what": "energies and gradients of 12 circuits (stratified) vs the single-circuit complex128 "
+                            "adjoint", "max_rel_l2_grad": out["max_rel_grad_err_vs_complex128"],
+                    "bar": "relative L2 <= 1e-4"}
     if cpu:
         from oracle import oracle as orc
         core = orc.oracle()
@@ -602,10 +813,9 @@ def batch_workload(torch, count: int, cpu: bool):
                                      which=list(range(8)))
             cnt += 8
         t = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(cnt / t, 2), "unit": "gradients/s",
-                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
-                               "kind": "port", "sample": f"oracle FP64 parameter shift, 8 parameters of each of "
-                                                          f"{len(sample)} circuits with n in (10, 12) ({t:.2f} s)"}
+        out["cpu_baseline"] = cpu_leg(round(cnt / t, 2), "gradients/s", "port",
+                                      f"oracle FP64 parameter shift, 8 parameters of each of {len(sample)} circuits "
+                                      f"with n in (10, 12) ({t:.2f} s)")
     return out
 
 
@@ -621,14 +831,16 @@ def main():
     ap.add_argument("--no-c64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--shift-params", type=int, default=24)
-    ap.add_argument("--only-fused", action="store_true", help="skip the c64, adjoint and CPU legs")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the circuit20, dm14 and batch sub-objects of the default line")
+    ap.add_argument("--only-fused", action="store_true", help="skip every leg but the complex128 fused pass")
     ap.add_argument("--workload", default="fused",
                     choices=["fused", "circuit20", "dm14", "batch4096", "adjoint24", "bandwidth", "primitives"],
                     help="fused: the headline line; circuit20 / dm14 / batch4096 / adjoint24: the other "
                          "workloads, each printed as one JSON line")
     a = ap.parse_args()
     if a.only_fused:
-        a.no_c64 = a.no_adjoint = a.no_cpu = True
+        a.no_c64 = a.no_adjoint = a.no_cpu = a.no_workloads = True
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU: the statevector is not sharded across GPUs, "
                                    "so only --gpus 1 is measured", "n_gpus": a.gpus}))
@@ -651,7 +863,7 @@ def main():
         print(json.dumps(density_matrix_workload(binding, torch, not a.no_cpu)), flush=True)
         return
     if a.workload == "adjoint24":
-        print(json.dumps(adjoint_vs_shift(torch, 24, 12, max(1, a.steps), a.shift_params)), flush=True)
+        print(json.dumps(adjoint_vs_shift(torch, 24, 12, max(1, a.steps), a.shift_params, not a.no_cpu)), flush=True)
         return
     if a.workload == "bandwidth":
         print(json.dumps(bandwidth_workload(binding, torch)), flush=True)
@@ -697,23 +909,44 @@ def main():
                "pct_hbm_peak": round(100 * b64 * a.blocks / wall64 / 1e12 / HBM_PEAK_TBPS, 2),
                "ms_per_step": round(wall64 * 1e3, 3),
                "fp32_TFLOPs": round(flops_block * a.blocks / wall64 / 1e12, 3),
-               "avg_kernel_ms": round(float(np.mean(mean64)) * 1e3, 4), "norm": norm64}
+               "avg_kernel_ms": round(float(np.mean(mean64)) * 1e3, 4), "norm": norm64,
+               "roofline": roofline("k_gate_mfma<float, ...> (all 200 blocks)", b64, float(np.mean(mean64)),
+                                    int(mean64.size))}
+        try:
+            c64["error"] = c64_fidelity(binding, torch, n, a.blocks)
+        except Exception as exc:  # the statement is reported, never required
+            c64["error"] = {"unavailable": str(exc)}
         log(f"c64 {c64}")
 
     adj = None
     if not a.no_adjoint:
         log("adjoint vs parameter shift: 24 qubits, 12 layers")
-        adj = adjoint_vs_shift(torch, 24, 12, max(1, a.steps), a.shift_params)
+        adj = adjoint_vs_shift(torch, 24, 12, max(1, min(a.steps, 10)), a.shift_params, not a.no_cpu)
         log(f"adjoint {adj}")
 
-    cpu = None
+    cpu, err = None, None
     if not a.no_cpu:
-        log(f"cpu baseline: {a.cpu_blocks} blocks at n={n}")
+        log(f"cpu baseline and oracle error: {a.cpu_blocks} blocks at n={n}")
         try:
-            cpu = cpu_fused_baseline(n, a.cpu_blocks)
+            cpu, err = cpu_fused_baseline(binding, n, a.cpu_blocks)
         except Exception as exc:  # the baseline is reported, never required
             cpu = {"error": str(exc)}
-        log(f"cpu {cpu}")
+        log(f"cpu {cpu} error {err}")
+
+    subs = {}
+    if not a.no_workloads:
+        wsteps = max(1, min(a.steps, 10))
+        for name, fn in (("circuit20", lambda: circuit_workload(binding, torch, wsteps, 2, not a.no_cpu)),
+                         ("dm14", lambda: density_matrix_workload(binding, torch, not a.no_cpu)),
+                         ("batch", lambda: batch_workload(torch, 4096, not a.no_cpu))):
+            log(f"workload {name}")
+            t0 = time.perf_counter()
+            try:
+                subs[name] = fn()
+            except Exception as exc:  # one workload's failure does not hide the others
+                subs[name] = {"error": f"{type(exc).__name__}: {exc}"}
+            subs[name]["bench_s"] = round(time.perf_counter() - t0, 1)
+            log(f"{name} {subs[name]}")
 
     out = {
         "metric": "30q fused-gate HBM TB/s (% peak); adjoint grads/s vs param-shift, same GPU",
@@ -739,10 +972,12 @@ def main():
                      "traffic": measured_traffic(n), "kernel": dom,
                      "algorithmic_bytes_per_launch": bytes_block},
         "kernels": kstats,
+        "error": err,
         "complex64": c64,
-        "adjoint": adj,
+        "adjoint24": adj,
         "cpu_baseline": cpu,
     }
+    out.update(subs)
     print(json.dumps(out), flush=True)
 
 

@@ -229,6 +229,13 @@ void*  iq_circuit_stream(void* c);
  * (default 16 GiB).  While they fit, the reverse sweep reads psi back from them
  * instead of taking it back gate by gate; 0 turns them off.  -1 for a null handle. */
 int    iq_circuit_set_stored_budget(void* c, size_t bytes);
+/* Per-launch timing (off by default; a profiling aid): while on, each launch of the tiled
+ * sweeps (16 qubits and up) is bracketed by HIP events on the circuit's stream.  After an
+ * expectation or gradient, iq_circuit_launch_times waits for the stream and returns the
+ * number of timed launches, writing up to `cap` kinds (0 forward pass, 1 observable, 2 reverse
+ * pass, 3 two-qubit gate pass, 4 gradient finalize) and durations in ms. */
+int    iq_circuit_set_timing(void* c, int on);
+size_t iq_circuit_launch_times(void* c, int* kinds, double* ms, size_t cap);
 
 /* ------------------------------------------- batched sweep, one persistent launch
  * No reference counterpart.  Circuits are the ansatz above (RY, RZ on every

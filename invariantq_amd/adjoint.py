@@ -123,6 +123,22 @@ class GpuCircuit:
         if binding._NATIVE_LIB.iq_circuit_set_stored_budget(self._ptr, nbytes) != 0:
             binding.check_error()
 
+    LAUNCH_KINDS = ("forward", "observable", "reverse", "two_qubit", "finalize")
+
+    def set_timing(self, on: bool) -> None:
+        """Bracket every launch of the tiled sweeps (16 qubits and up) with
+        HIP events on the circuit's stream (a profiling aid; off by default)."""
+        binding._NATIVE_LIB.iq_circuit_set_timing(self._ptr, 1 if on else 0)
+
+    def launch_times(self):
+        """[(kind, ms)] of the last expectation or gradient's launches, in
+        launch order, while timing is on."""
+        n = binding._NATIVE_LIB.iq_circuit_launch_times(self._ptr, None, None, 0)
+        kinds = (ctypes.c_int * max(n, 1))()
+        ms = np.empty(max(n, 1), dtype=np.float64)
+        n = binding._NATIVE_LIB.iq_circuit_launch_times(self._ptr, kinds, _dp(ms), n)
+        return [(self.LAUNCH_KINDS[kinds[i]], float(ms[i])) for i in range(n)]
+
     # ------------------------------------------------------------ evaluate
     def _params(self, params) -> np.ndarray:
         p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1))

@@ -1039,9 +1039,16 @@ struct Circuit {
     size_t stored_budget = size_t(16) << 30;
     bool stored_valid = false;  // the last forward sweep filled them
     ObsPlan oplan;  // the observable over tile passes (16 to 34 qubits)
+    // Per-launch timing (off by default): each launch of an evaluation is
+    // bracketed by two events on the circuit's stream; kinds are kLaunch*.
+    bool timing = false;
+    std::vector<hipEvent_t> tev;  // start, end pairs
+    std::vector<int> tkind;
+    size_t tused = 0;
 
     ~Circuit() {
         if (eng.stream) hipStreamSynchronize(eng.stream);
+        for (hipEvent_t ev : tev) hipEventDestroy(ev);
         if (lam) hipFree(lam);
         if (stored) hipFree(stored);
         if (gpart) hipFree(gpart);
@@ -1058,6 +1065,35 @@ struct Circuit {
 };
 
 namespace {
+
+enum LaunchKind : int { kLaunchForward = 0, kLaunchObservable = 1, kLaunchReverse = 2, kLaunchTwoQubit = 3,
+                        kLaunchFinalize = 4 };
+
+// Brackets the launches made during its lifetime with events when timing is on.
+struct TimedLaunch {
+    Circuit& c;
+    size_t slot = SIZE_MAX;
+    TimedLaunch(Circuit& circ, int kind) : c(circ) {
+        if (!c.timing) return;
+        if (c.tused == c.tkind.size()) {
+            hipEvent_t a = nullptr, b = nullptr;
+            if (hipEventCreate(&a) != hipSuccess) return;
+            if (hipEventCreate(&b) != hipSuccess) {
+                hipEventDestroy(a);
+                return;
+            }
+            c.tev.push_back(a);
+            c.tev.push_back(b);
+            c.tkind.push_back(kind);
+        }
+        slot = c.tused++;
+        c.tkind[slot] = kind;
+        hipEventRecord(c.tev[2 * slot], c.eng.stream);
+    }
+    ~TimedLaunch() {
+        if (slot != SIZE_MAX) hipEventRecord(c.tev[2 * slot + 1], c.eng.stream);
+    }
+};
 
 M2 sub_matrix(const TapeGate& g, int which, const double* params) {
     auto P = [&](int i) { return params[g.p[i]]; };
@@ -1318,6 +1354,7 @@ bool apply_observable_tiled(Circuit& c, bool write, double* energy_dev) {
 template <typename R>
 bool apply_observable(Circuit& c, bool write, double* energy_dev) {
     Engine& e = c.eng;
+    TimedLaunch tl(c, kLaunchObservable);
     if (c.oplan.obs_tiled) return apply_observable_tiled<R>(c, write, energy_dev);
     const unsigned b = nblocks_for(e.dim);
     if (write)
@@ -1918,6 +1955,7 @@ bool run_forward_tiled(Circuit& c, const double* params, bool store) {
         void* src = i == 0 ? e.amp : pass_out(c, st, i - 1, nl);
         void* dst = pass_out(c, st, i, nl);
         if (!l.tile) {
+            TimedLaunch tl(c, kLaunchTwoQubit);
             if (src != dst && !hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, e.stream), "state copy"))
                 return false;
             if (!launch_two_qubit(c, c.segs[l.seg2], false, dst, nullptr)) return false;
@@ -1926,6 +1964,7 @@ bool run_forward_tiled(Circuit& c, const double* params, bool store) {
         const size_t lds = ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>);
         const unsigned grid = (unsigned)std::min<uint64_t>(l.d.ntiles, 2048);
         hipFuncSetAttribute((const void*)k_tile_fwd<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        TimedLaunch tl(c, kLaunchForward);
         hipLaunchKernelGGL(k_tile_fwd<R>, dim3(grid), dim3(512), lds, e.stream, (const cv<R>*)src, (cv<R>*)dst, l.d,
                            dsegs);
         if (!hip_ok(hipGetLastError(), "tiled forward")) return false;
@@ -1951,18 +1990,21 @@ bool run_reverse_tiled(Circuit& c, const double* params) {
         const Launch& l = launches[k];
         const size_t i = nl - 1 - k;  // the pass's place in the forward sweep
         if (!l.tile) {
+            TimedLaunch tl(c, kLaunchTwoQubit);
             if (!launch_two_qubit(c, c.segs[l.seg2], true, st ? nullptr : e.amp, c.lam)) return false;
             continue;
         }
         const size_t lds = 2 * ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>) + (size_t)8 * kPassSegMax * kSubMax * sizeof(double);
         auto kern = st ? k_tile_rev<R, true> : k_tile_rev<R, false>;
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        TimedLaunch tl(c, kLaunchReverse);
         hipLaunchKernelGGL(kern, dim3(c.nblk), dim3(512), lds, e.stream, (cv<R>*)pass_out(c, st, i, nl),
                            (cv<R>*)c.lam, l.d, dsegs, c.gpart, (int)c.nblk);
         if (!hip_ok(hipGetLastError(), "tiled reverse")) return false;
     }
     c.stored_valid = false;
     if (c.nslots > 0) {
+        TimedLaunch tl(c, kLaunchFinalize);
         hipLaunchKernelGGL(k_finalize_slots, dim3(c.nslots * kSubMax), dim3(256), 0, e.stream, c.gpart, (int)c.nblk,
                            c.nslots, c.gout);
         if (!hip_ok(hipGetLastError(), "gradient finalize")) return false;
@@ -2200,11 +2242,35 @@ double iq_circuit_expectation(void* cp, const double* params, size_t nparams) {
     Circuit& c = *circ_of(cp);
     DeviceGuard dg(c.eng.dev);
     if (!prepare(c, params, nparams)) return 0.0;
+    c.tused = 0;
     const bool ok = c.prec == kC128 ? (run_forward<double>(c, params) && apply_observable<double>(c, false, c.eng.red_out))
                                     : (run_forward<float>(c, params) && apply_observable<float>(c, false, c.eng.red_out));
     double e = 0.0;
     if (ok) c.eng.fetch(1, &e);
     return e;
+}
+
+int iq_circuit_set_timing(void* cp, int on) {
+    if (!cp) return -1;
+    Circuit& c = *circ_of(cp);
+    c.timing = on != 0;
+    c.tused = 0;
+    return 0;
+}
+
+size_t iq_circuit_launch_times(void* cp, int* kinds, double* ms, size_t cap) {
+    if (!cp) return 0;
+    Circuit& c = *circ_of(cp);
+    DeviceGuard dg(c.eng.dev);
+    if (!c.eng.sync()) return 0;
+    const size_t m = std::min(cap, c.tused);
+    for (size_t i = 0; i < m; ++i) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, c.tev[2 * i], c.tev[2 * i + 1]) != hipSuccess) t = -1.f;
+        if (kinds) kinds[i] = c.tkind[i];
+        if (ms) ms[i] = t;
+    }
+    return c.tused;
 }
 
 // Adjoint gradient: grad_out[j] = d<H>/d params[j] (summed over every gate using j).
@@ -2213,6 +2279,7 @@ int iq_circuit_gradient(void* cp, const double* params, size_t nparams, double* 
     Circuit& c = *circ_of(cp);
     DeviceGuard dg(c.eng.dev);
     if (!prepare(c, params, nparams) || !ensure_grad_buffers(c)) return -1;
+    c.tused = 0;
     const bool ok = c.prec == kC128
                         ? (run_forward<double>(c, params, true) && apply_observable<double>(c, true, c.eng.red_out) &&
                            run_reverse<double>(c, params))
