@@ -229,6 +229,9 @@ void*  iq_circuit_stream(void* c);
  * (default 16 GiB).  While they fit, the reverse sweep reads psi back from them
  * instead of taking it back gate by gate; 0 turns them off.  -1 for a null handle. */
 int    iq_circuit_set_stored_budget(void* c, size_t bytes);
+/* Reverse passes over stored pass outputs stage lambda alone in LDS and read psi from its
+ * slot (1, the default), or stage both (0).  The gradients are the same bit for bit. */
+int    iq_circuit_set_lambda_tiles(void* c, int on);
 /* Per-launch timing (off by default; a profiling aid): while on, each launch of the tiled
  * sweeps (16 qubits and up) is bracketed by HIP events on the circuit's stream.  After an
  * expectation or gradient, iq_circuit_launch_times waits for the stream and returns the

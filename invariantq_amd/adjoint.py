@@ -123,6 +123,11 @@ class GpuCircuit:
         if binding._NATIVE_LIB.iq_circuit_set_stored_budget(self._ptr, nbytes) != 0:
             binding.check_error()
 
+    def set_lambda_tiles(self, on: bool) -> None:
+        """Reverse passes over stored outputs: lambda alone in LDS with psi read
+        from its slot (True, the default), or both staged (False)."""
+        binding._NATIVE_LIB.iq_circuit_set_lambda_tiles(self._ptr, 1 if on else 0)
+
     LAUNCH_KINDS = ("forward", "observable", "reverse", "two_qubit", "finalize")
 
     def set_timing(self, on: bool) -> None:

@@ -163,6 +163,7 @@ SIGNATURES = {
     "iq_circuit_stream": ([_vp], _vp),
     "iq_circuit_set_stored_budget": ([_vp, _sz], ctypes.c_int),
     "iq_circuit_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),
+    "iq_circuit_set_lambda_tiles": ([_vp, ctypes.c_int], ctypes.c_int),
     "iq_circuit_launch_times": ([_vp, ctypes.POINTER(ctypes.c_int), _d, _sz], _sz),
     "iq_batch_create": ([ctypes.c_int], _vp),
     "iq_batch_free": ([_vp], None),

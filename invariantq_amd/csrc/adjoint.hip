@@ -336,16 +336,17 @@ __device__ __forceinline__ void fwd_group(cv<R>* tile, uint32_t TS, int t0, cons
         for (int e = 0; e < 8; ++e) u[i][e] = (R)sg[i].u[e];
     for (uint32_t s = tid; s < cubes; s += T) {
         const uint32_t base = ((s & ~lo) << G) | (s & lo);
+        const uint32_t pb = lds_pos(base);  // lds_pos is linear over disjoint bits
         cv<R> x[D];
 #pragma unroll
-        for (uint32_t m = 0; m < D; ++m) x[m] = tile[lds_pos(base | (m << t0))];
+        for (uint32_t m = 0; m < D; ++m) x[m] = tile[(pb ^ lds_pos(m << t0))];
 #pragma unroll
         for (int i = 0; i < G; ++i)
 #pragma unroll
             for (uint32_t m = 0; m < D; ++m)
                 if (!(m & (1u << i))) mat2<R>(u[i], x[m], x[m | (1u << i)]);
 #pragma unroll
-        for (uint32_t m = 0; m < D; ++m) tile[lds_pos(base | (m << t0))] = x[m];
+        for (uint32_t m = 0; m < D; ++m) tile[(pb ^ lds_pos(m << t0))] = x[m];
     }
 }
 
@@ -360,21 +361,29 @@ __device__ __forceinline__ void fwd_group(cv<R>* tile, uint32_t TS, int t0, cons
 template <typename R, int G>
 __device__ __forceinline__ void sub_overlaps(const cv<R> (&x)[1 << G], const cv<R> (&y)[1 << G], double (&acc)[kGroupRuns][3]) {
     constexpr uint32_t D = 1u << G;
+    // T_Z of every run from one product per element, w_m = Im(conj(y_m) x_m):
+    // T_Z(i) = sum_m (-1)^(bit i of m) w_m = W - 2 (sum of w_m with bit i set)
+    R w[D], wall = 0;
+#pragma unroll
+    for (uint32_t m = 0; m < D; ++m) {
+        w[m] = y[m].x * x[m].y - y[m].y * x[m].x;
+        wall += w[m];
+    }
 #pragma unroll
     for (int i = 0; i < G; ++i) {
         const uint32_t bit = 1u << i;
-        R tx = 0, ty = 0, tz = 0;
+        R tx = 0, ty = 0, w1 = 0;
 #pragma unroll
         for (uint32_t m = 0; m < D; ++m) {
             if (m & bit) continue;
             const cv<R>&a0 = x[m], &a1 = x[m | bit], &b0 = y[m], &b1 = y[m | bit];
             tx += (b0.x * a1.y - b0.y * a1.x) + (b1.x * a0.y - b1.y * a0.x);
             ty += (b1.x * a0.x + b1.y * a0.y) - (b0.x * a1.x + b0.y * a1.y);
-            tz += (b0.x * a0.y - b0.y * a0.x) - (b1.x * a1.y - b1.y * a1.x);
+            w1 += w[m | bit];
         }
         acc[i][0] += (double)tx;
         acc[i][1] += (double)ty;
-        acc[i][2] += (double)tz;
+        acc[i][2] += (double)(wall - 2 * w1);
     }
 }
 
@@ -395,11 +404,12 @@ __device__ __forceinline__ void rev_group(cv<R>* tp, cv<R>* tl, uint32_t TS, int
     load_runs<R, G>(sg, u);
     for (uint32_t s = tid; s < cubes; s += T) {
         const uint32_t base = ((s & ~lo) << G) | (s & lo);
+        const uint32_t pb = lds_pos(base);  // lds_pos is linear over disjoint bits
         cv<R> x[D], y[D];
 #pragma unroll
         for (uint32_t m = 0; m < D; ++m) {
-            x[m] = tp[lds_pos(base | (m << t0))];
-            y[m] = tl[lds_pos(base | (m << t0))];
+            x[m] = tp[(pb ^ lds_pos(m << t0))];
+            y[m] = tl[(pb ^ lds_pos(m << t0))];
         }
         sub_overlaps<R, G>(x, y, acc);
 #pragma unroll
@@ -412,8 +422,8 @@ __device__ __forceinline__ void rev_group(cv<R>* tp, cv<R>* tl, uint32_t TS, int
             }
 #pragma unroll
         for (uint32_t m = 0; m < D; ++m) {
-            tp[lds_pos(base | (m << t0))] = x[m];
-            tl[lds_pos(base | (m << t0))] = y[m];
+            tp[(pb ^ lds_pos(m << t0))] = x[m];
+            tl[(pb ^ lds_pos(m << t0))] = y[m];
         }
     }
 }
@@ -426,11 +436,12 @@ __device__ __forceinline__ void rev_overlaps(const cv<R>* tp, const cv<R>* tl, u
     const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
     for (uint32_t s = tid; s < cubes; s += T) {
         const uint32_t base = ((s & ~lo) << G) | (s & lo);
+        const uint32_t pb = lds_pos(base);  // lds_pos is linear over disjoint bits
         cv<R> x[D], y[D];
 #pragma unroll
         for (uint32_t m = 0; m < D; ++m) {
-            x[m] = tp[lds_pos(base | (m << t0))];
-            y[m] = tl[lds_pos(base | (m << t0))];
+            x[m] = tp[(pb ^ lds_pos(m << t0))];
+            y[m] = tl[(pb ^ lds_pos(m << t0))];
         }
         sub_overlaps<R, G>(x, y, acc);
     }
@@ -445,16 +456,17 @@ __device__ __forceinline__ void lam_back(cv<R>* tl, uint32_t TS, int t0, const P
     load_runs<R, G>(sg, u);
     for (uint32_t s = tid; s < cubes; s += T) {
         const uint32_t base = ((s & ~lo) << G) | (s & lo);
+        const uint32_t pb = lds_pos(base);  // lds_pos is linear over disjoint bits
         cv<R> y[D];
 #pragma unroll
-        for (uint32_t m = 0; m < D; ++m) y[m] = tl[lds_pos(base | (m << t0))];
+        for (uint32_t m = 0; m < D; ++m) y[m] = tl[(pb ^ lds_pos(m << t0))];
 #pragma unroll
         for (int i = 0; i < G; ++i)
 #pragma unroll
             for (uint32_t m = 0; m < D; ++m)
                 if (!(m & (1u << i))) mat2<R>(u[i], y[m], y[m | (1u << i)]);
 #pragma unroll
-        for (uint32_t m = 0; m < D; ++m) tl[lds_pos(base | (m << t0))] = y[m];
+        for (uint32_t m = 0; m < D; ++m) tl[(pb ^ lds_pos(m << t0))] = y[m];
     }
 }
 
@@ -495,19 +507,30 @@ __global__ void __launch_bounds__(512) k_tile_fwd(const cv<R>* __restrict__ src,
 }
 
 
+// Wave sums of a group's overlaps (up to 4 runs x 3 values): a
+// reduce-scatter butterfly over the 12 values.  At offset 32 each lane keeps
+// one half (6 values) and adds the partner lane's copy of it, at offset 16
+// one half of those (3); each of the 3 is then summed over offsets 8..1 and
+// added into the wave's slot by the lane with bits 3..0 clear.  6 + 3 + 12
+// shuffles per group instead of 6 per value.
 template <typename R>
 __device__ __forceinline__ void flush_overlaps(const double (&acc)[kGroupRuns][3], int gs, double* wred, int wave, int lane,
                                                int first) {
+    double v[6];
+    const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0;
 #pragma unroll
-    for (int i = 0; i < kGroupRuns; ++i) {
-        if (i >= gs) continue;
+    for (int i = 0; i < 6; ++i) {
+        const double a = acc[i / 3][i % 3], b = acc[2 + i / 3][i % 3];  // values i and 6 + i
+        v[i] = (h5 ? b : a) + __shfl_xor(h5 ? a : b, 32, 64);
+    }
 #pragma unroll
-        for (int st = 0; st < 3; ++st) {
-            double x = acc[i][st];
+    for (int i = 0; i < 3; ++i) {
+        const double a = v[i], b = v[3 + i];
+        double x = (h4 ? b : a) + __shfl_xor(h4 ? a : b, 16, 64);
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-            if (lane == 0) wred[(wave * kPassSegMax + first + i) * kSubMax + st] += x;
-        }
+        for (int off = 8; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        const int k = (h5 ? 6 : 0) + (h4 ? 3 : 0) + i;  // value k = 3 run + component
+        if ((lane & 15) == 0 && k < 3 * gs) wred[(wave * kPassSegMax + first + k / 3) * kSubMax + k % 3] += x;
     }
 }
 
@@ -576,6 +599,125 @@ __global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>
             }
         }
         if (!STORED) tile_out<R>(d, t, tp, psi, TS);
+        tile_out<R>(d, t, tl, lam, TS);
+        __syncthreads();
+    }
+    for (int i = tid; i < d.nseg * kSubMax; i += T) {
+        const int k = i / kSubMax, st = i % kSubMax;
+        double v = 0.0;
+        for (int w = 0; w < nw; ++w) v += wred[(w * kPassSegMax + k) * kSubMax + st];
+        gpart[((uint64_t)ps[k].slot * nblk + blockIdx.x) * kSubMax + st] = v;
+    }
+}
+
+// Lambda's tile in with the pass's CZ signs, which also go into a bitmap
+// (bit j of word j / 32 set when element j's sign is odd) for psi's reads:
+// the 64 lanes of a wave hold 64 consecutive elements, so one ballot gives
+// two words.
+template <typename R>
+__device__ __forceinline__ void tile_in_signs(const PassDesc& d, uint64_t t, const cv<R>* __restrict__ src,
+                                              cv<R>* dst, uint32_t* bits, uint32_t TS, CzTile z) {
+    const uint32_t T = blockDim.x, tid = threadIdx.x, lane = tid & 63u;
+    for (uint32_t j0 = tid; j0 < TS; j0 += 16 * T) {
+        cv<R> v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = src[pass_global(d, t, min(j0 + u * T, TS - 1))];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const uint32_t j = j0 + u * T;
+            const bool odd = cz_odd(d, z, j);
+            v[u] = negate_if<R>(v[u], odd);
+            const uint64_t b = __ballot(odd);
+            if (lane == 0 && j < TS) {
+                bits[j >> 5] = (uint32_t)b;
+                bits[(j >> 5) + 1] = (uint32_t)(b >> 32);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (j0 + u * T < TS) dst[lds_pos(j0 + u * T)] = v[u];
+    }
+}
+
+// Overlaps of a group's runs with psi's sub-cubes read from global memory
+// (the stored pass output, taking the pass's CZ signs from the bitmap) and
+// lambda's from LDS.  The tile's low r >= 4 bits are contiguous global bits,
+// so sixteen consecutive threads read 128 contiguous bytes or more.
+template <typename R, int G>
+__device__ __forceinline__ void rev_overlaps_global(const PassDesc& d, uint64_t t, const uint32_t* sbits,
+                                                    const cv<R>* __restrict__ psi, const cv<R>* tl, uint32_t TS,
+                                                    int t0, double (&acc)[kGroupRuns][3]) {
+    constexpr uint32_t D = 1u << G;
+    const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
+    const cv<R>* pt = psi + pass_global(d, t, 0);  // the tile number's bits
+    const uint32_t rmask = (1u << d.r) - 1, mmask = (1u << d.m) - 1;
+    // global offset of tile element j within the tile: linear over disjoint
+    // bits, so a cube's offsets are its base's plus a per-m constant
+    auto local = [&](uint32_t j) { return (uint64_t)(j & rmask) | ((uint64_t)((j >> d.r) & mmask) << d.g0); };
+    for (uint32_t s = tid; s < cubes; s += T) {
+        const uint32_t base = ((s & ~lo) << G) | (s & lo);
+        const uint32_t pb = lds_pos(base);  // lds_pos is linear over disjoint bits
+        const cv<R>* pc = pt + local(base);
+        cv<R> x[D], y[D];
+#pragma unroll
+        for (uint32_t m = 0; m < D; ++m) x[m] = pc[local(m << t0)];
+#pragma unroll
+        for (uint32_t m = 0; m < D; ++m) y[m] = tl[(pb ^ lds_pos(m << t0))];
+        if (d.ncz) {
+#pragma unroll
+            for (uint32_t m = 0; m < D; ++m) {
+                const uint32_t j = base | (m << t0);
+                x[m] = negate_if<R>(x[m], (sbits[j >> 5] >> (j & 31u)) & 1u);
+            }
+        }
+        sub_overlaps<R, G>(x, y, acc);
+    }
+}
+
+// Reverse of a forward pass whose output psi was stored and whose groups sit
+// on distinct qubits (no group undoes an earlier one): lambda alone is staged
+// in LDS, half the footprint of staging psi too, so two workgroups share a
+// compute unit and one's loads overlap the other's arithmetic.  Every
+// group's overlaps are taken at the pass's end (psi as stored, lambda as
+// loaded; the runs commute), then lambda goes back through the groups.
+// Partials as k_tile_rev.
+template <typename R>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_tile_rev_lam(const cv<R>* __restrict__ psi, cv<R>* __restrict__ lam,
+                                                      PassDesc d, const PassSeg* __restrict__ segs,
+                                                      double* __restrict__ gpart, int nblk) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t T = blockDim.x, tid = threadIdx.x, TS = 1u << d.ts_bits;
+    const int lane = tid & 63, wave = tid >> 6, nw = T >> 6;
+    cv<R>* tl = reinterpret_cast<cv<R>*>(smem);
+    double* wred = reinterpret_cast<double*>(tl + TS);  // [nw][kPassSegMax][kSubMax]
+    uint32_t* sbits = reinterpret_cast<uint32_t*>(wred + nw * kPassSegMax * kSubMax);  // [TS / 32]
+    for (int i = tid; i < nw * kPassSegMax * kSubMax; i += T) wred[i] = 0.0;
+    __syncthreads();
+    const PassSeg* ps = segs + d.first;
+    for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        if (d.ncz) tile_in_signs<R>(d, t, lam, tl, sbits, TS, cz_setup(d, t));
+        else tile_in<R>(d, t, lam, tl, TS);
+        __syncthreads();
+        for (int gi = d.ngroups - 1; gi >= 0; --gi) {
+            // a wave-uniform index: the descriptor reads below become scalar loads
+            const int g = __builtin_amdgcn_readfirstlane(gi);
+            const int t0 = d.gt0[g], gs = d.gsize[g];
+            double acc[kGroupRuns][3] = {};
+#define IQ_CALL(G) rev_overlaps_global<R, G>(d, t, sbits, psi, tl, TS, t0, acc)
+            IQ_GROUP_SWITCH(gs, IQ_CALL)
+#undef IQ_CALL
+            flush_overlaps<R>(acc, gs, wred, wave, lane, d.gfirst[g]);
+        }
+        __syncthreads();
+        for (int gi = d.ngroups - 1; gi >= 0; --gi) {
+            const int g = __builtin_amdgcn_readfirstlane(gi);
+            const int t0 = d.gt0[g], gs = d.gsize[g];
+            const PassSeg* sg = ps + d.gfirst[g];
+#define IQ_CALL(G) lam_back<R, G>(tl, TS, t0, sg)
+            IQ_GROUP_SWITCH(gs, IQ_CALL)
+#undef IQ_CALL
+            __syncthreads();
+        }
         tile_out<R>(d, t, tl, lam, TS);
         __syncthreads();
     }
@@ -1038,6 +1180,9 @@ struct Circuit {
     size_t stored_cap = 0;   // bytes
     size_t stored_budget = size_t(16) << 30;
     bool stored_valid = false;  // the last forward sweep filled them
+    // Reverse passes over stored outputs stage lambda alone in LDS and read
+    // psi from its slot (true), or stage both (false); the same sums either way.
+    bool rev_lam_only = true;
     ObsPlan oplan;  // the observable over tile passes (16 to 34 qubits)
     // Per-launch timing (off by default): each launch of an evaluation is
     // bracketed by two events on the circuit's stream; kinds are kLaunch*.
@@ -1459,8 +1604,9 @@ PassDesc pass_desc(const TileGeom& g, int n, int k) {
         d.g0 = d.r;
     } else {
         const int P = high_passes(g, n), total = n - g.b, base = total / P, extra = total % P;
-        d.m = base + (k - 1 < extra ? 1 : 0);
-        d.g0 = g.b + (k - 1) * base + std::min(k - 1, extra);
+        const int kk = P + 1 - k;  // pass 1 takes the highest qubits
+        d.m = base + (kk - 1 < extra ? 1 : 0);
+        d.g0 = g.b + (kk - 1) * base + std::min(kk - 1, extra);
         d.r = g.b - d.m;
     }
     d.ts_bits = d.r + d.m;
@@ -1472,7 +1618,7 @@ int pass_of(const TileGeom& g, int n, int q) {
     if (q < g.b) return 0;
     for (int k = 1;; ++k) {
         const PassDesc d = pass_desc(g, n, k);
-        if (q < d.g0 + d.m) return k;
+        if (q >= d.g0 && q < d.g0 + d.m) return k;
     }
 }
 
@@ -1553,7 +1699,7 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
         }
     }
     if (!cur.runs.empty() || cur.cz >= 0) stages.push_back(cur);
-    const int npass = pass_of(g, c.n, c.n - 1) + 1;
+    const int npass = high_passes(g, c.n) + 1;
     auto fill = [&](const Segment& sg, const PassDesc& d, int k) {
         PassSeg ps{};
         ps.tbit = tile_bit(g, d, k, sg.q);
@@ -1674,7 +1820,7 @@ bool plan_observable_tiles(ObsPlan& p, int n, int prec, uint64_t dim, const std:
                            const std::vector<int>& start, const std::vector<uint64_t>& zy, const std::vector<int>& ny,
                            const std::vector<double>& co) {
     const TileGeom g = tile_geom(prec);
-    const int npass = pass_of(g, n, n - 1) + 1;
+    const int npass = high_passes(g, n) + 1;
     std::vector<PassDesc> pd(npass);
     for (int k = 0; k < npass; ++k) pd[k] = pass_desc(g, n, k);
     auto split = [&](const PassDesc& d, uint64_t m, uint32_t& tile, uint64_t& fixed) {
@@ -1994,7 +2140,18 @@ bool run_reverse_tiled(Circuit& c, const double* params) {
             if (!launch_two_qubit(c, c.segs[l.seg2], true, st ? nullptr : e.amp, c.lam)) return false;
             continue;
         }
-        const size_t lds = 2 * ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>) + (size_t)8 * kPassSegMax * kSubMax * sizeof(double);
+        const size_t wred_bytes = (size_t)8 * kPassSegMax * kSubMax * sizeof(double);
+        if (st && l.d.gundo == 0 && c.rev_lam_only) {
+            // psi read from its stored slot, lambda alone in LDS
+            const size_t lds = ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>) + wred_bytes + ((size_t)1 << l.d.ts_bits) / 8;
+            hipFuncSetAttribute((const void*)k_tile_rev_lam<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            TimedLaunch tl(c, kLaunchReverse);
+            hipLaunchKernelGGL(k_tile_rev_lam<R>, dim3(c.nblk), dim3(512), lds, e.stream,
+                               (const cv<R>*)pass_out(c, st, i, nl), (cv<R>*)c.lam, l.d, dsegs, c.gpart, (int)c.nblk);
+            if (!hip_ok(hipGetLastError(), "tiled reverse")) return false;
+            continue;
+        }
+        const size_t lds = 2 * ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>) + wred_bytes;
         auto kern = st ? k_tile_rev<R, true> : k_tile_rev<R, false>;
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         TimedLaunch tl(c, kLaunchReverse);
@@ -2072,7 +2229,7 @@ bool pauli_sum_tiled(Engine& e, size_t nterms, const PauliTerm* terms, const dou
     size_t mask_passes = 0;  // of the per-mask reduction, eight strings per pass
     for (size_t gi = 0; gi + 1 < start.size(); ++gi) mask_passes += (size_t)(start[gi + 1] - start[gi] + 7) / 8;
     const TileGeom g = tile_geom(e.prec);
-    const int npass = pass_of(g, e.n, e.n - 1) + 1;
+    const int npass = high_passes(g, e.n) + 1;
     if (mask_passes <= (size_t)npass) return true;
     ObsPlan plan;
     if (!plan_observable_tiles(plan, e.n, e.prec, e.dim, xm, start, zy, ny, co)) return false;
@@ -2248,6 +2405,12 @@ double iq_circuit_expectation(void* cp, const double* params, size_t nparams) {
     double e = 0.0;
     if (ok) c.eng.fetch(1, &e);
     return e;
+}
+
+int iq_circuit_set_lambda_tiles(void* cp, int on) {
+    if (!cp) return -1;
+    circ_of(cp)->rev_lam_only = on != 0;
+    return 0;
 }
 
 int iq_circuit_set_timing(void* cp, int on) {

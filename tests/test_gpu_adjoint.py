@@ -235,3 +235,25 @@ def test_full_size_24q_adjoint_vs_fp64_parameter_shift(A):
     assert np.max(np.abs(g128[which] - ps)) < 1e-10
     assert abs(e64 - e128) < 1e-4
     assert np.linalg.norm(g64 - g128) / np.linalg.norm(g128) < 1e-4
+
+
+@pytest.mark.parametrize("precision,n", [("complex64", 24), ("complex128", 20), ("complex64", 17)])
+def test_lambda_only_reverse_tiles_bit_identical(A, precision, n):
+    """Reverse passes over stored outputs with lambda alone in LDS (psi read
+    from its slot) against both staged: the same sums in the same order, so
+    the same gradient bit for bit; and timed launches cover every pass."""
+    layers = 4
+    rng = np.random.default_rng(n)
+    p = rng.uniform(0, 2 * np.pi, 2 * n * layers)
+    c = _hea(A, n, layers, precision)
+    print(f"lambda-only reverse tiles n={n} {precision}", flush=True)
+    c.set_lambda_tiles(False)
+    e0, g0 = c.gradient(p)
+    c.set_lambda_tiles(True)
+    c.set_timing(True)
+    e1, g1 = c.gradient(p)
+    kinds = [k for k, _ in c.launch_times()]
+    c.set_timing(False)
+    assert e0 == e1 and np.array_equal(g0, g1)
+    assert kinds.count("reverse") == kinds.count("forward") > 0
+    assert kinds.count("observable") == 1 and kinds.count("finalize") == 1
