@@ -220,9 +220,10 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             const uint32_t cubes = TS >> G, lo = (1u << t0) - 1;
             for (uint32_t s = tid; s < cubes; s += T) {
                 const uint32_t base = ((s & ~lo) << G) | (s & lo);
+                const uint32_t pb0 = P(base);  // P is linear over disjoint bits
                 cv<R> x[D];
 #pragma unroll
-                for (uint32_t m = 0; m < D; ++m) x[m] = tile[P(base | (m << t0))];
+                for (uint32_t m = 0; m < D; ++m) x[m] = tile[pb0 ^ P(m << t0)];
 #pragma unroll
                 for (int jb = 0; jb < G; ++jb) {
                     const R* u = RT + (n * l + (t0 + jb + qoff)) * 24;
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                         if (!(m & (1u << jb))) mul2<R>(u, x[m].x, x[m].y, x[m | (1u << jb)].x, x[m | (1u << jb)].y);
                 }
 #pragma unroll
-                for (uint32_t m = 0; m < D; ++m) tile[P(base | (m << t0))] = x[m];
+                for (uint32_t m = 0; m < D; ++m) tile[pb0 ^ P(m << t0)] = x[m];
             }
             __syncthreads();
         };
@@ -276,45 +277,77 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             for (int jb = 0; jb < G; ++jb) az[jb] = aa[jb] = ab[jb] = 0.0;
             for (uint32_t s = tid; s < cubes; s += T) {
                 const uint32_t base = ((s & ~lo) << G) | (s & lo);
+                const uint32_t pb0 = P(base);  // P is linear over disjoint bits
                 cv<R> x[D], y[D];
 #pragma unroll
                 for (uint32_t m = 0; m < D; ++m) {
-                    x[m] = tp[P(base | (m << t0))];
-                    y[m] = tl[P(base | (m << t0))];
+                    x[m] = tp[pb0 ^ P(m << t0)];
+                    y[m] = tl[pb0 ^ P(m << t0)];
+                }
+                // every qubit's products from the same registers (rotations on
+                // the group's other qubits, applied to both, leave them
+                // unchanged); Im<lambda|Z psi> from one product per element:
+                // sum over bit = 0 minus bit = 1 of w_m = Im(conj(l_m) p_m)
+                R w[D], wall = 0;
+#pragma unroll
+                for (uint32_t m = 0; m < D; ++m) {
+                    w[m] = y[m].x * x[m].y - y[m].y * x[m].x;
+                    wall += w[m];
+                }
+#pragma unroll
+                for (int jb = G - 1; jb >= 0; --jb) {
+                    const uint32_t bit = 1u << jb;
+                    R w1 = 0, pa = 0, pb = 0;
+#pragma unroll
+                    for (uint32_t m = 0; m < D; ++m) {
+                        if (m & bit) continue;
+                        const cv<R>&a0 = x[m], &a1 = x[m | bit], &b0 = y[m], &b1 = y[m | bit];
+                        w1 += w[m | bit];
+                        pa += (b1.x * a0.x + b1.y * a0.y) - (b0.x * a1.x + b0.y * a1.y);
+                        pb += (b1.x * a0.y - b1.y * a0.x) + (b0.x * a1.y - b0.y * a1.x);
+                    }
+                    az[jb] += (double)(wall - 2 * w1);
+                    aa[jb] += (double)pa;
+                    ab[jb] += (double)pb;
                 }
 #pragma unroll
                 for (int jb = G - 1; jb >= 0; --jb) {
                     const R* inv = RT + (n * l + (t0 + jb + qoff)) * 24 + 8;
                     const uint32_t bit = 1u << jb;
-                    R pz = 0, pa = 0, pb = 0;
 #pragma unroll
                     for (uint32_t m = 0; m < D; ++m) {
                         if (m & bit) continue;
-                        cv<R>&a0 = x[m], &a1 = x[m | bit], &b0 = y[m], &b1 = y[m | bit];
-                        pz += (b0.x * a0.y - b0.y * a0.x) - (b1.x * a1.y - b1.y * a1.x);
-                        pa += (b1.x * a0.x + b1.y * a0.y) - (b0.x * a1.x + b0.y * a1.y);
-                        pb += (b1.x * a0.y - b1.y * a0.x) + (b0.x * a1.y - b0.y * a1.x);
-                        mul2<R>(inv, a0.x, a0.y, a1.x, a1.y);
-                        mul2<R>(inv, b0.x, b0.y, b1.x, b1.y);
+                        mul2<R>(inv, x[m].x, x[m].y, x[m | bit].x, x[m | bit].y);
+                        mul2<R>(inv, y[m].x, y[m].y, y[m | bit].x, y[m | bit].y);
                     }
-                    az[jb] += (double)pz;
-                    aa[jb] += (double)pa;
-                    ab[jb] += (double)pb;
                 }
 #pragma unroll
                 for (uint32_t m = 0; m < D; ++m) {
-                    tp[P(base | (m << t0))] = x[m];
-                    tl[P(base | (m << t0))] = y[m];
+                    tp[pb0 ^ P(m << t0)] = x[m];
+                    tl[pb0 ^ P(m << t0)] = y[m];
                 }
             }
+            // wave sums of the 3G products (qubit jb: z, a, b at 3 jb + 0..2),
+            // reduce-scatter: offsets 32 and 16 halve the values a lane
+            // carries (12 -> 6 -> 3), offsets 8..1 finish each of the 3
+            {
+                double acc[12];
 #pragma unroll
-            for (int jb = 0; jb < G; ++jb) {
-                const double z = wave_sum(az[jb]), va = wave_sum(aa[jb]), vb = wave_sum(ab[jb]);
-                if (lane == 0) {
-                    double* w = wred + wave * kMaxSlots + 3 * (t0 + jb + qoff);
-                    w[0] += z;
-                    w[1] += va;
-                    w[2] += vb;
+                for (int k = 0; k < 12; ++k) {
+                    const int jb = k / 3;
+                    acc[k] = jb >= G ? 0.0 : (k % 3 == 0 ? az[jb < G ? jb : 0] : k % 3 == 1 ? aa[jb < G ? jb : 0] : ab[jb < G ? jb : 0]);
+                }
+                const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0;
+                double v6[6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) v6[i] = (h5 ? acc[6 + i] : acc[i]) + __shfl_xor(h5 ? acc[i] : acc[6 + i], 32, 64);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    double xs = (h4 ? v6[3 + i] : v6[i]) + __shfl_xor(h4 ? v6[i] : v6[3 + i], 16, 64);
+#pragma unroll
+                    for (int off = 8; off > 0; off >>= 1) xs += __shfl_xor(xs, off, 64);
+                    const int k = (h5 ? 6 : 0) + (h4 ? 3 : 0) + i;
+                    if ((lane & 15) == 0 && k < 3 * G) wred[wave * kMaxSlots + 3 * (t0 + qoff) + k] += xs;
                 }
             }
             __syncthreads();
