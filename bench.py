@@ -738,9 +738,10 @@ def batch_workload(torch, count: int, cpu: bool):
     t0 = time.perf_counter()
     tune = batch.autotune(runner, specs, cache=None, per_n=128)
     t_tune = time.perf_counter() - t0
-    runner.run(specs[:64], tile_bits=tune["tile_bits"], block_threads=tune["block_threads"])  # warm-up
+    gw = tune["group_width"]
+    runner.run(specs[:64], tile_bits=tune["tile_bits"], block_threads=tune["block_threads"], group_width=gw)  # warm-up
     t0 = time.perf_counter()
-    e, g, ms = runner.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"])
+    e, g, ms = runner.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"], group_width=gw)
     wall = time.perf_counter() - t0
     nparams = int(sum(2 * n * l for n, l, _ in specs))
     gate_pass_bytes = sum(3 * n * l * (2 + 4) * (1 << n) * 8 for n, l, _ in specs)
@@ -760,7 +761,8 @@ def batch_workload(torch, count: int, cpu: bool):
            "circuits_per_s": round(count / (ms * 1e-3), 1),
            "gradients_per_s": round(nparams / (ms * 1e-3), 1),
            "gate_pass_equivalent_TBps": round(gate_pass_bytes / (ms * 1e-3) / 1e12, 3),
-           "autotune": {"tile_bits": tune["tile_bits"], "block_threads": tune["block_threads"],
+           "autotune": {"tile_bits": tune["tile_bits"], "block_threads": tune["block_threads"], "group_width": gw,
+                        "group_width_ms": tune["timings"].get("group_width"),
                         "seconds": round(t_tune, 2)},
            "max_rel_grad_err_vs_complex128": max(errs), "checked_circuits": [specs[i][0] for i in check]}
     # per size class: each class launched alone.  Circuits that fit the tile
@@ -771,7 +773,7 @@ def batch_workload(torch, count: int, cpu: bool):
     classes = {}
     for n in sorted({s[0] for s in specs}):
         cls = [s for s in specs if s[0] == n]
-        _, _, cms = runner.run(cls, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"])
+        _, _, cms = runner.run(cls, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"], group_width=gw)
         b, N = tune["tile_bits"][n], 1 << n
         L = sum(s[1] for s in cls)
         if n <= b:

@@ -246,6 +246,10 @@ size_t iq_circuit_launch_times(void* c, int* kinds, double* ms, size_t cap);
 void*  iq_batch_create(int precision);
 void   iq_batch_free(void* b);
 int    iq_batch_max_tile_bits(int precision);
+/* Qubits fused per register pass of the tile sweeps (2..4 for complex64, 2..3 for complex128;
+ * 0 restores the default, the largest), chosen per batch by the autotuner. */
+int    iq_batch_max_group_width(int precision);
+int    iq_batch_set_group_width(void* b, int width);
 /* params: 2*nq[c]*layers[c] values per circuit, concatenated; grads_out has the same
  * layout.  tile_bits[c] (0 = widest) and block_threads (0 = 512) come from the
  * autotuner; num_blocks 0 = one workgroup per compute unit. */

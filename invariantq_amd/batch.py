@@ -3,9 +3,11 @@
 ``BatchRunner.run`` sends thousands of ansatz circuits (RY, RZ on every qubit
 and a CZ ring per layer; open-chain TFIM observable) of uneven qubit count and
 depth to the GPU kernel ``k_batch_hea``: energies and complete adjoint
-gradients come back in one launch.  ``autotune`` chooses the LDS tile width
-per qubit count and the workgroup size by timing candidates on a sample of
-the batch, and caches the choice per (device, precision) in a JSON file.
+gradients come back in one launch.  ``autotune`` chooses, by timing
+candidates on a sample of the batch, the LDS tile width per qubit count
+(amplitudes per workgroup), the workgroup size, and the fusion width (qubits
+whose rotations are fused into one register pass of the tile sweeps), and
+caches the choice per (device, precision) in a JSON file.
 
 No reference counterpart: the reference evaluates one circuit per simulator
 call (invariantq/benchmarks/vqe.py:44-62).
@@ -45,6 +47,7 @@ class BatchRunner:
             binding.check_error()
             raise MemoryError("could not create the batch context")
         self.max_tile_bits = L.iq_batch_max_tile_bits(self.prec_code)
+        self.max_group_width = L.iq_batch_max_group_width(self.prec_code)
 
     def __del__(self):
         if getattr(self, "_ptr", None):
@@ -52,8 +55,12 @@ class BatchRunner:
             self._ptr = None
 
     def run(self, specs: Sequence[Tuple[int, int, np.ndarray]], hx: float = 0.5,
-            tile_bits: Optional[Dict[int, int]] = None, block_threads: int = 0, num_blocks: int = 0):
-        """specs: (n, layers, params) per circuit.  Returns (energies, [grad per circuit], kernel ms)."""
+            tile_bits: Optional[Dict[int, int]] = None, block_threads: int = 0, num_blocks: int = 0,
+            group_width: int = 0):
+        """specs: (n, layers, params) per circuit; group_width 0 = the widest.
+        Returns (energies, [grad per circuit], kernel ms)."""
+        if binding._NATIVE_LIB.iq_batch_set_group_width(self._ptr, int(group_width)) != 0:
+            binding.check_error()
         nq = np.ascontiguousarray([s[0] for s in specs], dtype=np.int32)
         ly = np.ascontiguousarray([s[1] for s in specs], dtype=np.int32)
         sizes = 2 * nq.astype(np.int64) * ly
@@ -73,43 +80,61 @@ class BatchRunner:
 
 
 def _device_key(precision: str) -> str:
-    try:
-        import torch
-        name = torch.cuda.get_device_properties(0).gcnArchName
-    except Exception:  # the cache key only needs to be stable per machine
-        name = "gfx950"
+    """Cache key: the library's backend (its one target architecture) and the precision."""
+    name = binding.lib().iq_backend_name()
+    name = name.decode() if isinstance(name, bytes) else str(name)
     return f"{name}:{precision}"
+
+
+def _load_table(cache: Optional[Path]) -> Dict:
+    if cache is None or not Path(cache).exists():
+        return {}
+    try:
+        table = json.loads(Path(cache).read_text())
+        return table if isinstance(table, dict) else {}
+    except (OSError, ValueError):
+        return {}
 
 
 def autotune(runner: BatchRunner, specs: Sequence[Tuple[int, int, np.ndarray]], hx: float = 0.5,
              cache: Optional[Path] = DEFAULT_CACHE, per_n: int = 256, force: bool = False) -> Dict:
-    """Choose the tile width for every qubit count present in `specs` and the
-    workgroup size, by timing candidates on up to `per_n` circuits of each count.
+    """Choose the tile width for every qubit count present in `specs`, the
+    workgroup size and the fusion width, by timing candidates on up to `per_n`
+    circuits of each count.
 
-    Candidates: tile width in {max, max-1, max-2} (circuits wider than the tile
-    take the two-pass tiled path; a width >= n keeps the circuit in LDS) and
-    256 or 512 threads per workgroup.  Returns {"tile_bits": {n: width},
-    "block_threads": t, "timings": ...} and stores it in `cache`.
+    Stage 1 (default fusion width): tile width in {max, max-1, max-2} (circuits
+    wider than the tile take the two-pass tiled path; a width >= n keeps the
+    circuit in LDS; widths below n/2 do not fit) x 256 or 512 threads per
+    workgroup; the fastest width per n and the thread count fastest overall.
+    Stage 2: with those, the fusion width (qubits per register pass of the
+    tile sweeps, 2 up to the precision's widest) timed over the whole sample.
+    Returns {"tile_bits": {n: width}, "block_threads": t, "group_width": g,
+    "timings": ..., "cached": bool}.  The cache keeps one entry per (device,
+    precision): tile widths of qubit counts tuned earlier are kept and new
+    ones merged in; the thread count and fusion width are those of the
+    latest tuning.
     """
     key = _device_key(runner.precision)
-    table = {}
-    if cache is not None and Path(cache).exists():
-        try:
-            table = json.loads(Path(cache).read_text())
-        except (OSError, ValueError):
-            table = {}
+    table = _load_table(cache)
     ns = sorted({int(s[0]) for s in specs})
-    entry = table.get(key)
-    if entry and not force and all(str(n) in entry["tile_bits"] for n in ns):
-        return {"tile_bits": {int(k): v for k, v in entry["tile_bits"].items()},
-                "block_threads": entry["block_threads"], "timings": entry.get("timings", {}), "cached": True}
+    entry = table.get(key) or {}
+    known = entry.get("tile_bits", {})
+    if (entry and not force and all(str(n) in known for n in ns) and "group_width" in entry
+            and "block_threads" in entry):
+        return {"tile_bits": {int(k): v for k, v in known.items()}, "block_threads": entry["block_threads"],
+                "group_width": entry["group_width"], "timings": entry.get("timings", {}), "cached": True}
     bmax = runner.max_tile_bits
     timings: Dict[str, Dict[str, float]] = {}
     best: Dict[int, int] = {}
     per_threads = {256: 0.0, 512: 0.0}
+    samples = {}
     for n in ns:
         sample = [s for s in specs if int(s[0]) == n][:per_n]
+        samples[n] = sample
         cands = sorted({w for w in (bmax, bmax - 1, bmax - 2) if 2 * w >= n})
+        if not cands:
+            raise ValueError(f"no tile width fits {n} qubits: the tiled path needs 2 * width >= n, "
+                             f"and the widest tile holds {bmax} qubits")
         res = {}
         for w in cands:
             for t in (256, 512):
@@ -122,12 +147,22 @@ def autotune(runner: BatchRunner, specs: Sequence[Tuple[int, int, np.ndarray]], 
         for t in per_threads:
             per_threads[t] += res[f"{w_best}/{t}"]
     threads = min(per_threads, key=per_threads.get)
-    out = {"tile_bits": best, "block_threads": threads, "timings": timings, "cached": False}
+    everything = [s for n in ns for s in samples[n]]
+    gw_ms = {}
+    for g in range(2, runner.max_group_width + 1):
+        runner.run(everything[:8], hx, best, threads, group_width=g)  # warm-up
+        _, _, gw_ms[g] = runner.run(everything, hx, best, threads, group_width=g)
+    group_width = min(gw_ms, key=gw_ms.get)
+    timings["group_width"] = {str(g): ms for g, ms in gw_ms.items()}
+    out = {"tile_bits": best, "block_threads": threads, "group_width": group_width, "timings": timings,
+           "cached": False}
     if cache is not None:
         try:
-            Path(cache).parent.mkdir(parents=True, exist_ok=True)
-            table[key] = {"tile_bits": {str(k): v for k, v in best.items()}, "block_threads": threads,
+            merged = {str(k): v for k, v in known.items()}
+            merged.update({str(k): v for k, v in best.items()})
+            table[key] = {"tile_bits": merged, "block_threads": threads, "group_width": group_width,
                           "timings": timings}
+            Path(cache).parent.mkdir(parents=True, exist_ok=True)
             Path(cache).write_text(json.dumps(table, indent=1))
         except OSError:
             pass

@@ -168,6 +168,8 @@ SIGNATURES = {
     "iq_batch_create": ([ctypes.c_int], _vp),
     "iq_batch_free": ([_vp], None),
     "iq_batch_max_tile_bits": ([ctypes.c_int], ctypes.c_int),
+    "iq_batch_max_group_width": ([ctypes.c_int], ctypes.c_int),
+    "iq_batch_set_group_width": ([_vp, ctypes.c_int], ctypes.c_int),
     "iq_batch_hea_tfim": ([_vp, _sz, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _d, ctypes.c_double,
                            ctypes.POINTER(ctypes.c_int32), ctypes.c_int, ctypes.c_int, _d, _d, _d], ctypes.c_int),
 }

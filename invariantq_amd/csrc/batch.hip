@@ -114,7 +114,9 @@ __device__ __forceinline__ bool cz_ring_sign(uint64_t g, int n) {
     return par != 0;
 }
 
-template <typename R>
+// GW: qubits per register group (the fusion width of the tile sweeps): 2..4
+// for complex64, 2..3 for complex128.
+template <typename R, int GW>
 __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restrict__ circs, int ncirc,
                                                    const R* __restrict__ rots, const double* __restrict__ params,
                                                    double* __restrict__ grads,
@@ -150,11 +152,10 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         // four, so sixteen lanes reading sixteen elements of a register group
         // hit distinct 8-byte slots whichever tile bits the group holds
         auto P = [](uint32_t j) __attribute__((always_inline)) -> uint32_t { return j ^ ((j >> 4) & 15u); };
-        // qubits per register group: forward sweeps 4 (16 elements per
-        // sub-cube) for complex64, 3 for complex128; reverse sweeps hold psi and
-        // lambda next to the prefetched tile, so one fewer
-        constexpr int GM = sizeof(R) == 4 ? 4 : 3;
-        constexpr int GA = GM;
+        // qubits per register group, forward and reverse (sub-cubes of 2^GW
+        // elements; the default is 4 for complex64 and 3 for complex128)
+        constexpr int GM = GW;
+        constexpr int GA = GW;
         // --- tile transfer helpers: eight elements per thread in flight before
         // any is written on, so a tile costs about one memory latency, not
         // one per element
@@ -249,12 +250,12 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         };
         // dispatch on the group width; callers pass 1 <= g <= GM
         auto rot_group = [&](cv<R>* tile, int t0, int g, int qoff, int l) __attribute__((always_inline)) {
-            static_assert(GM >= 3 && GM <= 4, "group width");
+            static_assert(GM >= 2 && GM <= 4, "group width");
             switch (g) {
                 case 1: rot_g(std::integral_constant<int, 1>{}, tile, t0, qoff, l); break;
                 case 2: rot_g(std::integral_constant<int, 2>{}, tile, t0, qoff, l); break;
-                case 3: rot_g(std::integral_constant<int, 3>{}, tile, t0, qoff, l); break;
-                case 4: rot_g(std::integral_constant<int, GM>{}, tile, t0, qoff, l); break;
+                case 3: if constexpr (GM >= 3) rot_g(std::integral_constant<int, 3>{}, tile, t0, qoff, l); break;
+                case 4: if constexpr (GM >= 4) rot_g(std::integral_constant<int, 4>{}, tile, t0, qoff, l); break;
                 default: break;
             }
         };
@@ -354,12 +355,12 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         };
         // dispatch on the group width; callers pass 1 <= g <= GA
         auto adj_group = [&](int t0, int g, int qoff, int l) __attribute__((always_inline)) {
-            static_assert(GA >= 3 && GA <= 4, "group width");
+            static_assert(GA >= 2 && GA <= 4, "group width");
             switch (g) {
                 case 1: adj_g(std::integral_constant<int, 1>{}, t0, qoff, l); break;
                 case 2: adj_g(std::integral_constant<int, 2>{}, t0, qoff, l); break;
-                case 3: adj_g(std::integral_constant<int, 3>{}, t0, qoff, l); break;
-                case 4: adj_g(std::integral_constant<int, GA>{}, t0, qoff, l); break;
+                case 3: if constexpr (GA >= 3) adj_g(std::integral_constant<int, 3>{}, t0, qoff, l); break;
+                case 4: if constexpr (GA >= 4) adj_g(std::integral_constant<int, 4>{}, t0, qoff, l); break;
                 default: break;
             }
         };
@@ -560,6 +561,7 @@ struct BatchContext {
     size_t energy_cap = 0;
     void* rots = nullptr;
     size_t rots_cap = 0;
+    int group_width = 0;  // qubits per register group, 0 = the precision's default
     ~BatchContext() {
         if (rots) hipFree(rots);
         if (stream) hipStreamSynchronize(stream);
@@ -617,6 +619,21 @@ void iq_batch_free(void* bp) {
 
 // Largest tile width (qubits held in LDS) for the batch kernel in this precision.
 int iq_batch_max_tile_bits(int precision) { return precision == kC128 ? 12 : 13; }
+
+// Largest register-group width (qubits fused per register pass of the tile sweeps).
+int iq_batch_max_group_width(int precision) { return precision == kC128 ? 3 : 4; }
+
+int iq_batch_set_group_width(void* bp, int width) {
+    if (!bp) return -1;
+    BatchContext& B = *static_cast<BatchContext*>(bp);
+    if (width != 0 && (width < 2 || width > iq_batch_max_group_width(B.prec))) {
+        set_error(kErrInvalidArgument, "group width %d outside 2..%d (0 = default)", width,
+                  iq_batch_max_group_width(B.prec));
+        return -1;
+    }
+    B.group_width = width;
+    return 0;
+}
 
 // Energies and adjoint gradients of ncirc ansatz circuits in one persistent launch.
 //   nq[c], layers[c]: circuit shape (1 <= nq <= 30, layers >= 1);
@@ -676,7 +693,11 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
     // LDS for the widest tile in use (psi and lambda) plus the per-wave
     // gradient slots; as many workgroups as fit a CU at that size run at once
     const size_t lds = 2 * ((size_t)1 << bused) * elem + (size_t)(T / 64) * kMaxSlots * sizeof(double);
-    const void* kern = B.prec == kC128 ? (const void*)k_batch_hea<double> : (const void*)k_batch_hea<float>;
+    const int gw = B.group_width ? B.group_width : iq_batch_max_group_width(B.prec);
+    const void* kern = B.prec == kC128 ? (gw == 2 ? (const void*)k_batch_hea<double, 2> : (const void*)k_batch_hea<double, 3>)
+                       : gw == 2       ? (const void*)k_batch_hea<float, 2>
+                       : gw == 3       ? (const void*)k_batch_hea<float, 3>
+                                       : (const void*)k_batch_hea<float, 4>;
     hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int per_cu = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, T, lds) != hipSuccess || per_cu < 1) per_cu = 1;
@@ -706,12 +727,18 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     hipEventRecord(e0, st);
-    if (B.prec == kC128)
-        hipLaunchKernelGGL(k_batch_hea<double>, dim3(nblk), dim3(T), lds, st, B.circs, (int)ncirc, (const double*)B.rots, B.params,
-                           B.grads, B.energies, (double2*)B.scratch, scratch_elems, B.queue, hx, bused);
-    else
-        hipLaunchKernelGGL(k_batch_hea<float>, dim3(nblk), dim3(T), lds, st, B.circs, (int)ncirc, (const float*)B.rots, B.params,
-                           B.grads, B.energies, (float2*)B.scratch, scratch_elems, B.queue, hx, bused);
+#define IQ_BATCH_LAUNCH(RT, GWV, CV)                                                                              \
+    hipLaunchKernelGGL((k_batch_hea<RT, GWV>), dim3(nblk), dim3(T), lds, st, B.circs, (int)ncirc, (const RT*)B.rots, \
+                       B.params, B.grads, B.energies, (CV*)B.scratch, scratch_elems, B.queue, hx, bused)
+    if (B.prec == kC128) {
+        if (gw == 2) IQ_BATCH_LAUNCH(double, 2, double2);
+        else IQ_BATCH_LAUNCH(double, 3, double2);
+    } else {
+        if (gw == 2) IQ_BATCH_LAUNCH(float, 2, float2);
+        else if (gw == 3) IQ_BATCH_LAUNCH(float, 3, float2);
+        else IQ_BATCH_LAUNCH(float, 4, float2);
+    }
+#undef IQ_BATCH_LAUNCH
     if (!hip_ok(hipGetLastError(), "batch launch")) return -1;
     hipEventRecord(e1, st);
     std::vector<double> en(ncirc);

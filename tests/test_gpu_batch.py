@@ -85,10 +85,30 @@ def test_autotuner_table(M, tmp_path):
     r = batch.BatchRunner("complex64")
     t = batch.autotune(r, specs, cache=tmp_path / "tune.json", per_n=16)
     assert set(t["tile_bits"]) == {10, 14} and t["block_threads"] in (256, 512)
+    assert t["group_width"] in (2, 3, 4) and set(t["timings"]["group_width"]) == {"2", "3", "4"}
     again = batch.autotune(r, specs, cache=tmp_path / "tune.json", per_n=16)
-    assert again["cached"] and again["tile_bits"] == t["tile_bits"]
-    e, g, _ = r.run(specs, tile_bits=t["tile_bits"], block_threads=t["block_threads"])
+    assert again["cached"] and again["tile_bits"] == t["tile_bits"] and again["group_width"] == t["group_width"]
+    e, g, _ = r.run(specs, tile_bits=t["tile_bits"], block_threads=t["block_threads"], group_width=t["group_width"])
     assert np.all(np.isfinite(e))
+
+
+@pytest.mark.parametrize("precision,widths", [("complex64", (2, 3, 4)), ("complex128", (2, 3))])
+def test_group_widths_agree(M, precision, widths):
+    """Every fusion width the autotuner may pick gives the same energies and
+    gradients (the rotations of a register group commute; only the order of
+    the sums changes), on resident and tiled circuits."""
+    _, batch = M
+    specs = _specs(7)
+    r = batch.BatchRunner(precision)
+    ref_e, ref_g, _ = r.run(specs, tile_bits=TILES, block_threads=512)
+    tol = 1e-12 if precision == "complex128" else 2e-5
+    for w in widths:
+        e, g, _ = r.run(specs, tile_bits=TILES, block_threads=512, group_width=w)
+        for (n, l, _), a, b, ga, gb in zip(specs, e, ref_e, g, ref_g):
+            assert abs(a - b) < tol, (w, n, l)
+            assert np.max(np.abs(ga - gb)) < tol * max(1.0, np.max(np.abs(gb))), (w, n, l)
+    with pytest.raises(ValueError):
+        r.run(specs[:1], group_width=5)
 
 
 def test_batch_1024_circuits_every_size_stratified(M):
@@ -102,8 +122,10 @@ def test_batch_1024_circuits_every_size_stratified(M):
     assert ns == list(range(10, 21)) and max(s[1] for s in specs) == 16
     r = batch.BatchRunner("complex64")
     tune = batch.autotune(r, specs, cache=None, per_n=16)
-    print(f"1024-circuit batch, tiles {tune['tile_bits']}, {tune['block_threads']} threads", flush=True)
-    e, g, ms = r.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"])
+    print(f"1024-circuit batch, tiles {tune['tile_bits']}, {tune['block_threads']} threads, "
+          f"fusion width {tune['group_width']}", flush=True)
+    e, g, ms = r.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"],
+                     group_width=tune["group_width"])
     assert ms > 0 and np.all(np.isfinite(e))
     rng = np.random.default_rng(12)
     for n in ns:
