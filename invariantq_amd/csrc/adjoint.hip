@@ -54,8 +54,8 @@ struct Seg1qArgs {
 };
 
 struct EdgeList {
-    uint8_t a[kEdgeMax];
-    uint8_t b[kEdgeMax];
+    uint32_t a[kEdgeMax];  // 32-bit: indexed by a wave-uniform value, a scalar load
+    uint32_t b[kEdgeMax];
     int ne;
 };
 
@@ -225,7 +225,9 @@ struct PassDesc {
     int r, m, g0, ts_bits;
     uint64_t ntiles;
     int nseg, first, ncz, ngroups;
-    uint8_t gt0[kGroupMax], gsize[kGroupMax], gfirst[kGroupMax];  // first tile bit, runs, first run (in the pass)
+    // 32-bit entries throughout: a byte cannot be a scalar load, and a vector
+    // load of a descriptor byte costs a memory round trip per use
+    uint32_t gt0[kGroupMax], gsize[kGroupMax], gfirst[kGroupMax];  // first tile bit, runs, first run (in the pass)
     // Bit g set when an earlier group of the pass shares a qubit with group g.
     // With none set and psi stored, the reverse sweep takes every group's
     // overlaps at the pass's end and then moves lambda alone.
@@ -236,8 +238,8 @@ struct PassDesc {
     // bit) in cza/czb[0, ncross).  Neither: (global, global) in
     // cza/czb[ncross, ncross + nfixed).
     int ncross, nfixed, nshift;
-    uint8_t cza[kEdgeMax], czb[kEdgeMax];
-    uint8_t qshift[kShiftMax];
+    uint32_t cza[kEdgeMax], czb[kEdgeMax];
+    uint32_t qshift[kShiftMax];
     uint32_t qmask[kShiftMax];
 };
 
@@ -270,9 +272,12 @@ __device__ __forceinline__ CzTile cz_setup(const PassDesc& d, uint64_t t) {
     return CzTile{(uint32_t)__builtin_amdgcn_readfirstlane(M), (uint32_t)__builtin_amdgcn_readfirstlane(c)};
 }
 
+// The first two distance classes are evaluated without a branch (entries past
+// nshift have a zero mask: their term is 0); a CZ ring has at most two in a tile.
 __device__ __forceinline__ bool cz_odd(const PassDesc& d, CzTile z, uint32_t j) {
     uint32_t par = z.c ^ __popc(j & z.M);
-    for (int k = 0; k < d.nshift; ++k) par ^= __popc(j & (j >> d.qshift[k]) & d.qmask[k]);
+    par ^= __popc(j & (j >> d.qshift[0]) & d.qmask[0]) ^ __popc(j & (j >> d.qshift[1]) & d.qmask[1]);
+    for (int k = 2; k < d.nshift; ++k) par ^= __popc(j & (j >> d.qshift[k]) & d.qmask[k]);
     return (par & 1u) != 0;
 }
 
@@ -1421,8 +1426,8 @@ bool run_forward_segments(Circuit& c, const double* params) {
             EdgeList el{};
             el.ne = (int)s.edges.size();
             for (int k = 0; k < el.ne; ++k) {
-                el.a[k] = (uint8_t)s.edges[k].first;
-                el.b[k] = (uint8_t)s.edges[k].second;
+                el.a[k] = (uint32_t)s.edges[k].first;
+                el.b[k] = (uint32_t)s.edges[k].second;
             }
             const uint64_t count = e.dim;
             const unsigned b = (unsigned)std::min<uint64_t>((count + 255) / 256, 65536);
@@ -1536,8 +1541,8 @@ bool run_reverse_segments(Circuit& c, const double* params) {
             EdgeList el{};
             el.ne = (int)s.edges.size();
             for (int k = 0; k < el.ne; ++k) {
-                el.a[k] = (uint8_t)s.edges[k].first;
-                el.b[k] = (uint8_t)s.edges[k].second;
+                el.a[k] = (uint32_t)s.edges[k].first;
+                el.b[k] = (uint32_t)s.edges[k].second;
             }
             const unsigned b = (unsigned)std::min<uint64_t>((e.dim + 255) / 256, 65536);
             hipLaunchKernelGGL(k_cz_set<R>, dim3(b), dim3(256), 0, e.stream, (cv<R>*)e.amp, (cv<R>*)c.lam, e.dim, el);
@@ -1636,6 +1641,9 @@ int local_bit(const PassDesc& d, int q) {
 void set_cz(PassDesc& d, const std::vector<std::pair<int, int>>& edges) {
     d.ncz = (int)edges.size();
     d.ncross = d.nfixed = d.nshift = 0;
+    // cz_odd evaluates the first two distance classes unconditionally: unused
+    // entries must contribute nothing
+    for (int k = 0; k < kShiftMax; ++k) d.qshift[k] = d.qmask[k] = 0;
     std::vector<std::pair<int, int>> fixed;
     for (const auto& e : edges) {
         int la = local_bit(d, e.first), lb = local_bit(d, e.second);
@@ -1645,22 +1653,22 @@ void set_cz(PassDesc& d, const std::vector<std::pair<int, int>>& edges) {
             int k = 0;
             while (k < d.nshift && d.qshift[k] != sh) ++k;
             if (k == d.nshift) {
-                d.qshift[k] = (uint8_t)sh;
+                d.qshift[k] = (uint32_t)sh;
                 d.qmask[k] = 0;
                 ++d.nshift;
             }
             d.qmask[k] ^= 1u << la;
         } else if (la >= 0 || lb >= 0) {
-            d.cza[d.ncross] = (uint8_t)(la >= 0 ? la : lb);
-            d.czb[d.ncross] = (uint8_t)(la >= 0 ? e.second : e.first);
+            d.cza[d.ncross] = (uint32_t)(la >= 0 ? la : lb);
+            d.czb[d.ncross] = (uint32_t)(la >= 0 ? e.second : e.first);
             ++d.ncross;
         } else {
             fixed.push_back(e);
         }
     }
     for (const auto& e : fixed) {
-        d.cza[d.ncross + d.nfixed] = (uint8_t)e.first;
-        d.czb[d.ncross + d.nfixed] = (uint8_t)e.second;
+        d.cza[d.ncross + d.nfixed] = (uint32_t)e.first;
+        d.czb[d.ncross + d.nfixed] = (uint32_t)e.second;
         ++d.nfixed;
     }
 }
@@ -1769,9 +1777,9 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
             };
             for (const auto& gr : groups) {
                 if (l.d.ngroups == kGroupMax || (int)chunk.size() + gr.second > kPassSegMax) emit();
-                l.d.gt0[l.d.ngroups] = (uint8_t)tb_of(mine[gr.first]);
-                l.d.gsize[l.d.ngroups] = (uint8_t)gr.second;
-                l.d.gfirst[l.d.ngroups] = (uint8_t)chunk.size();
+                l.d.gt0[l.d.ngroups] = (uint32_t)tb_of(mine[gr.first]);
+                l.d.gsize[l.d.ngroups] = (uint32_t)gr.second;
+                l.d.gfirst[l.d.ngroups] = (uint32_t)chunk.size();
                 ++l.d.ngroups;
                 for (int q = 0; q < gr.second; ++q) chunk.push_back(mine[gr.first + q]);
             }
