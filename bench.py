@@ -121,11 +121,11 @@ def time_fused(binding, torch, n: int, precision: str, steps: int, warmup: int, 
     return wall, per_kernel, targets, norm
 
 
-def measured_traffic(n: int):
-    """HBM bytes per fused-gate launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (profiles/fused_gate_30q_traffic.json), or None."""
-    p = ROOT / "profiles" / "fused_gate_30q_traffic.json"
-    if n != 30 or not p.exists():
+def measured_traffic(name: str):
+    """HBM bytes per launch of a dominant kernel from a committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/<name>_traffic.json), or None."""
+    p = ROOT / "profiles" / f"{name}_traffic.json"
+    if not p.exists():
         return None
     try:
         return json.loads(p.read_text()).get("hbm_bytes_per_launch")
@@ -276,7 +276,8 @@ def adjoint_vs_shift(torch, n: int, layers: int, steps: int, shift_params: int, 
     out_roof = None
     if rev:
         # a reverse pass reads the stored psi and reads and writes lambda: 3 N 8 B
-        out_roof = roofline("k_tile_rev<float, true> (reverse tile pass)", 3 * n_amp * 8, float(np.mean(rev)), len(rev))
+        out_roof = roofline("k_tile_rev_lam<float> (reverse tile pass)", 3 * n_amp * 8, float(np.mean(rev)), len(rev),
+                            traffic=measured_traffic("adjoint_24q_c64") if n == 24 else None)
         out_roof["per_pass_in_layer_TBps"] = [round(3 * n_amp * 8 / float(np.mean(rev[i::3])) / 1e12, 3)
                                               for i in range(min(3, len(rev)))]
     cpu_out = None
@@ -608,6 +609,12 @@ def primitives_workload(binding, torch, n: int = 30, reps: int = 3):
         dev(lambda: sv.apply_2q(20, 3, gates.CX)), "events")
     row("a3 apply_2q CZ (7,25), diagonal", "statevector.cpp:49-87", 2 * N * s,
         dev(lambda: sv.apply_2q(7, 25, gates.CZ)), "events")
+    # Toffoli (controls 21 and 9, target 3): blockdiag(I6, X) on the 3 qubits,
+    # MSB-first; only the quarter with both controls set is read and written
+    toffoli = np.eye(8, dtype=np.complex128)
+    toffoli[6:, 6:] = gates.X
+    row("a4 apply_fused Toffoli (21,9,3), controlled: a quarter of the state", "statevector.cpp:89-152",
+        0.5 * N * s, dev(lambda: sv.apply_fused([21, 9, 3], toffoli)), "events")
     row("a4 apply_fused k=3 (4,12,27)", "statevector.cpp:89-152", 2 * N * s,
         dev(lambda: sv.apply_fused([4, 12, 27], u3m)), "events")
     row("a4 apply_fused k=4 (0,2,15,22), LDS-tiled", "statevector.cpp:89-152", 2 * N * s,
@@ -971,7 +978,7 @@ def main():
         "gate_applications_per_s": round(a.blocks / wall, 2),
         "roofline": {"bound": "hbm", "achieved": round(bytes_block / dom_d / 1e9, 1), "peak": HBM_PEAK_TBPS * 1000,
                      "unit": "GB/s", "frac": round(bytes_block / dom_d / 1e12 / HBM_PEAK_TBPS, 4),
-                     "traffic": measured_traffic(n), "kernel": dom,
+                     "traffic": measured_traffic("fused_gate_30q") if n == 30 else None, "kernel": dom,
                      "algorithmic_bytes_per_launch": bytes_block},
         "kernels": kstats,
         "error": err,
