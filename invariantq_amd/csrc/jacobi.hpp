@@ -22,10 +22,10 @@
 // pivots, and therefore the arithmetic and the result, are those of the full
 // scan, bit for bit.
 //
-// The first scan of every row (the O(d^2) part: 2^29 entries for a 15-qubit
-// reduced matrix) is split over host threads, rows dealt round-robin so each
-// thread gets a similar share of the triangle; rows are independent, so the
-// result does not depend on the split.
+// The first scan of every row (2^29 entries for a 15-qubit reduced matrix)
+// and each rotation's row updates are split over host threads, rows dealt
+// round-robin so each thread gets a similar share of the triangle; rows are
+// independent, so the result does not depend on the split.
 #pragma once
 
 #include <algorithm>
@@ -87,22 +87,28 @@ inline std::vector<double> jacobi_eigenvalues_inplace(std::complex<double>* A, s
         offer(j1);
         if (two) offer(j2);
     };
-    {
-        const size_t hw = std::max(1u, std::thread::hardware_concurrency());
-        const size_t nt = d >= 512 ? std::min<size_t>(16, hw) : 1;
-        auto scan_rows = [&](size_t t) {
-            for (size_t i = t; i + 1 < d; i += nt) rescan(i);
+    // rows dealt round-robin over host threads (rows are independent: each
+    // writes only its own row maximum), on the calling thread for small d or
+    // when no thread can be had
+    const size_t hw = std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+    auto for_rows = [&](size_t end, size_t nthreads, auto&& body) {
+        auto part = [&](size_t t) {
+            for (size_t i = t; i < end; i += nthreads) body(i);
         };
         std::vector<std::thread> pool;
         size_t t = 1;
         try {
-            for (; t < nt; ++t) pool.emplace_back(scan_rows, t);
-        } catch (const std::exception&) {  // no thread to be had: scan the rest here
-            for (; t < nt; ++t) scan_rows(t);
+            for (; t < nthreads; ++t) pool.emplace_back(part, t);
+        } catch (const std::exception&) {
+            for (; t < nthreads; ++t) part(t);
         }
-        scan_rows(0);
+        part(0);
         for (auto& th : pool) th.join();
-    }
+    };
+    for_rows(d - 1, d >= 512 ? hw : 1, [&](size_t i) { rescan(i); });
+    // a rotation's updates cost about 20 us per thread started: threads from
+    // d = 2048 (below, the whole 100 rotations take a few ms on one thread)
+    const size_t rot_threads = d >= 2048 ? hw : 1;
     for (int it = 0; it < max_iter; ++it) {
         double best = 0.0;
         size_t p = 0, q = 1;
@@ -134,12 +140,14 @@ inline std::vector<double> jacobi_eigenvalues_inplace(std::complex<double>* A, s
         at(q, q) = nqq;
         at(p, q) = 0.0;
         at(q, p) = 0.0;
-        // rows above p and q: their entries in columns p and q changed
-        for (size_t i = 0; i < q; ++i) {
-            if (i == p) continue;
+        // rows above p and q: their entries in columns p and q changed (a row
+        // whose maximum sat there is rescanned, O(d): at d = 2^15 about 13,600
+        // rows per rotation, so the rows go over host threads)
+        for_rows(q, rot_threads, [&](size_t i) {
+            if (i == p) return;
             if (i < p) update(i, p, q, true);
             else update(i, q, q, false);
-        }
+        });
         rescan(p);
         if (q + 1 < d) rescan(q);
     }

@@ -717,6 +717,79 @@ __global__ void __launch_bounds__(64) k_herk_mfma(const double2* __restrict__ ph
     }
 }
 
+// The same product by 64x64 output blocks, for 6 or more kept and 4 or more
+// traced qubits: a workgroup of four waves stages 16 environment rows of its
+// row strip and its column strip in LDS per step (each row a 1 KiB
+// contiguous load), and each wave accumulates a 32x32 quarter of the block
+// as 2x2 tiles of v_mfma_f64_16x16x4 with the operand and result layouts of
+// k_herk_mfma.  A strip element is read from memory once per block instead
+// of once per 16x16 tile.  Blocks on the diagonal compute all four quarters
+// (the lower half agrees with the conjugate of the upper to rounding: its
+// imaginary parts interleave the two product sets in the other order); the
+// others also write their conjugate transpose.
+constexpr uint32_t kHerkBlock = 64, kHerkRows = 16;
+
+__global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict__ phi, uint64_t m, uint32_t d,
+                                                      double2* __restrict__ out) {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    const uint32_t bR = blockIdx.y, bC = blockIdx.x;
+    if (bC < bR) return;
+    __shared__ double2 sa[kHerkRows][kHerkBlock], sb[kHerkRows][kHerkBlock];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t wr = wave >> 1, wc = wave & 1u, kk = lane >> 4, j = lane & 15u;
+    const uint64_t r0 = (uint64_t)bR * kHerkBlock, c0 = (uint64_t)bC * kHerkBlock;
+    const bool diag = bR == bC;
+    v4 re[2][2], im[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            re[a][b] = v4{0, 0, 0, 0};
+            im[a][b] = v4{0, 0, 0, 0};
+        }
+    for (uint64_t e0 = 0; e0 < m; e0 += kHerkRows) {
+        // 16 rows x 64 of each strip: four elements per thread per strip
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t idx = tid + (uint32_t)i * 256u, er = idx >> 6, col = idx & 63u;
+            const uint64_t row = (e0 + er) * d;
+            sa[er][col] = phi[row + r0 + col];
+            if (!diag) sb[er][col] = phi[row + c0 + col];
+        }
+        __syncthreads();
+        const double2(*B)[kHerkBlock] = diag ? sa : sb;
+#pragma unroll
+        for (uint32_t ks = 0; ks < kHerkRows; ks += 4) {
+            double2 a[2], b[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                a[t] = sa[ks + kk][wr * 32 + t * 16 + j];
+                b[t] = B[ks + kk][wc * 32 + t * 16 + j];
+            }
+#pragma unroll
+            for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+                for (int tb = 0; tb < 2; ++tb) {
+                    re[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].x, b[tb].x, re[ta][tb], 0, 0, 0);
+                    re[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].y, b[tb].y, re[ta][tb], 0, 0, 0);
+                    im[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].y, b[tb].x, im[ta][tb], 0, 0, 0);
+                    im[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[ta].x, b[tb].y, im[ta][tb], 0, 0, 0);
+                }
+        }
+        __syncthreads();  // the strips are rewritten by the next step
+    }
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const uint64_t row = r0 + wr * 32 + ta * 16 + kk + 4 * jj, col = c0 + wc * 32 + tb * 16 + j;
+                out[row * d + col] = make_double2(re[ta][tb][jj], im[ta][tb][jj]);
+                if (!diag) out[col * d + row] = make_double2(re[ta][tb][jj], -im[ta][tb][jj]);
+            }
+}
+
 // Gate-matrix upload on the compute stream: reads pinned (coherent, uncached)
 // host memory over PCIe and writes device memory, so the gate kernel that
 // follows needs no copy-engine round trip.  One 16-byte element per thread
@@ -981,8 +1054,13 @@ hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t
                                    total, scratch);
             phi = scratch;
         }
-        const uint32_t nt = d / 16;
-        hipLaunchKernelGGL(k_herk_mfma, dim3(nt, nt), dim3(64), 0, st, phi, m, d, out);
+        if (k >= 6 && n - k >= 4) {  // d a multiple of 64, m of 16
+            const uint32_t nb = d / kHerkBlock;
+            hipLaunchKernelGGL(k_herk_blocked, dim3(nb, nb), dim3(256), 0, st, phi, m, d, out);
+        } else {
+            const uint32_t nt = d / 16;
+            hipLaunchKernelGGL(k_herk_mfma, dim3(nt, nt), dim3(64), 0, st, phi, m, d, out);
+        }
         return hipGetLastError();
     }
     const uint64_t total = (uint64_t)d * d;

@@ -123,3 +123,15 @@ def test_entropy_above_15_is_nan_not_a_value():
     lib.iq_clear_error()
     # an empty or full subsystem is 0, as in the reference
     assert lib.iq_statevector_entropy(sv._ptr, q.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)), 0) == 0.0
+
+
+def test_reduced_matrix_blocked_kernel_off_diagonal_blocks(oracle_lib):
+    """10 kept of 16 (a 1024 x 1024 rho_A: 16 x 16 blocks of 64, the blocked
+    matrix-core kernel with its off-diagonal blocks and their conjugate
+    transposes), kept qubits in scrambled order, against the oracle."""
+    n, keep = 16, [9, 2, 15, 0, 7, 4, 12, 1, 10, 5]
+    sv, ref = states(oracle_lib, n, seed=31)
+    print(f"reduced matrix n={n} k={len(keep)} (blocked)", flush=True)
+    got, want = sv.reduced_density_matrix(keep), ref.partial_trace(keep)
+    assert np.max(np.abs(got - want)) < 1e-13
+    assert np.max(np.abs(got - got.conj().T)) < 1e-15
