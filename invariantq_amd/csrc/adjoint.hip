@@ -368,23 +368,33 @@ __device__ __forceinline__ void sub_overlaps(const cv<R> (&x)[1 << G], const cv<
     constexpr uint32_t D = 1u << G;
     // T_Z of every run from one product per element, w_m = Im(conj(y_m) x_m):
     // T_Z(i) = sum_m (-1)^(bit i of m) w_m = W - 2 (sum of w_m with bit i set)
-    R w[D], wall = 0;
+    R w[D], wall;
 #pragma unroll
     for (uint32_t m = 0; m < D; ++m) {
-        w[m] = y[m].x * x[m].y - y[m].y * x[m].x;
-        wall += w[m];
+        w[m] = fma(y[m].x, x[m].y, -(y[m].y * x[m].x));
+        wall = m ? wall + w[m] : w[m];
     }
 #pragma unroll
     for (int i = 0; i < G; ++i) {
         const uint32_t bit = 1u << i;
+        // one fused multiply-add per product term, chained through the sum
+        // (no separate product, difference and accumulate per pair)
         R tx = 0, ty = 0, w1 = 0;
+        bool first = true;
 #pragma unroll
         for (uint32_t m = 0; m < D; ++m) {
             if (m & bit) continue;
             const cv<R>&a0 = x[m], &a1 = x[m | bit], &b0 = y[m], &b1 = y[m | bit];
-            tx += (b0.x * a1.y - b0.y * a1.x) + (b1.x * a0.y - b1.y * a0.x);
-            ty += (b1.x * a0.x + b1.y * a0.y) - (b0.x * a1.x + b0.y * a1.y);
-            w1 += w[m | bit];
+            tx = first ? b0.x * a1.y : fma(b0.x, a1.y, tx);
+            ty = first ? b1.x * a0.x : fma(b1.x, a0.x, ty);
+            w1 = first ? w[m | bit] : w1 + w[m | bit];
+            first = false;
+            tx = fma(-b0.y, a1.x, tx);
+            tx = fma(b1.x, a0.y, tx);
+            tx = fma(-b1.y, a0.x, tx);
+            ty = fma(b1.y, a0.y, ty);
+            ty = fma(-b0.x, a1.x, ty);
+            ty = fma(-b0.y, a1.y, ty);
         }
         acc[i][0] += (double)tx;
         acc[i][1] += (double)ty;

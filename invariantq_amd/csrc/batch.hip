@@ -14,12 +14,17 @@
 //   n >  b  they live in a per-workgroup HBM scratch and every ansatz layer is
 //           two tiled passes: pass A tiles the b low qubits (contiguous), pass B
 //           tiles the n-b high qubits plus the low 2b-n ones (runs of 2^(2b-n)
-//           amplitudes) and also applies the CZ ring, which depends only on the
-//           global index.  The reverse sweep runs B then A per layer.
-// Inside a tile each qubit is one LDS sweep over 2^(b-1) pairs.  Gradient inner
-// products Im<lambda|G psi> accumulate in FP64 per lane, reduce across the
-// wave with shuffles into per-wave slots, and are summed over waves in a fixed
-// order at the end of each layer: deterministic, no atomics on results.
+//           amplitudes) and also applies the layer's diagonal, which depends
+//           only on the global index.  The reverse sweep runs B then A per layer.
+// A layer is RY on every qubit (real rotations), then its diagonal D = CZ ring
+// times the RZ set: one phase per amplitude, e^{i sum_q phi_q (g_q - 1/2)} with
+// the CZ sign, applied as pass B's tile is stored (forward) or loaded (D^dagger,
+// reverse).  Undoing D first leaves psi and lambda at the RY outputs, where
+// dE/dtheta_q = Im<lambda|Y_q psi> and dE/dphi_q = Im<lambda|Z_q psi>.  Inside a
+// tile, rotations go in register groups of GW qubits, one LDS round trip per
+// group.  Gradient inner products accumulate in FP64 per lane, reduce across
+// the wave with shuffles into per-wave slots, and are summed over waves in a
+// fixed order at the end of each layer: deterministic, no atomics on results.
 
 #include <hip/hip_runtime.h>
 
@@ -49,35 +54,24 @@ struct BatchCircuit {
 
 constexpr int kMaxSlots = 96;  // gradient slots per layer: 3 per qubit (n <= 30)
 
-template <typename R>
-struct Rot {
-    R u[8];  // re00 im00 re01 im01 re10 im10 re11 im11
-};
+// Per qubit and layer the kernel keeps (cos theta/2, sin theta/2) of the RY;
+// the layer's RZ set is applied as one phase per amplitude (see k_batch_hea).
+constexpr int kRotStride = 2;
+// LDS phase tables: tile bits 0..6 and 7..12 (tiles of at most 2^13)
+constexpr int kPhaseLo = 128, kPhaseHi = 64;
 
+// (a, b) <- RY (a, b) with RY = [[c, -s], [s, c]] (a real rotation of a complex pair)
 template <typename R>
-__device__ __forceinline__ void mul2(const R* u, R& x0r, R& x0i, R& x1r, R& x1i) {
-    const R y0r = u[0] * x0r - u[1] * x0i + u[2] * x1r - u[3] * x1i;
-    const R y0i = u[0] * x0i + u[1] * x0r + u[2] * x1i + u[3] * x1r;
-    const R y1r = u[4] * x0r - u[5] * x0i + u[6] * x1r - u[7] * x1i;
-    const R y1i = u[4] * x0i + u[5] * x0r + u[6] * x1i + u[7] * x1r;
-    x0r = y0r; x0i = y0i; x1r = y1r; x1i = y1i;
+__device__ __forceinline__ void ry(R c, R s, cv<R>& a, cv<R>& b) {
+    const cv<R> a0 = a, b0 = b;
+    a.x = fma(c, a0.x, -(s * b0.x));
+    a.y = fma(c, a0.y, -(s * b0.y));
+    b.x = fma(s, a0.x, c * b0.x);
+    b.y = fma(s, a0.y, c * b0.y);
 }
 
-// RZ(phi) RY(theta) and its inverse (RZ RY)^dagger, from FP64 angles.
-template <typename R>
-__device__ __forceinline__ void make_rots(double th, double ph, R* fwd, R* inv) {
-    const double c = cos(0.5 * th), s = sin(0.5 * th);
-    const double cp = cos(0.5 * ph), sp = sin(0.5 * ph);
-    // RZ RY = [[e^{-i ph/2} c, -e^{-i ph/2} s], [e^{i ph/2} s, e^{i ph/2} c]]
-    const double f[8] = {cp * c, -sp * c, -cp * s, sp * s, cp * s, sp * s, cp * c, sp * c};
-    for (int i = 0; i < 8; ++i) fwd[i] = (R)f[i];
-    // conjugate transpose: entry (r, c) = conj(f(c, r))
-    const double g[8] = {f[0], -f[1], f[4], -f[5], f[2], -f[3], f[6], -f[7]};
-    for (int i = 0; i < 8; ++i) inv[i] = (R)g[i];
-}
-
-// One block per circuit: rots[(param_off / 2 + n*l + q) * 24 + (0..7 | 8..15)]
-// = (RZ RY, (RZ RY)^dagger) of qubit q in layer l (16..23 unused).
+// One block per circuit: rots[(param_off / 2 + n*l + q) * kRotStride + (0, 1)]
+// = (cos, sin) of half the RY angle of qubit q in layer l, from FP64 angles.
 template <typename R>
 __global__ void k_prepare_rots(const BatchCircuit* __restrict__ circs, const double* __restrict__ params,
                                R* __restrict__ rots) {
@@ -85,9 +79,10 @@ __global__ void k_prepare_rots(const BatchCircuit* __restrict__ circs, const dou
     const int n = C.n;
     for (int k = threadIdx.x; k < n * C.layers; k += blockDim.x) {
         const int l = k / n, q = k % n;
-        const double* P = params + C.param_off + 2 * n * l;
-        R* o = rots + (C.param_off / 2 + k) * 24;
-        make_rots<R>(P[q], P[n + q], o, o + 8);
+        const double th = params[C.param_off + 2 * n * l + q];
+        R* o = rots + (C.param_off / 2 + k) * kRotStride;
+        o[0] = (R)cos(0.5 * th);
+        o[1] = (R)sin(0.5 * th);
     }
 }
 
@@ -115,18 +110,20 @@ __device__ __forceinline__ bool cz_ring_sign(uint64_t g, int n) {
 }
 
 // GW: qubits per register group (the fusion width of the tile sweeps): 2..4
-// for complex64, 2..3 for complex128.
+// for complex64, 2..3 for complex128.  tile_elems: amplitudes of LDS for the
+// tiles (psi and lambda of the most demanding circuit of the launch).
 template <typename R, int GW>
 __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restrict__ circs, int ncirc,
                                                    const R* __restrict__ rots, const double* __restrict__ params,
                                                    double* __restrict__ grads,
                                                    double* __restrict__ energies, cv<R>* __restrict__ scratch,
-                                                   uint64_t scratch_elems, int* __restrict__ queue, double hx, int bmax) {
+                                                   uint64_t scratch_elems, int* __restrict__ queue, double hx,
+                                                   uint32_t tile_elems) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = T >> 6;
-    cv<R>* tp = reinterpret_cast<cv<R>*>(smem);
-    cv<R>* tl = tp + (1u << bmax);
-    double* wred = reinterpret_cast<double*>(tl + (1u << bmax));  // [nw][kMaxSlots]
+    cv<R>* const tiles = reinterpret_cast<cv<R>*>(smem);
+    double* wred = reinterpret_cast<double*>(tiles + tile_elems);  // [nw][kMaxSlots]
+    cv<R>* ptab = reinterpret_cast<cv<R>*>(wred + nw * kMaxSlots);  // [kPhaseLo + kPhaseHi]
     __shared__ int s_next;
     cv<R>* gpsi = scratch + (uint64_t)blockIdx.x * 2 * scratch_elems;
     cv<R>* glam = gpsi + scratch_elems;
@@ -143,10 +140,12 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         const bool resident = (n <= b);
         const uint32_t TS = 1u << b;
         const uint64_t N = 1ULL << n;
-        const R* RT = rots + (C.param_off / 2) * 24;  // [l][q][24]
+        const R* RT = rots + (C.param_off / 2) * kRotStride;  // [l][q][cos, sin]
         double* G = grads + C.param_off;
         const uint64_t ntA = resident ? 1 : (N >> b);
         const int h = n - b;
+        cv<R>* const tp = tiles;       // psi's tile
+        cv<R>* const tl = tiles + TS;  // lambda's
 
         // LDS position of tile element j: the low four index bits XOR the next
         // four, so sixteen lanes reading sixteen elements of a register group
@@ -156,24 +155,53 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         // elements; the default is 4 for complex64 and 3 for complex128)
         constexpr int GM = GW;
         constexpr int GA = GW;
-        // --- tile transfer helpers: eight elements per thread in flight before
-        // any is written on, so a tile costs about one memory latency, not
-        // one per element
+        // The layer's diagonal D = CZ ring * prod_q RZ_q(phi_q): amplitude g takes
+        // (-1)^cz(g) * e^{i sum_q phi_q (g_q - 1/2)}.  The phase of tile element j
+        // (j < TS) is ptab[j & 127] * ptab[128 + (j >> 7)] times the CZ sign,
+        // conjugated for D^dagger.
+        auto phase_at = [&](int kind, uint64_t t, uint32_t j, bool dag, R& pr, R& pi) __attribute__((always_inline)) {
+            const cv<R> a = ptab[j & 127u], c = ptab[kPhaseLo + (j >> 7)];
+            pr = a.x * c.x - a.y * c.y;
+            pi = a.x * c.y + a.y * c.x;
+            if (cz_ring_sign(tile_global(kind, t, j, b, n), n)) {
+                pr = -pr;
+                pi = -pi;
+            }
+            if (dag) pi = -pi;
+        };
+        auto cmul = [](cv<R> v, R pr, R pi) __attribute__((always_inline)) {
+            cv<R> o;
+            o.x = v.x * pr - v.y * pi;
+            o.y = v.x * pi + v.y * pr;
+            return o;
+        };
+        // --- tile transfer helpers: eight or sixteen elements per thread in
+        // flight before any is written on, so a tile costs about one memory
+        // latency, not one per element
         // psi (and lambda) tile in one round of loads: up to 16 elements per
-        // thread per array in flight (64 KiB per CU for a 2^13 complex64 tile)
-        auto load2 = [&](int kind, uint64_t t, bool two) __attribute__((always_inline)) {
-            for (uint32_t j0 = tid; j0 < TS; j0 += 16 * T) {
-                cv<R> v[16], w[16];
+        // thread per array in flight (64 KiB per CU for a 2^13 complex64 tile;
+        // 8 for complex128)
+        // dg: 0 as stored, 1 with D, 2 with D^dagger on the way into LDS
+        auto load2 = [&](int kind, uint64_t t, bool two, int dg) __attribute__((always_inline)) {
+            constexpr int U = sizeof(R) == 4 ? 16 : 8;  // complex128: 8 in flight, so nothing spills
+            for (uint32_t j0 = tid; j0 < TS; j0 += U * T) {
+                cv<R> v[U], w[U];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {  // every element assigned: the arrays stay in registers
+                for (int u = 0; u < U; ++u) {  // every element assigned: the arrays stay in registers
                     const uint64_t gi = tile_global(kind, t, min(j0 + u * T, TS - 1), b, n);
                     v[u] = gpsi[gi];
                     w[u] = two ? glam[gi] : v[u];
                 }
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
+                for (int u = 0; u < U; ++u) {
                     const uint32_t j = j0 + u * T;
                     if (j < TS) {
+                        if (dg) {
+                            R pr, pi;
+                            phase_at(kind, t, j, dg == 2, pr, pi);
+                            v[u] = cmul(v[u], pr, pi);
+                            if (two) w[u] = cmul(w[u], pr, pi);
+                        }
                         tp[P(j)] = v[u];
                         if (two) tl[P(j)] = w[u];
                     }
@@ -195,7 +223,8 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                 }
             }
         };
-        auto store = [&](int kind, uint64_t t, const cv<R>* src, cv<R>* dst) __attribute__((always_inline)) {
+        // dg as load2, on the way out of LDS
+        auto store = [&](int kind, uint64_t t, const cv<R>* src, cv<R>* dst, int dg) __attribute__((always_inline)) {
             for (uint32_t j0 = tid; j0 < TS; j0 += 8 * T) {
                 cv<R> v[8];
 #pragma unroll
@@ -206,7 +235,14 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const uint32_t j = j0 + u * T;
-                    if (j < TS) dst[tile_global(kind, t, j, b, n)] = v[u];
+                    if (j < TS) {
+                        if (dg) {
+                            R pr, pi;
+                            phase_at(kind, t, j, dg == 2, pr, pi);
+                            v[u] = cmul(v[u], pr, pi);
+                        }
+                        dst[tile_global(kind, t, j, b, n)] = v[u];
+                    }
                 }
             }
         };
@@ -215,7 +251,7 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         // 2^G elements that differ only in those bits into registers and
         // applies the G rotations there.  G is a compile-time constant so the
         // sub-cube stays in registers.
-        auto rot_g = [&](auto Gc, cv<R>* tile, int t0, int qoff, int l) __attribute__((always_inline)) {
+        auto rot_g = [&](auto Gc, int t0, int qoff, int l) __attribute__((always_inline)) {
             constexpr int G = decltype(Gc)::value;
             constexpr uint32_t D = 1u << G;
             const uint32_t cubes = TS >> G, lo = (1u << t0) - 1;
@@ -224,16 +260,16 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                 const uint32_t pb0 = P(base);  // P is linear over disjoint bits
                 cv<R> x[D];
 #pragma unroll
-                for (uint32_t m = 0; m < D; ++m) x[m] = tile[pb0 ^ P(m << t0)];
+                for (uint32_t m = 0; m < D; ++m) x[m] = tp[pb0 ^ P(m << t0)];
 #pragma unroll
                 for (int jb = 0; jb < G; ++jb) {
-                    const R* u = RT + (n * l + (t0 + jb + qoff)) * 24;
+                    const R c = RT[(n * l + (t0 + jb + qoff)) * kRotStride], sn = RT[(n * l + (t0 + jb + qoff)) * kRotStride + 1];
 #pragma unroll
                     for (uint32_t m = 0; m < D; ++m)
-                        if (!(m & (1u << jb))) mul2<R>(u, x[m].x, x[m].y, x[m | (1u << jb)].x, x[m | (1u << jb)].y);
+                        if (!(m & (1u << jb))) ry<R>(c, sn, x[m], x[m | (1u << jb)]);
                 }
 #pragma unroll
-                for (uint32_t m = 0; m < D; ++m) tile[pb0 ^ P(m << t0)] = x[m];
+                for (uint32_t m = 0; m < D; ++m) tp[pb0 ^ P(m << t0)] = x[m];
             }
             __syncthreads();
         };
@@ -249,33 +285,33 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             }
         };
         // dispatch on the group width; callers pass 1 <= g <= GM
-        auto rot_group = [&](cv<R>* tile, int t0, int g, int qoff, int l) __attribute__((always_inline)) {
+        auto rot_group = [&](int t0, int g, int qoff, int l) __attribute__((always_inline)) {
             static_assert(GM >= 2 && GM <= 4, "group width");
             switch (g) {
-                case 1: rot_g(std::integral_constant<int, 1>{}, tile, t0, qoff, l); break;
-                case 2: rot_g(std::integral_constant<int, 2>{}, tile, t0, qoff, l); break;
-                case 3: if constexpr (GM >= 3) rot_g(std::integral_constant<int, 3>{}, tile, t0, qoff, l); break;
-                case 4: if constexpr (GM >= 4) rot_g(std::integral_constant<int, 4>{}, tile, t0, qoff, l); break;
+                case 1: rot_g(std::integral_constant<int, 1>{}, t0, qoff, l); break;
+                case 2: rot_g(std::integral_constant<int, 2>{}, t0, qoff, l); break;
+                case 3: if constexpr (GM >= 3) rot_g(std::integral_constant<int, 3>{}, t0, qoff, l); break;
+                case 4: if constexpr (GM >= 4) rot_g(std::integral_constant<int, 4>{}, t0, qoff, l); break;
                 default: break;
             }
         };
-        // Reverse steps of tile bits [t0, t0 + G) on psi and lambda.  For qubit
-        // q with U = RZ(phi) RY(theta), both gradients are read before undoing U:
-        //   dE/dphi   = Im<lambda|Z psi>  = sum Im(l0* p0) - Im(l1* p1)
-        //   dE/dtheta = Im<lambda|RZ Y RZ^dag psi>,  RZ Y RZ^dag = [[0, -i e^{-i phi}], [i e^{i phi}, 0]]
-        //             = cos(phi) A - sin(phi) B,  A = sum Re(l1* p0) - Re(l0* p1),
-        //                                         B = sum Im(l1* p0) + Im(l0* p1)
-        // then one (RZ RY)^dagger on psi and lambda.  Products are summed in the
-        // state's precision over a sub-cube, then in FP64; cos and sin are applied
-        // in FP64 when the layer's gradients are written.  Rotations on different
-        // qubits commute, so the qubits of a group may go in any order.
+        // Reverse steps of tile bits [t0, t0 + G) on psi and lambda.  The layer's
+        // CZ ring and RZ set are already undone (one phase per amplitude), so
+        // psi and lambda sit at the RY outputs, where for qubit q
+        //   dE/dtheta = Im<lambda|Y psi> = sum Re(l1* p0) - Re(l0* p1),
+        //   dE/dphi   = Im<lambda|Z psi> (Z commutes with the RZ set)
+        //             = sum over bit = 0 minus bit = 1 of w_m = Im(conj(l_m) p_m);
+        // then RY^dagger (real) on psi and lambda.  Every qubit's products come
+        // from the same registers (rotations on the group's other qubits,
+        // applied to both, leave them unchanged), summed in the state's
+        // precision over a sub-cube, then in FP64.
         auto adj_g = [&](auto Gc, int t0, int qoff, int l) __attribute__((always_inline)) {
             constexpr int G = decltype(Gc)::value;
             constexpr uint32_t D = 1u << G;
             const uint32_t cubes = TS >> G, lo = (1u << t0) - 1;
-            double az[G], aa[G], ab[G];
+            double az[G], aa[G];
 #pragma unroll
-            for (int jb = 0; jb < G; ++jb) az[jb] = aa[jb] = ab[jb] = 0.0;
+            for (int jb = 0; jb < G; ++jb) az[jb] = aa[jb] = 0.0;
             for (uint32_t s = tid; s < cubes; s += T) {
                 const uint32_t base = ((s & ~lo) << G) | (s & lo);
                 const uint32_t pb0 = P(base);  // P is linear over disjoint bits
@@ -285,41 +321,41 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                     x[m] = tp[pb0 ^ P(m << t0)];
                     y[m] = tl[pb0 ^ P(m << t0)];
                 }
-                // every qubit's products from the same registers (rotations on
-                // the group's other qubits, applied to both, leave them
-                // unchanged); Im<lambda|Z psi> from one product per element:
-                // sum over bit = 0 minus bit = 1 of w_m = Im(conj(l_m) p_m)
-                R w[D], wall = 0;
+                R w[D], wall;
 #pragma unroll
                 for (uint32_t m = 0; m < D; ++m) {
-                    w[m] = y[m].x * x[m].y - y[m].y * x[m].x;
-                    wall += w[m];
+                    w[m] = fma(y[m].x, x[m].y, -(y[m].y * x[m].x));
+                    wall = m ? wall + w[m] : w[m];
                 }
 #pragma unroll
                 for (int jb = G - 1; jb >= 0; --jb) {
                     const uint32_t bit = 1u << jb;
-                    R w1 = 0, pa = 0, pb = 0;
+                    // one fused multiply-add per product term, chained through the sums
+                    R w1 = 0, pa = 0;
+                    bool first = true;
 #pragma unroll
                     for (uint32_t m = 0; m < D; ++m) {
                         if (m & bit) continue;
                         const cv<R>&a0 = x[m], &a1 = x[m | bit], &b0 = y[m], &b1 = y[m | bit];
-                        w1 += w[m | bit];
-                        pa += (b1.x * a0.x + b1.y * a0.y) - (b0.x * a1.x + b0.y * a1.y);
-                        pb += (b1.x * a0.y - b1.y * a0.x) + (b0.x * a1.y - b0.y * a1.x);
+                        w1 = first ? w[m | bit] : w1 + w[m | bit];
+                        pa = first ? b1.x * a0.x : fma(b1.x, a0.x, pa);
+                        first = false;
+                        pa = fma(b1.y, a0.y, pa);
+                        pa = fma(-b0.x, a1.x, pa);
+                        pa = fma(-b0.y, a1.y, pa);
                     }
                     az[jb] += (double)(wall - 2 * w1);
                     aa[jb] += (double)pa;
-                    ab[jb] += (double)pb;
                 }
 #pragma unroll
                 for (int jb = G - 1; jb >= 0; --jb) {
-                    const R* inv = RT + (n * l + (t0 + jb + qoff)) * 24 + 8;
+                    const R c = RT[(n * l + (t0 + jb + qoff)) * kRotStride], sn = RT[(n * l + (t0 + jb + qoff)) * kRotStride + 1];
                     const uint32_t bit = 1u << jb;
 #pragma unroll
                     for (uint32_t m = 0; m < D; ++m) {
                         if (m & bit) continue;
-                        mul2<R>(inv, x[m].x, x[m].y, x[m | bit].x, x[m | bit].y);
-                        mul2<R>(inv, y[m].x, y[m].y, y[m | bit].x, y[m | bit].y);
+                        ry<R>(c, -sn, x[m], x[m | bit]);
+                        ry<R>(c, -sn, y[m], y[m | bit]);
                     }
                 }
 #pragma unroll
@@ -328,28 +364,27 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                     tl[pb0 ^ P(m << t0)] = y[m];
                 }
             }
-            // wave sums of the 3G products (qubit jb: z, a, b at 3 jb + 0..2),
-            // reduce-scatter: offsets 32 and 16 halve the values a lane
-            // carries (12 -> 6 -> 3), offsets 8..1 finish each of the 3
+            // wave sums of the 2G products (qubit jb: z, a at 2 jb + 0, 1),
+            // reduce-scatter: offsets 32, 16 and 8 halve the values a lane
+            // carries (8 -> 4 -> 2 -> 1), offsets 4..1 finish the last
             {
-                double acc[12];
+                double acc[8];
 #pragma unroll
-                for (int k = 0; k < 12; ++k) {
-                    const int jb = k / 3;
-                    acc[k] = jb >= G ? 0.0 : (k % 3 == 0 ? az[jb < G ? jb : 0] : k % 3 == 1 ? aa[jb < G ? jb : 0] : ab[jb < G ? jb : 0]);
+                for (int k = 0; k < 8; ++k) {
+                    const int jb = k / 2;
+                    acc[k] = jb >= G ? 0.0 : (k % 2 == 0 ? az[jb < G ? jb : 0] : aa[jb < G ? jb : 0]);
                 }
-                const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0;
-                double v6[6];
+                const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0, h3 = (lane & 8) != 0;
+                double v4[4], v2[2];
 #pragma unroll
-                for (int i = 0; i < 6; ++i) v6[i] = (h5 ? acc[6 + i] : acc[i]) + __shfl_xor(h5 ? acc[i] : acc[6 + i], 32, 64);
+                for (int i = 0; i < 4; ++i) v4[i] = (h5 ? acc[4 + i] : acc[i]) + __shfl_xor(h5 ? acc[i] : acc[4 + i], 32, 64);
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    double xs = (h4 ? v6[3 + i] : v6[i]) + __shfl_xor(h4 ? v6[i] : v6[3 + i], 16, 64);
+                for (int i = 0; i < 2; ++i) v2[i] = (h4 ? v4[2 + i] : v4[i]) + __shfl_xor(h4 ? v4[i] : v4[2 + i], 16, 64);
+                double xs = (h3 ? v2[1] : v2[0]) + __shfl_xor(h3 ? v2[0] : v2[1], 8, 64);
 #pragma unroll
-                    for (int off = 8; off > 0; off >>= 1) xs += __shfl_xor(xs, off, 64);
-                    const int k = (h5 ? 6 : 0) + (h4 ? 3 : 0) + i;
-                    if ((lane & 15) == 0 && k < 3 * G) wred[wave * kMaxSlots + 3 * (t0 + qoff) + k] += xs;
-                }
+                for (int off = 4; off > 0; off >>= 1) xs += __shfl_xor(xs, off, 64);
+                const int k = (h5 ? 4 : 0) + (h4 ? 2 : 0) + (h3 ? 1 : 0);
+                if ((lane & 7) == 0 && k < 2 * G) wred[wave * kMaxSlots + 2 * (t0 + qoff) + k] += xs;
             }
             __syncthreads();
         };
@@ -366,30 +401,56 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         };
         // One pass over the tiles of a decomposition (kind 1: the b low qubits,
         // 2: the high qubits): each tile goes global -> LDS, through `body`,
-        // and back.  (Prefetching the next tile into registers across `body`
-        // needs 64 more VGPRs per lane and spilled at 512 threads.)
-        auto tile_pass = [&](int kind, bool two, auto&& body) __attribute__((always_inline)) {
+        // and back; pre(t) writes tile t's LDS tables, and dl, ds say which
+        // diagonal the loads and the stores take (as load2).  (Prefetching the
+        // next tile into registers across `body` needs 64 more VGPRs per lane
+        // and spilled at 512 threads.)
+        auto tile_pass = [&](int kind, bool two, int dl, int ds, auto&& pre, auto&& body) __attribute__((always_inline)) {
             for (uint64_t t = 0; t < ntA; ++t) {
-                load2(kind, t, two);
+                pre(t);  // the last tile's users passed the barrier below
+                load2(kind, t, two, dl);
                 __syncthreads();
                 body(t);
-                store(kind, t, tp, gpsi);
-                if (two) store(kind, t, tl, glam);
+                store(kind, t, tp, gpsi, ds);
+                if (two) store(kind, t, tl, glam, ds);
                 __syncthreads();
             }
         };
-        auto cz = [&](int kind, uint64_t t, bool both) __attribute__((always_inline)) {
-            for (uint32_t j = tid; j < TS; j += T) {
-                if (cz_ring_sign(tile_global(kind, t, j, b, n), n)) {
-                    cv<R> v = tp[P(j)];
-                    v.x = -v.x; v.y = -v.y;
-                    tp[P(j)] = v;
-                    if (both) {
-                        cv<R> w = tl[P(j)];
-                        w.x = -w.x; w.y = -w.y;
-                        tl[P(j)] = w;
-                    }
+        auto no_pre = [](uint64_t) __attribute__((always_inline)) {};
+        // The tables of phase_at, made in FP64 (angle sums, then sincos) for the
+        // tile's bit-to-qubit map, the second with the angles of the qubits the
+        // tile number fixes folded in.
+        auto build_phase = [&](int kind, uint64_t t, int l) __attribute__((always_inline)) {
+            const double* ph = params + C.param_off + 2 * n * l + n;
+            const int lo = 2 * b - n;  // kind 2: tile bits [0, lo) on qubits [0, lo), the rest on [b, n)
+            const int nb = kind == 0 ? n : b;
+            auto qubit = [&](int k) { return kind == 0 || k < lo ? k : k - lo + b; };
+            const uint32_t na = TS < 128u ? TS : 128u, nbt = TS > 128u ? TS >> 7 : 1u;
+            for (uint32_t i = tid; i < na + nbt; i += T) {
+                double ang = 0.0;
+                if (i < na) {
+                    for (int k = 0; k < 7 && k < nb; ++k) ang += ph[qubit(k)] * ((double)((i >> k) & 1u) - 0.5);
+                } else {
+                    const uint32_t e = i - na;
+                    for (int k = 7; k < nb; ++k) ang += ph[qubit(k)] * ((double)((e >> (k - 7)) & 1u) - 0.5);
+                    if (kind == 2)  // tile number bit k on qubit lo + k
+                        for (int k = 0; k < n - b; ++k) ang += ph[lo + k] * ((double)((t >> k) & 1u) - 0.5);
                 }
+                double sv, cvl;
+                sincos(ang, &sv, &cvl);
+                cv<R> v;
+                v.x = (R)cvl;
+                v.y = (R)sv;
+                ptab[i < na ? i : kPhaseLo + (i - na)] = v;
+            }
+        };
+        // D (forward) or D^dagger (reverse, on psi and lambda) over an LDS-resident state
+        auto diag = [&](bool both) __attribute__((always_inline)) {
+            for (uint32_t j = tid; j < TS; j += T) {
+                R pr, pi;
+                phase_at(0, 0, j, both, pr, pi);
+                tp[P(j)] = cmul(tp[P(j)], pr, pi);
+                if (both) tl[P(j)] = cmul(tl[P(j)], pr, pi);
             }
             __syncthreads();
         };
@@ -397,18 +458,16 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             for (int i = tid; i < nw * kMaxSlots; i += T) wred[i] = 0.0;
             __syncthreads();
         };
-        // layer gradients from the slots (3q: dE/dphi_q, 3q+1 / 3q+2: A, B)
+        // layer gradients from the slots (2q: dE/dphi_q, 2q+1: dE/dtheta_q)
         auto flush_slots = [&](int l) __attribute__((always_inline)) {
             for (int q = tid; q < n; q += T) {
-                double z = 0.0, va = 0.0, vb = 0.0;
+                double z = 0.0, va = 0.0;
                 for (int w = 0; w < nw; ++w) {
-                    z += wred[w * kMaxSlots + 3 * q];
-                    va += wred[w * kMaxSlots + 3 * q + 1];
-                    vb += wred[w * kMaxSlots + 3 * q + 2];
+                    z += wred[w * kMaxSlots + 2 * q];
+                    va += wred[w * kMaxSlots + 2 * q + 1];
                 }
-                const double ph = params[C.param_off + 2 * n * l + n + q];
                 G[2 * n * l + n + q] = z;
-                G[2 * n * l + q] = cos(ph) * va - sin(ph) * vb;
+                G[2 * n * l + q] = va;
             }
             __syncthreads();
         };
@@ -433,15 +492,18 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         }
         for (int l = 0; l < L; ++l) {
             if (resident) {
-                for_groups(0, n, GM, [&](int t0, int g) { rot_group(tp, t0, g, 0, l); });
-                cz(0, 0, false);
+                for_groups(0, n, GM, [&](int t0, int g) { rot_group(t0, g, 0, l); });
+                build_phase(0, 0, l);
+                __syncthreads();
+                diag(false);
             } else {
-                tile_pass(1, false, [&](uint64_t) {  // pass A: qubits 0..b-1
-                    for_groups(0, b, GM, [&](int t0, int g) { rot_group(tp, t0, g, 0, l); });
+                tile_pass(1, false, 0, 0, no_pre, [&](uint64_t) {  // pass A: RY on qubits 0..b-1
+                    for_groups(0, b, GM, [&](int t0, int g) { rot_group(t0, g, 0, l); });
                 });
-                tile_pass(2, false, [&](uint64_t t) {  // pass B: qubits b..n-1, then the CZ ring
-                    for_groups(b - h, b, GM, [&](int t0, int g) { rot_group(tp, t0, g, h, l); });
-                    cz(2, t, false);
+                // pass B: RY on qubits b..n-1, then the RZ set and the CZ ring as
+                // the tile is stored (the loads' barrier orders the tables first)
+                tile_pass(2, false, 0, 1, [&](uint64_t t) { build_phase(2, t, l); }, [&](uint64_t) {
+                    for_groups(b - h, b, GM, [&](int t0, int g) { rot_group(t0, g, h, l); });
                 });
             }
         }
@@ -507,7 +569,7 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                     e += (double)v.x * sr + (double)v.y * si;
                 }
                 __syncthreads();
-                store(2, t, tl, glam);
+                store(2, t, tl, glam, 0);
                 __syncthreads();
             }
         }
@@ -525,14 +587,20 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         for (int l = L - 1; l >= 0; --l) {
             clear_slots();
             if (resident) {
-                cz(0, 0, true);
+                build_phase(0, 0, l);
+                __syncthreads();
+                diag(true);
                 for_groups(0, n, GA, [&](int t0, int g) { adj_group(t0, g, 0, l); });
             } else {
-                tile_pass(2, true, [&](uint64_t t) {  // pass B': CZ ring, qubits n-1..b
-                    cz(2, t, true);
+                // pass B': the CZ ring and RZ set undone as the tile is loaded,
+                // then qubits b..n-1
+                tile_pass(2, true, 2, 0, [&](uint64_t t) {
+                    build_phase(2, t, l);
+                    __syncthreads();
+                }, [&](uint64_t) {
                     for_groups(b - h, b, GA, [&](int t0, int g) { adj_group(t0, g, h, l); });
                 });
-                tile_pass(1, true, [&](uint64_t) {  // pass A': qubits b-1..0
+                tile_pass(1, true, 0, 0, no_pre, [&](uint64_t) {  // pass A': qubits 0..b-1
                     for_groups(0, b, GA, [&](int t0, int g) { adj_group(t0, g, 0, l); });
                 });
             }
@@ -658,7 +726,9 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
     std::vector<int64_t> off(ncirc);
     int64_t total = 0;
     int nmax = 0;
-    int bused = 6;  // widest tile of the batch: the LDS each workgroup needs
+    // LDS amplitudes the most demanding circuit needs: its psi and lambda
+    // (resident, or tiles of its width)
+    uint32_t tile_elems = 0;
     for (size_t c = 0; c < ncirc; ++c) {
         if (nq[c] < 2 || nq[c] > 30 || layers[c] < 1 || 3 * nq[c] > kMaxSlots) {
             set_error(kErrInvalidArgument, "circuit %zu: unsupported shape (n=%d, layers=%d)", c, nq[c], layers[c]);
@@ -672,7 +742,7 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
             return -1;
         }
         if (tb < 6) tb = 6;
-        bused = std::max(bused, std::min(tb, nq[c] > tb ? tb : std::max(nq[c], 6)));
+        tile_elems = std::max<uint32_t>(tile_elems, 2u << std::min(tb, (int)nq[c]));
         cs[c] = BatchCircuit{nq[c], layers[c], tb, 0, total, (int64_t)c};
         off[c] = total;
         total += 2LL * nq[c] * layers[c];
@@ -692,7 +762,8 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
     const size_t elem = B.prec == kC128 ? 16 : 8;
     // LDS for the widest tile in use (psi and lambda) plus the per-wave
     // gradient slots; as many workgroups as fit a CU at that size run at once
-    const size_t lds = 2 * ((size_t)1 << bused) * elem + (size_t)(T / 64) * kMaxSlots * sizeof(double);
+    const size_t lds = (size_t)tile_elems * elem + (size_t)(T / 64) * kMaxSlots * sizeof(double) +
+                       (size_t)(kPhaseLo + kPhaseHi) * elem;
     const int gw = B.group_width ? B.group_width : iq_batch_max_group_width(B.prec);
     const void* kern = B.prec == kC128 ? (gw == 2 ? (const void*)k_batch_hea<double, 2> : (const void*)k_batch_hea<double, 3>)
                        : gw == 2       ? (const void*)k_batch_hea<float, 2>
@@ -707,7 +778,7 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
         !grow(reinterpret_cast<void**>(&B.params), B.param_cap, (size_t)total * 8) ||
         !grow(reinterpret_cast<void**>(&B.energies), B.energy_cap, ncirc * 8) ||
         !grow(&B.scratch, B.scratch_bytes, (size_t)nblk * 2 * scratch_elems * elem) ||
-        !grow(&B.rots, B.rots_cap, (size_t)(total / 2) * 24 * (B.prec == kC128 ? 8 : 4)))
+        !grow(&B.rots, B.rots_cap, (size_t)(total / 2) * kRotStride * (B.prec == kC128 ? 8 : 4)))
         return -1;
     if (B.grads) hipFree(B.grads);
     B.grads = nullptr;
@@ -729,7 +800,7 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
     hipEventRecord(e0, st);
 #define IQ_BATCH_LAUNCH(RT, GWV, CV)                                                                              \
     hipLaunchKernelGGL((k_batch_hea<RT, GWV>), dim3(nblk), dim3(T), lds, st, B.circs, (int)ncirc, (const RT*)B.rots, \
-                       B.params, B.grads, B.energies, (CV*)B.scratch, scratch_elems, B.queue, hx, bused)
+                       B.params, B.grads, B.energies, (CV*)B.scratch, scratch_elems, B.queue, hx, tile_elems)
     if (B.prec == kC128) {
         if (gw == 2) IQ_BATCH_LAUNCH(double, 2, double2);
         else IQ_BATCH_LAUNCH(double, 3, double2);
