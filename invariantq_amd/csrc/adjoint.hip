@@ -36,6 +36,7 @@
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "reduce.hpp"
+#include "split.hpp"
 
 namespace iqg {
 
@@ -206,6 +207,9 @@ constexpr int kPassSegMax = 32;
 
 struct PassSeg {
     double u[8];  // the run's product (forward) or its inverse (reverse sweep), re/im row-major
+    // What the register groups apply in a pass split into diagonals and real
+    // rotations (see PassDesc::greal): the run's part of the split form, or u.
+    double v[8];
     int tbit;     // tile bit of the run's qubit
     int slot;     // gradient slot (reverse)
 };
@@ -241,7 +245,17 @@ struct PassDesc {
     uint32_t cza[kEdgeMax], czb[kEdgeMax];
     uint32_t qshift[kShiftMax];
     uint32_t qmask[kShiftMax];
+    // Split form (groups with no qubit in common): every run is
+    // g diag(1, p) R diag(1, q), R a real rotation.  The group applied first
+    // takes the q of every other run as one phase per sub-cube (table tpre),
+    // the group applied last their p and the product of the g (table tpost);
+    // each group's runs apply v (bit g of greal: v is the real rotation
+    // [[v0, -v4], [v4, v0]]).  Tables hold 128 entries for tile bits 0-6, then
+    // 64 for bits 7-12, as offsets into the launch's table array (-1: none).
+    uint32_t greal;
+    int tpre, tpost;
 };
+
 
 __device__ __forceinline__ uint64_t pass_global(const PassDesc& d, uint64_t t, uint32_t j) {
     const uint64_t low = j & ((1u << d.r) - 1);
@@ -328,28 +342,79 @@ __device__ __forceinline__ void mat2(const R* u, cv<R>& a, cv<R>& c) {
     a.x = y0r; a.y = y0i; c.x = y1r; c.y = y1i;
 }
 
-// Forward: the G runs of a group (tile bits t0 .. t0+G-1), each its product
-// matrix, applied to sub-cubes of 2^G elements held in registers.
-template <typename R, int G>
-__device__ __forceinline__ void fwd_group(cv<R>* tile, uint32_t TS, int t0, const PassSeg* __restrict__ sg) {
+// (a, c) <- [[co, -si], [si, co]] (a, c): a real rotation of a complex pair.
+template <typename R>
+__device__ __forceinline__ void rot2(R co, R si, cv<R>& a, cv<R>& c) {
+    const cv<R> a0 = a, c0 = c;
+    a.x = fma(co, a0.x, -(si * c0.x));
+    a.y = fma(co, a0.y, -(si * c0.y));
+    c.x = fma(si, a0.x, co * c0.x);
+    c.y = fma(si, a0.y, co * c0.y);
+}
+
+// Phase of a sub-cube from a split pass's table: the group's own tile bits are
+// zero in base, and every diagonal in the table is 1 on a zero bit, so the
+// sub-cube's elements share tab[base & 127] * tab[128 + (base >> 7)].
+template <typename R>
+__device__ __forceinline__ cv<R> cube_phase(const cv<R>* tab, uint32_t base) {
+    const cv<R> a = tab[base & 127u], b = tab[128u + (base >> 7)];
+    cv<R> o;
+    o.x = a.x * b.x - a.y * b.y;
+    o.y = a.x * b.y + a.y * b.x;
+    return o;
+}
+
+template <typename R>
+__device__ __forceinline__ cv<R> cmulv(cv<R> v, cv<R> ph) {
+    cv<R> o;
+    o.x = v.x * ph.x - v.y * ph.y;
+    o.y = v.x * ph.y + v.y * ph.x;
+    return o;
+}
+
+// The G runs of a group (tile bits t0 .. t0+G-1) on sub-cubes of 2^G elements
+// held in registers: each run's v, real rotations when REAL; with `pre` the
+// sub-cube first takes its phase from that table, with `post` afterwards.
+template <typename R, int G, bool REAL>
+__device__ __forceinline__ void group_apply(cv<R>* tile, uint32_t TS, int t0, const PassSeg* __restrict__ sg,
+                                            const cv<R>* pre, const cv<R>* post) {
     constexpr uint32_t D = 1u << G;
     const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
-    R u[G][8];
+    R u[G][REAL ? 2 : 8];
 #pragma unroll
-    for (int i = 0; i < G; ++i)
+    for (int i = 0; i < G; ++i) {
+        if (REAL) {
+            u[i][0] = (R)sg[i].v[0];
+            u[i][1] = (R)sg[i].v[4];
+        } else {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) u[i][e] = (R)sg[i].u[e];
+            for (int e = 0; e < (REAL ? 2 : 8); ++e) u[i][e] = (R)sg[i].v[e];
+        }
+    }
     for (uint32_t s = tid; s < cubes; s += T) {
         const uint32_t base = ((s & ~lo) << G) | (s & lo);
         const uint32_t pb = lds_pos(base);  // lds_pos is linear over disjoint bits
         cv<R> x[D];
 #pragma unroll
         for (uint32_t m = 0; m < D; ++m) x[m] = tile[(pb ^ lds_pos(m << t0))];
+        if (pre) {
+            const cv<R> ph = cube_phase<R>(pre, base);
+#pragma unroll
+            for (uint32_t m = 0; m < D; ++m) x[m] = cmulv<R>(x[m], ph);
+        }
 #pragma unroll
         for (int i = 0; i < G; ++i)
 #pragma unroll
             for (uint32_t m = 0; m < D; ++m)
-                if (!(m & (1u << i))) mat2<R>(u[i], x[m], x[m | (1u << i)]);
+                if (!(m & (1u << i))) {
+                    if (REAL) rot2<R>(u[i][0], u[i][1], x[m], x[m | (1u << i)]);
+                    else mat2<R>(u[i], x[m], x[m | (1u << i)]);
+                }
+        if (post) {
+            const cv<R> ph = cube_phase<R>(post, base);
+#pragma unroll
+            for (uint32_t m = 0; m < D; ++m) x[m] = cmulv<R>(x[m], ph);
+        }
 #pragma unroll
         for (uint32_t m = 0; m < D; ++m) tile[(pb ^ lds_pos(m << t0))] = x[m];
     }
@@ -462,29 +527,6 @@ __device__ __forceinline__ void rev_overlaps(const cv<R>* tp, const cv<R>* tl, u
     }
 }
 
-// A group's inverses on lambda only.
-template <typename R, int G>
-__device__ __forceinline__ void lam_back(cv<R>* tl, uint32_t TS, int t0, const PassSeg* __restrict__ sg) {
-    constexpr uint32_t D = 1u << G;
-    const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
-    R u[G][8];
-    load_runs<R, G>(sg, u);
-    for (uint32_t s = tid; s < cubes; s += T) {
-        const uint32_t base = ((s & ~lo) << G) | (s & lo);
-        const uint32_t pb = lds_pos(base);  // lds_pos is linear over disjoint bits
-        cv<R> y[D];
-#pragma unroll
-        for (uint32_t m = 0; m < D; ++m) y[m] = tl[(pb ^ lds_pos(m << t0))];
-#pragma unroll
-        for (int i = 0; i < G; ++i)
-#pragma unroll
-            for (uint32_t m = 0; m < D; ++m)
-                if (!(m & (1u << i))) mat2<R>(u[i], y[m], y[m | (1u << i)]);
-#pragma unroll
-        for (uint32_t m = 0; m < D; ++m) tl[(pb ^ lds_pos(m << t0))] = y[m];
-    }
-}
-
 // Group width dispatch (register groups hold up to 4 complex64 or 3
 // complex128 qubits; the complex128 instantiation of 4 is never run).
 #define IQ_GROUP_SWITCH(gs, CALL)                                          \
@@ -495,24 +537,56 @@ __device__ __forceinline__ void lam_back(cv<R>* tl, uint32_t TS, int t0, const P
         default: if (sizeof(R) == 4) CALL(sizeof(R) == 4 ? 4 : 3); break;  \
     }
 
+// A pass's group g of a split-form sweep: its runs' v (real rotations when
+// greal says so), the pre table when g is applied first, the post table when
+// last (with one group, neither: the pass is not split).
+#define IQ_APPLY_GROUP(tile, g, first_g, last_g)                                                           \
+    do {                                                                                                   \
+        const cv<R>* pre_ = ((g) == (first_g) && d.tpre >= 0) ? ltab : nullptr;                            \
+        const cv<R>* post_ = ((g) == (last_g) && d.tpost >= 0) ? ltab + kPhaseTab : nullptr;                \
+        const PassSeg* sg_ = ps + d.gfirst[g];                                                             \
+        if ((d.greal >> (g)) & 1u) {                                                                       \
+            IQ_GROUP_SWITCH(d.gsize[g], IQ_GA_REAL)                                                        \
+        } else {                                                                                           \
+            IQ_GROUP_SWITCH(d.gsize[g], IQ_GA_CPLX)                                                        \
+        }                                                                                                  \
+    } while (0)
+#define IQ_GA_REAL(G) group_apply<R, G, true>(tile_, TS, d.gt0[g_], sg_, pre_, post_)
+#define IQ_GA_CPLX(G) group_apply<R, G, false>(tile_, TS, d.gt0[g_], sg_, pre_, post_)
+
+// A split pass's phase tables (pre, then post) from the launch's table array
+// into LDS, in the state's precision.
+template <typename R>
+__device__ __forceinline__ void load_phase_tables(const PassDesc& d, const double2* __restrict__ tabs, cv<R>* ltab) {
+    if (d.tpre < 0 && d.tpost < 0) return;
+    for (int i = threadIdx.x; i < 2 * kPhaseTab; i += blockDim.x) {
+        const int off = i < kPhaseTab ? d.tpre : d.tpost;
+        if (off < 0) continue;
+        const double2 v = tabs[off + i % kPhaseTab];
+        cv<R> o;
+        o.x = (R)v.x;
+        o.y = (R)v.y;
+        ltab[i] = o;
+    }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(512) k_tile_fwd(const cv<R>* __restrict__ src, cv<R>* __restrict__ dst, PassDesc d,
-                                                  const PassSeg* __restrict__ segs) {
+                                                  const PassSeg* __restrict__ segs, const double2* __restrict__ tabs) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cv<R>* tile = reinterpret_cast<cv<R>*>(smem);
     const uint32_t TS = 1u << d.ts_bits;
+    cv<R>* tile = reinterpret_cast<cv<R>*>(smem);
+    cv<R>* ltab = tile + TS;  // [2][kPhaseTab]
     const PassSeg* ps = segs + d.first;
+    load_phase_tables<R>(d, tabs, ltab);  // read after the first tile's barrier
     for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
         tile_in<R>(d, t, src, tile, TS);
         __syncthreads();
         for (int gi = 0; gi < d.ngroups; ++gi) {
             // a wave-uniform index: the descriptor reads below become scalar loads
-            const int g = __builtin_amdgcn_readfirstlane(gi);
-            const int t0 = d.gt0[g];
-            const PassSeg* sg = ps + d.gfirst[g];
-#define IQ_CALL(G) fwd_group<R, G>(tile, TS, t0, sg)
-            IQ_GROUP_SWITCH(d.gsize[g], IQ_CALL)
-#undef IQ_CALL
+            const int g_ = __builtin_amdgcn_readfirstlane(gi);
+            cv<R>* tile_ = tile;
+            IQ_APPLY_GROUP(tile_, g_, 0, d.ngroups - 1);
             __syncthreads();
         }
         // the stage's CZ set comes last: its signs go on with the store
@@ -558,15 +632,17 @@ __device__ __forceinline__ void flush_overlaps(const double (&acc)[kGroupRuns][3
 // exactly `nblk` blocks, each writing every run's partials, zeros included.
 template <typename R, bool STORED>
 __global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>* __restrict__ lam, PassDesc d,
-                                                  const PassSeg* __restrict__ segs, double* __restrict__ gpart,
-                                                  int nblk) {
+                                                  const PassSeg* __restrict__ segs, const double2* __restrict__ tabs,
+                                                  double* __restrict__ gpart, int nblk) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t T = blockDim.x, tid = threadIdx.x, TS = 1u << d.ts_bits;
     const int lane = tid & 63, wave = tid >> 6, nw = T >> 6;
     cv<R>* tp = reinterpret_cast<cv<R>*>(smem);
     cv<R>* tl = tp + TS;
     double* wred = reinterpret_cast<double*>(tl + TS);  // [nw][kPassSegMax][kSubMax]
+    cv<R>* ltab = reinterpret_cast<cv<R>*>(wred + nw * kPassSegMax * kSubMax);  // [2][kPhaseTab]
     for (int i = tid; i < nw * kPassSegMax * kSubMax; i += T) wred[i] = 0.0;
+    if (STORED && d.gundo == 0) load_phase_tables<R>(d, tabs, ltab);
     __syncthreads();
     const PassSeg* ps = segs + d.first;
     for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
@@ -590,12 +666,9 @@ __global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>
             }
             __syncthreads();
             for (int gi = d.ngroups - 1; gi >= 0; --gi) {
-                const int g = __builtin_amdgcn_readfirstlane(gi);
-                const int t0 = d.gt0[g], gs = d.gsize[g];
-                const PassSeg* sg = ps + d.gfirst[g];
-#define IQ_CALL(G) lam_back<R, G>(tl, TS, t0, sg)
-                IQ_GROUP_SWITCH(gs, IQ_CALL)
-#undef IQ_CALL
+                const int g_ = __builtin_amdgcn_readfirstlane(gi);
+                cv<R>* tile_ = tl;
+                IQ_APPLY_GROUP(tile_, g_, d.ngroups - 1, 0);
                 __syncthreads();
             }
         } else {
@@ -625,112 +698,136 @@ __global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>
     }
 }
 
-// Lambda's tile in with the pass's CZ signs, which also go into a bitmap
-// (bit j of word j / 32 set when element j's sign is odd) for psi's reads:
-// the 64 lanes of a wave hold 64 consecutive elements, so one ballot gives
-// two words.
+// Wave sums of one run's three overlap sums (FP32 per lane): a reduce-scatter
+// over (x, y, z, 0) at offsets 32 and 16 leaves one value per lane, offsets
+// 8..1 finish it, and lanes 0, 16, 32, 48 add their component to the wave's
+// FP64 slot (component 3 is the padding, zero).
 template <typename R>
-__device__ __forceinline__ void tile_in_signs(const PassDesc& d, uint64_t t, const cv<R>* __restrict__ src,
-                                              cv<R>* dst, uint32_t* bits, uint32_t TS, CzTile z) {
-    const uint32_t T = blockDim.x, tid = threadIdx.x, lane = tid & 63u;
-    for (uint32_t j0 = tid; j0 < TS; j0 += 16 * T) {
-        cv<R> v[16];
+__device__ __forceinline__ void flush_run(R tx, R ty, R tz, double* slot, int lane) {
+    const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0;
+    const R a0 = (h5 ? tz : tx) + __shfl_xor(h5 ? tx : tz, 32, 64);
+    const R a1 = (h5 ? R(0) : ty) + __shfl_xor(h5 ? ty : R(0), 32, 64);
+    R v = (h4 ? a1 : a0) + __shfl_xor(h4 ? a0 : a1, 16, 64);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = src[pass_global(d, t, min(j0 + u * T, TS - 1))];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const uint32_t j = j0 + u * T;
-            const bool odd = cz_odd(d, z, j);
-            v[u] = negate_if<R>(v[u], odd);
-            const uint64_t b = __ballot(odd);
-            if (lane == 0 && j < TS) {
-                bits[j >> 5] = (uint32_t)b;
-                bits[(j >> 5) + 1] = (uint32_t)(b >> 32);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-            if (j0 + u * T < TS) dst[lds_pos(j0 + u * T)] = v[u];
-    }
-}
-
-// Overlaps of a group's runs with psi's sub-cubes read from global memory
-// (the stored pass output, taking the pass's CZ signs from the bitmap) and
-// lambda's from LDS.  The tile's low r >= 4 bits are contiguous global bits,
-// so sixteen consecutive threads read 128 contiguous bytes or more.
-template <typename R, int G>
-__device__ __forceinline__ void rev_overlaps_global(const PassDesc& d, uint64_t t, const uint32_t* sbits,
-                                                    const cv<R>* __restrict__ psi, const cv<R>* tl, uint32_t TS,
-                                                    int t0, double (&acc)[kGroupRuns][3]) {
-    constexpr uint32_t D = 1u << G;
-    const uint32_t T = blockDim.x, tid = threadIdx.x, cubes = TS >> G, lo = (1u << t0) - 1;
-    const cv<R>* pt = psi + pass_global(d, t, 0);  // the tile number's bits
-    const uint32_t rmask = (1u << d.r) - 1, mmask = (1u << d.m) - 1;
-    // global offset of tile element j within the tile: linear over disjoint
-    // bits, so a cube's offsets are its base's plus a per-m constant
-    auto local = [&](uint32_t j) { return (uint64_t)(j & rmask) | ((uint64_t)((j >> d.r) & mmask) << d.g0); };
-    for (uint32_t s = tid; s < cubes; s += T) {
-        const uint32_t base = ((s & ~lo) << G) | (s & lo);
-        const uint32_t pb = lds_pos(base);  // lds_pos is linear over disjoint bits
-        const cv<R>* pc = pt + local(base);
-        cv<R> x[D], y[D];
-#pragma unroll
-        for (uint32_t m = 0; m < D; ++m) x[m] = pc[local(m << t0)];
-#pragma unroll
-        for (uint32_t m = 0; m < D; ++m) y[m] = tl[(pb ^ lds_pos(m << t0))];
-        if (d.ncz) {
-#pragma unroll
-            for (uint32_t m = 0; m < D; ++m) {
-                const uint32_t j = base | (m << t0);
-                x[m] = negate_if<R>(x[m], (sbits[j >> 5] >> (j & 31u)) & 1u);
-            }
-        }
-        sub_overlaps<R, G>(x, y, acc);
-    }
+    for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int c = (h5 ? 2 : 0) + (h4 ? 1 : 0);  // 0 x, 1 y, 2 z, 3 padding
+    if ((lane & 15) == 0 && c < 3) slot[c] += (double)v;
 }
 
 // Reverse of a forward pass whose output psi was stored and whose groups sit
 // on distinct qubits (no group undoes an earlier one): lambda alone is staged
 // in LDS, half the footprint of staging psi too, so two workgroups share a
-// compute unit and one's loads overlap the other's arithmetic.  Every
-// group's overlaps are taken at the pass's end (psi as stored, lambda as
-// loaded; the runs commute), then lambda goes back through the groups.
-// Partials as k_tile_rev.
+// compute unit and one's loads overlap the other's arithmetic.
+//   Overlaps: each thread reads its E = TS / 512 elements j = tid + 512 u of
+//   psi (the stored pass output) and lambda once, in one coalesced round,
+//   with the pass's CZ signs; lambda goes to LDS.  For every run, on tile bit
+//   b, the thread's terms pair its own psi_j with lambda_{j ^ 2^b} from LDS:
+//     T_X = sum_j Im(conj(lambda_{j^b}) psi_j),
+//     T_Y = sum_j (-1)^{j_b} Re(conj(lambda_{j^b}) psi_j),
+//     T_Z = sum_j (-1)^{j_b} Im(conj(lambda_j) psi_j),
+//   (the runs commute, so all are taken at the pass's end).  For b < 9 the
+//   sign is the thread's own; for b >= 9 it is u's, the same for every lane.
+//   Then lambda goes back through the groups.  Partials as k_tile_rev.
 template <typename R>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_tile_rev_lam(const cv<R>* __restrict__ psi, cv<R>* __restrict__ lam,
                                                       PassDesc d, const PassSeg* __restrict__ segs,
+                                                      const double2* __restrict__ tabs,
                                                       double* __restrict__ gpart, int nblk) {
     extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int E = sizeof(R) == 4 ? 16 : 8;  // elements per thread: 2^13 (2^12) tiles over 512 threads
     const uint32_t T = blockDim.x, tid = threadIdx.x, TS = 1u << d.ts_bits;
     const int lane = tid & 63, wave = tid >> 6, nw = T >> 6;
     cv<R>* tl = reinterpret_cast<cv<R>*>(smem);
     double* wred = reinterpret_cast<double*>(tl + TS);  // [nw][kPassSegMax][kSubMax]
-    uint32_t* sbits = reinterpret_cast<uint32_t*>(wred + nw * kPassSegMax * kSubMax);  // [TS / 32]
+    cv<R>* ltab = reinterpret_cast<cv<R>*>(wred + nw * kPassSegMax * kSubMax);  // [2][kPhaseTab]
     for (int i = tid; i < nw * kPassSegMax * kSubMax; i += T) wred[i] = 0.0;
+    load_phase_tables<R>(d, tabs, ltab);
     __syncthreads();
     const PassSeg* ps = segs + d.first;
+    const uint32_t pt = lds_pos(tid);  // element tid + 512 u sits at pt + 512 u (lds_pos moves bits 4-7 only)
     for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
-        if (d.ncz) tile_in_signs<R>(d, t, lam, tl, sbits, TS, cz_setup(d, t));
-        else tile_in<R>(d, t, lam, tl, TS);
+        cv<R> x[E];  // psi_j, j = tid + 512 u
+        // T_Z sums: wall = sum_u w_u, wz[i] = sum_u (-1)^{u_i} w_u (tile bit 9 + i),
+        // w_u = Im(conj(lambda_j) psi_j)
+        constexpr int UB = E == 16 ? 4 : 3;
+        R wall = 0, wz[UB];
+#pragma unroll
+        for (int i = 0; i < UB; ++i) wz[i] = 0;
+        {
+            const CzTile z = d.ncz ? cz_setup(d, t) : CzTile{0, 0};
+            // the element indices from an opaque copy of tid: per-element
+            // terms that do not depend on the tile are recomputed each tile,
+            // not kept live (spilled) across the tile loop
+            uint32_t me = tid;
+            asm volatile("" : "+v"(me));
+            // two rounds of E / 2 elements per array in flight (registers)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                cv<R> y[E / 2];
+#pragma unroll
+                for (int v = 0; v < E / 2; ++v) {
+                    const int u = h * (E / 2) + v;
+                    const uint64_t g = pass_global(d, t, me + (uint32_t)u * T);
+                    x[u] = psi[g];
+                    y[v] = lam[g];
+                }
+#pragma unroll
+                for (int v = 0; v < E / 2; ++v) {
+                    const int u = h * (E / 2) + v;
+                    if (d.ncz) {
+                        const bool odd = cz_odd(d, z, me + (uint32_t)u * T);
+                        x[u] = negate_if<R>(x[u], odd);
+                        y[v] = negate_if<R>(y[v], odd);
+                    }
+                    const R w = fma(y[v].x, x[u].y, -(y[v].y * x[u].x));
+                    wall += w;
+#pragma unroll
+                    for (int i = 0; i < UB; ++i) wz[i] += ((u >> i) & 1) ? -w : w;
+                    tl[pt + (uint32_t)u * T] = y[v];
+                }
+            }
+        }
         __syncthreads();
-        for (int gi = d.ngroups - 1; gi >= 0; --gi) {
-            // a wave-uniform index: the descriptor reads below become scalar loads
-            const int g = __builtin_amdgcn_readfirstlane(gi);
-            const int t0 = d.gt0[g], gs = d.gsize[g];
-            double acc[kGroupRuns][3] = {};
-#define IQ_CALL(G) rev_overlaps_global<R, G>(d, t, sbits, psi, tl, TS, t0, acc)
-            IQ_GROUP_SWITCH(gs, IQ_CALL)
-#undef IQ_CALL
-            flush_overlaps<R>(acc, gs, wred, wave, lane, d.gfirst[g]);
+        for (int k = 0; k < d.nseg; ++k) {
+            // a wave-uniform index: the run's tile bit is a scalar load
+            const int kk = __builtin_amdgcn_readfirstlane(k);
+            const int b = ps[kk].tbit;
+            R tx, ty, tz;
+            if (b < 9) {  // partner in another thread, same u; sign from the thread's own bit
+                const uint32_t q = pt ^ lds_pos(1u << b);
+#pragma unroll
+                for (int u = 0; u < E; ++u) {
+                    const cv<R> l = tl[q + (uint32_t)u * T];
+                    tx = u ? fma(l.x, x[u].y, tx) : l.x * x[u].y;
+                    tx = fma(-l.y, x[u].x, tx);
+                    ty = u ? fma(l.x, x[u].x, ty) : l.x * x[u].x;
+                    ty = fma(l.y, x[u].y, ty);
+                }
+                const bool up = (tid >> b) & 1u;
+                ty = up ? -ty : ty;
+                tz = up ? -wall : wall;
+            } else {  // partner u ^ 2^(b-9) of this thread; the sign is u's bit, the same on every lane
+                const uint32_t ub = 1u << (b - 9);
+                tx = ty = 0;
+#pragma unroll
+                for (int u = 0; u < E; ++u) {
+                    const cv<R> l = tl[pt + ((uint32_t)u ^ ub) * T];
+                    const R sg = ((uint32_t)u & ub) ? R(-1) : R(1);
+                    tx = fma(l.x, x[u].y, tx);
+                    tx = fma(-l.y, x[u].x, tx);
+                    ty = fma(sg, fma(l.x, x[u].x, l.y * x[u].y), ty);
+                }
+                tz = wz[0];
+#pragma unroll
+                for (int i = 1; i < UB; ++i) tz = (b - 9 == i) ? wz[i] : tz;
+            }
+            flush_run<R>(tx, ty, tz, wred + (wave * kPassSegMax + kk) * kSubMax, lane);
         }
         __syncthreads();
         for (int gi = d.ngroups - 1; gi >= 0; --gi) {
-            const int g = __builtin_amdgcn_readfirstlane(gi);
-            const int t0 = d.gt0[g], gs = d.gsize[g];
-            const PassSeg* sg = ps + d.gfirst[g];
-#define IQ_CALL(G) lam_back<R, G>(tl, TS, t0, sg)
-            IQ_GROUP_SWITCH(gs, IQ_CALL)
-#undef IQ_CALL
+            const int g_ = __builtin_amdgcn_readfirstlane(gi);
+            cv<R>* tile_ = tl;
+            IQ_APPLY_GROUP(tile_, g_, d.ngroups - 1, 0);
             __syncthreads();
         }
         tile_out<R>(d, t, tl, lam, TS);
@@ -1692,7 +1789,7 @@ struct Launch {
 // Passes of the whole tape in forward order (reverse = false) or the reverse
 // sweep's order, with their run descriptors.
 void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vector<PassSeg>& segv,
-                std::vector<Launch>& launches) {
+                std::vector<Launch>& launches, std::vector<double>& tabs) {
     const TileGeom g = tile_geom(c);
     struct Stage {
         std::vector<int> runs;
@@ -1727,8 +1824,8 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
         for (const auto& sub : sg.subs) m = mul(sub_matrix(c.gates[sub.gate], sub.which, params), m);
         if (reverse) m = dag(m);
         for (int i = 0; i < 4; ++i) {
-            ps.u[2 * i] = m.m[i].real();
-            ps.u[2 * i + 1] = m.m[i].imag();
+            ps.u[2 * i] = ps.v[2 * i] = m.m[i].real();
+            ps.u[2 * i + 1] = ps.v[2 * i + 1] = m.m[i].imag();
         }
         segv.push_back(ps);
     };
@@ -1822,6 +1919,10 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
                         l.d.gundo |= 1u << g2;
             const int k = l.d.m == 0 ? 0 : pass_of(g, c.n, l.d.g0);
             for (int ri : run_chunks[i]) fill(c.segs[ri], l.d, k);
+            split_pass(l.d, segv.data() + l.d.first, reverse, tabs);
+        } else {
+            l.d.greal = 0;
+            l.d.tpre = l.d.tpost = -1;
         }
         launches.push_back(l);
     }
@@ -2014,12 +2115,16 @@ bool plan_observable_tiles(ObsPlan& p, int n, int prec, uint64_t dim, const std:
     return true;
 }
 
-Circuit::DescSlot* upload_descs(Circuit& c, const std::vector<PassSeg>& segv) {
+// Run descriptors, then (at a 16-byte boundary, `tab_off` bytes in) the split
+// passes' phase tables as (re, im) pairs.
+Circuit::DescSlot* upload_descs(Circuit& c, const std::vector<PassSeg>& segv, const std::vector<double>& tabs,
+                                size_t& tab_off) {
     Circuit::DescSlot& d = c.dring[c.dnext];
     c.dnext = (c.dnext + 1) % 4;
     if (d.pending) hip_ok(hipEventSynchronize(d.ev), "descriptor wait");
     d.pending = false;
-    const size_t bytes = std::max<size_t>(16, (segv.size() * sizeof(PassSeg) + 15) & ~size_t(15));
+    tab_off = (segv.size() * sizeof(PassSeg) + 15) & ~size_t(15);
+    const size_t bytes = std::max<size_t>(16, (tab_off + tabs.size() * sizeof(double) + 15) & ~size_t(15));
     if (bytes > d.cap) {
         if (d.host) hipHostFree(d.host);
         if (d.dev) hipFree(d.dev);
@@ -2033,6 +2138,7 @@ Circuit::DescSlot* upload_descs(Circuit& c, const std::vector<PassSeg>& segv) {
     }
     if (!d.ev && !hip_ok(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming), "event")) return nullptr;
     std::memcpy(d.host, segv.data(), segv.size() * sizeof(PassSeg));
+    if (!tabs.empty()) std::memcpy(static_cast<char*>(d.host) + tab_off, tabs.data(), tabs.size() * sizeof(double));
     if (!hip_ok(launch_upload(d.dev, d.host_dev, bytes, c.eng.stream), "descriptor upload")) return nullptr;
     return &d;
 }
@@ -2108,10 +2214,13 @@ bool run_forward_tiled(Circuit& c, const double* params, bool store) {
     if (!e.reset_basis(0)) return false;
     std::vector<PassSeg> segv;
     std::vector<Launch> launches;
-    plan_tiles(c, params, false, segv, launches);
-    Circuit::DescSlot* ds = upload_descs(c, segv);
+    std::vector<double> tabs;
+    plan_tiles(c, params, false, segv, launches, tabs);
+    size_t tab_off = 0;
+    Circuit::DescSlot* ds = upload_descs(c, segv, tabs, tab_off);
     if (!ds) return false;
     const PassSeg* dsegs = static_cast<const PassSeg*>(ds->dev);
+    const double2* dtabs = reinterpret_cast<const double2*>(static_cast<const char*>(ds->dev) + tab_off);
     const size_t nl = launches.size(), bytes = e.dim * e.elem;
     const bool st = store && ensure_stored(c, nl);
     for (size_t i = 0; i < nl; ++i) {
@@ -2125,12 +2234,12 @@ bool run_forward_tiled(Circuit& c, const double* params, bool store) {
             if (!launch_two_qubit(c, c.segs[l.seg2], false, dst, nullptr)) return false;
             continue;
         }
-        const size_t lds = ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>);
+        const size_t lds = (((size_t)1 << l.d.ts_bits) + 2 * kPhaseTab) * sizeof(cv<R>);
         const unsigned grid = (unsigned)std::min<uint64_t>(l.d.ntiles, 2048);
         hipFuncSetAttribute((const void*)k_tile_fwd<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         TimedLaunch tl(c, kLaunchForward);
         hipLaunchKernelGGL(k_tile_fwd<R>, dim3(grid), dim3(512), lds, e.stream, (const cv<R>*)src, (cv<R>*)dst, l.d,
-                           dsegs);
+                           dsegs, dtabs);
         if (!hip_ok(hipGetLastError(), "tiled forward")) return false;
     }
     ds->pending = hip_ok(hipEventRecord(ds->ev, e.stream), "event record");
@@ -2143,10 +2252,13 @@ bool run_reverse_tiled(Circuit& c, const double* params) {
     Engine& e = c.eng;
     std::vector<PassSeg> segv;
     std::vector<Launch> launches;
-    plan_tiles(c, params, true, segv, launches);
-    Circuit::DescSlot* ds = upload_descs(c, segv);
+    std::vector<double> tabs;
+    plan_tiles(c, params, true, segv, launches, tabs);
+    size_t tab_off = 0;
+    Circuit::DescSlot* ds = upload_descs(c, segv, tabs, tab_off);
     if (!ds) return false;
     const PassSeg* dsegs = static_cast<const PassSeg*>(ds->dev);
+    const double2* dtabs = reinterpret_cast<const double2*>(static_cast<const char*>(ds->dev) + tab_off);
     // with the forward sweep's stored outputs, psi is read, never taken back
     const bool st = c.stored_valid;
     const size_t nl = launches.size();
@@ -2161,20 +2273,21 @@ bool run_reverse_tiled(Circuit& c, const double* params) {
         const size_t wred_bytes = (size_t)8 * kPassSegMax * kSubMax * sizeof(double);
         if (st && l.d.gundo == 0 && c.rev_lam_only) {
             // psi read from its stored slot, lambda alone in LDS
-            const size_t lds = ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>) + wred_bytes + ((size_t)1 << l.d.ts_bits) / 8;
+            const size_t lds = ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>) + wred_bytes + 2 * kPhaseTab * sizeof(cv<R>);
             hipFuncSetAttribute((const void*)k_tile_rev_lam<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             TimedLaunch tl(c, kLaunchReverse);
             hipLaunchKernelGGL(k_tile_rev_lam<R>, dim3(c.nblk), dim3(512), lds, e.stream,
-                               (const cv<R>*)pass_out(c, st, i, nl), (cv<R>*)c.lam, l.d, dsegs, c.gpart, (int)c.nblk);
+                               (const cv<R>*)pass_out(c, st, i, nl), (cv<R>*)c.lam, l.d, dsegs, dtabs, c.gpart,
+                               (int)c.nblk);
             if (!hip_ok(hipGetLastError(), "tiled reverse")) return false;
             continue;
         }
-        const size_t lds = 2 * ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>) + wred_bytes;
+        const size_t lds = 2 * ((size_t)1 << l.d.ts_bits) * sizeof(cv<R>) + wred_bytes + 2 * kPhaseTab * sizeof(cv<R>);
         auto kern = st ? k_tile_rev<R, true> : k_tile_rev<R, false>;
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         TimedLaunch tl(c, kLaunchReverse);
         hipLaunchKernelGGL(kern, dim3(c.nblk), dim3(512), lds, e.stream, (cv<R>*)pass_out(c, st, i, nl),
-                           (cv<R>*)c.lam, l.d, dsegs, c.gpart, (int)c.nblk);
+                           (cv<R>*)c.lam, l.d, dsegs, dtabs, c.gpart, (int)c.nblk);
         if (!hip_ok(hipGetLastError(), "tiled reverse")) return false;
     }
     c.stored_valid = false;
@@ -2349,7 +2462,8 @@ size_t iq_circuit_num_segments(void* cp) {
     const std::vector<double> zeros((size_t)np, 0.0);
     std::vector<PassSeg> segv;
     std::vector<Launch> launches;
-    plan_tiles(c, zeros.data(), false, segv, launches);
+    std::vector<double> tabs;
+    plan_tiles(c, zeros.data(), false, segv, launches, tabs);
     return launches.size();
 }
 

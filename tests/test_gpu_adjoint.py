@@ -238,10 +238,12 @@ def test_full_size_24q_adjoint_vs_fp64_parameter_shift(A):
 
 
 @pytest.mark.parametrize("precision,n", [("complex64", 24), ("complex128", 20), ("complex64", 17)])
-def test_lambda_only_reverse_tiles_bit_identical(A, precision, n):
-    """Reverse passes over stored outputs with lambda alone in LDS (psi read
-    from its slot) against both staged: the same sums in the same order, so
-    the same gradient bit for bit; and timed launches cover every pass."""
+def test_lambda_only_reverse_tiles_agree(A, precision, n):
+    """Reverse passes over stored outputs with lambda alone in LDS (each
+    thread pairs its own psi elements with lambda's partners in LDS) against
+    both staged (sub-cube sums): the same overlaps summed in another order, so
+    the same gradient to the precision's rounding; and timed launches cover
+    every pass."""
     layers = 4
     rng = np.random.default_rng(n)
     p = rng.uniform(0, 2 * np.pi, 2 * n * layers)
@@ -254,6 +256,8 @@ def test_lambda_only_reverse_tiles_bit_identical(A, precision, n):
     e1, g1 = c.gradient(p)
     kinds = [k for k, _ in c.launch_times()]
     c.set_timing(False)
-    assert e0 == e1 and np.array_equal(g0, g1)
+    tol = 1e-12 if precision == "complex128" else 2e-5
+    assert e0 == e1  # the energy comes from the forward sweep and the observable alone
+    assert np.linalg.norm(g1 - g0) / np.linalg.norm(g0) < tol
     assert kinds.count("reverse") == kinds.count("forward") > 0
     assert kinds.count("observable") == 1 and kinds.count("finalize") == 1
