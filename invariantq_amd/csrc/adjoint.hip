@@ -254,6 +254,12 @@ struct PassDesc {
     // 64 for bits 7-12, as offsets into the launch's table array (-1: none).
     uint32_t greal;
     int tpre, tpost;
+    // Register group: the last group, when it holds exactly tile bits 9 ..
+    // 9 + log2(E) - 1 (E = tile / 512 elements per thread).  Each thread's
+    // elements tid + 512 u are one sub-cube over those bits, so the group is
+    // applied in registers as the tile is stored, one LDS round trip fewer.
+    // Applied last in both sweeps (reverse: groups ng-2 .. 0, then it).  -1: none.
+    int greg;
 };
 
 
@@ -371,6 +377,53 @@ __device__ __forceinline__ cv<R> cmulv(cv<R> v, cv<R> ph) {
     o.y = v.x * ph.y + v.y * ph.x;
     return o;
 }
+
+// Tile out with the pass's register group (runs on tile bits 9 .. 9 + UB - 1,
+// UB = log2 E): each thread takes its elements tid + 512 u from LDS, applies
+// the group's runs (v; real rotations when REAL) to the sub-cube, then the
+// post table's phase (one value: the sub-cube's base is tid) and the CZ signs,
+// and stores.
+template <typename R, bool REAL>
+__device__ __forceinline__ void tile_out_reg(const PassDesc& d, uint64_t t, const cv<R>* src, cv<R>* __restrict__ dst,
+                                             const PassSeg* __restrict__ sg, const cv<R>* post, bool cz, CzTile z) {
+    constexpr int E = sizeof(R) == 4 ? 16 : 8, UB = sizeof(R) == 4 ? 4 : 3;
+    const uint32_t T = blockDim.x, tid = threadIdx.x, pt = lds_pos(tid);
+    R u[UB][REAL ? 2 : 8];
+#pragma unroll
+    for (int i = 0; i < UB; ++i) {
+        if (REAL) {
+            u[i][0] = (R)sg[i].v[0];
+            u[i][1] = (R)sg[i].v[4];
+        } else {
+#pragma unroll
+            for (int e = 0; e < (REAL ? 2 : 8); ++e) u[i][e] = (R)sg[i].v[e];
+        }
+    }
+    cv<R> x[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) x[k] = src[pt + (uint32_t)k * T];
+#pragma unroll
+    for (int i = 0; i < UB; ++i)
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            if (!(k & (1 << i))) {
+                if (REAL) rot2<R>(u[i][0], u[i][1], x[k], x[k | (1 << i)]);
+                else mat2<R>(u[i], x[k], x[k | (1 << i)]);
+            }
+    if (post) {
+        const cv<R> ph = cube_phase<R>(post, tid);
+#pragma unroll
+        for (int k = 0; k < E; ++k) x[k] = cmulv<R>(x[k], ph);
+    }
+    uint32_t me = tid;
+    asm volatile("" : "+v"(me));  // per-element terms recomputed per tile, not held across the tile loop
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const uint32_t j = me + (uint32_t)k * T;
+        dst[pass_global(d, t, j)] = cz ? negate_if<R>(x[k], cz_odd(d, z, j)) : x[k];
+    }
+}
+
 
 // The G runs of a group (tile bits t0 .. t0+G-1) on sub-cubes of 2^G elements
 // held in registers: each run's v, real rotations when REAL; with `pre` the
@@ -582,7 +635,8 @@ __global__ void __launch_bounds__(512) k_tile_fwd(const cv<R>* __restrict__ src,
     for (uint64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
         tile_in<R>(d, t, src, tile, TS);
         __syncthreads();
-        for (int gi = 0; gi < d.ngroups; ++gi) {
+        const int nlds = d.greg >= 0 ? d.ngroups - 1 : d.ngroups;  // the register group is the last
+        for (int gi = 0; gi < nlds; ++gi) {
             // a wave-uniform index: the descriptor reads below become scalar loads
             const int g_ = __builtin_amdgcn_readfirstlane(gi);
             cv<R>* tile_ = tile;
@@ -590,7 +644,15 @@ __global__ void __launch_bounds__(512) k_tile_fwd(const cv<R>* __restrict__ src,
             __syncthreads();
         }
         // the stage's CZ set comes last: its signs go on with the store
-        tile_out<R>(d, t, tile, dst, TS, d.ncz != 0, d.ncz ? cz_setup(d, t) : CzTile{0, 0});
+        const CzTile z = d.ncz ? cz_setup(d, t) : CzTile{0, 0};
+        if (d.greg >= 0) {
+            const int g = d.greg;
+            const cv<R>* post = d.tpost >= 0 ? ltab + kPhaseTab : nullptr;
+            if ((d.greal >> g) & 1u) tile_out_reg<R, true>(d, t, tile, dst, ps + d.gfirst[g], post, d.ncz != 0, z);
+            else tile_out_reg<R, false>(d, t, tile, dst, ps + d.gfirst[g], post, d.ncz != 0, z);
+        } else {
+            tile_out<R>(d, t, tile, dst, TS, d.ncz != 0, z);
+        }
         __syncthreads();
     }
 }
@@ -665,10 +727,20 @@ __global__ void __launch_bounds__(512) k_tile_rev(cv<R>* __restrict__ psi, cv<R>
                 flush_overlaps<R>(acc, gs, wred, wave, lane, d.gfirst[g]);
             }
             __syncthreads();
-            for (int gi = d.ngroups - 1; gi >= 0; --gi) {
+            // lambda back: groups ng-1 .. 0, or with a register group ng-2 .. 0
+            // and then it (the order the split form's tables assume)
+            const int nlds = d.greg >= 0 ? d.ngroups - 1 : d.ngroups;
+            const int fg = nlds - 1 >= 0 ? nlds - 1 : d.greg, lg = d.greg >= 0 ? d.greg : 0;
+            for (int gi = nlds - 1; gi >= 0; --gi) {
                 const int g_ = __builtin_amdgcn_readfirstlane(gi);
                 cv<R>* tile_ = tl;
-                IQ_APPLY_GROUP(tile_, g_, d.ngroups - 1, 0);
+                IQ_APPLY_GROUP(tile_, g_, fg, lg);
+                __syncthreads();
+            }
+            if (d.greg >= 0) {
+                const int g_ = d.greg;
+                cv<R>* tile_ = tl;
+                IQ_APPLY_GROUP(tile_, g_, fg, lg);
                 __syncthreads();
             }
         } else {
@@ -824,13 +896,24 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
             flush_run<R>(tx, ty, tz, wred + (wave * kPassSegMax + kk) * kSubMax, lane);
         }
         __syncthreads();
-        for (int gi = d.ngroups - 1; gi >= 0; --gi) {
+        // lambda back: groups ng-1 .. 0, or with a register group ng-2 .. 0 in
+        // LDS and then it in registers as the tile is stored
+        const int nlds = d.greg >= 0 ? d.ngroups - 1 : d.ngroups;
+        const int fg = nlds - 1 >= 0 ? nlds - 1 : d.greg, lg = d.greg >= 0 ? d.greg : 0;
+        for (int gi = nlds - 1; gi >= 0; --gi) {
             const int g_ = __builtin_amdgcn_readfirstlane(gi);
             cv<R>* tile_ = tl;
-            IQ_APPLY_GROUP(tile_, g_, d.ngroups - 1, 0);
+            IQ_APPLY_GROUP(tile_, g_, fg, lg);
             __syncthreads();
         }
-        tile_out<R>(d, t, tl, lam, TS);
+        if (d.greg >= 0) {
+            const int g = d.greg;
+            const cv<R>* post = (g == lg && d.tpost >= 0) ? ltab + kPhaseTab : nullptr;
+            if ((d.greal >> g) & 1u) tile_out_reg<R, true>(d, t, tl, lam, ps + d.gfirst[g], post, false, CzTile{0, 0});
+            else tile_out_reg<R, false>(d, t, tl, lam, ps + d.gfirst[g], post, false, CzTile{0, 0});
+        } else {
+            tile_out<R>(d, t, tl, lam, TS);
+        }
         __syncthreads();
     }
     for (int i = tid; i < d.nseg * kSubMax; i += T) {
@@ -1723,6 +1806,7 @@ PassDesc pass_desc(const TileGeom& g, int n, int k) {
     }
     d.ts_bits = d.r + d.m;
     d.ntiles = 1ULL << (n - d.ts_bits);
+    d.greg = d.tpre = d.tpost = -1;
     return d;
 }
 
@@ -1857,9 +1941,17 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
             });
             auto tb_of = [&](int ri) { return tile_bit(g, dk, k, c.segs[ri].q); };
             std::vector<std::pair<size_t, int>> groups;  // first entry of mine, size
-            for (size_t i = 0; i < mine.size();) {
+            // runs on exactly tile bits 9 .. 8 + gmax (one each, the pass's
+            // highest) form the register group, applied as the tile is stored
+            size_t nchain = mine.size();
+            bool reg = dk.ts_bits == 9 + gmax && mine.size() >= (size_t)gmax;
+            for (int q = 0; reg && q < gmax; ++q)
+                reg = tb_of(mine[mine.size() - gmax + q]) == 9 + q;
+            if (reg && mine.size() > (size_t)gmax) reg = tb_of(mine[mine.size() - gmax - 1]) < 9;
+            if (reg) nchain = mine.size() - gmax;
+            for (size_t i = 0; i < nchain;) {
                 size_t j = i + 1;
-                while (j < mine.size() && tb_of(mine[j]) == tb_of(mine[j - 1]) + 1) ++j;
+                while (j < nchain && tb_of(mine[j]) == tb_of(mine[j - 1]) + 1) ++j;
                 const int len = (int)(j - i), ng = (len + gmax - 1) / gmax;
                 size_t at = i;
                 for (int q = 0; q < ng; ++q) {
@@ -1869,6 +1961,7 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
                 }
                 i = j;
             }
+            if (reg) groups.push_back({nchain, gmax});
             Launch l;
             l.d = dk;
             std::vector<int> chunk;
@@ -1884,6 +1977,7 @@ void plan_tiles(const Circuit& c, const double* params, bool reverse, std::vecto
             };
             for (const auto& gr : groups) {
                 if (l.d.ngroups == kGroupMax || (int)chunk.size() + gr.second > kPassSegMax) emit();
+                if (reg && gr.first == nchain) l.d.greg = l.d.ngroups;
                 l.d.gt0[l.d.ngroups] = (uint32_t)tb_of(mine[gr.first]);
                 l.d.gsize[l.d.ngroups] = (uint32_t)gr.second;
                 l.d.gfirst[l.d.ngroups] = (uint32_t)chunk.size();

@@ -83,7 +83,10 @@ void split_pass(Desc& d, Seg* ps, bool reverse, std::vector<double>& tabs) {
     d.tpre = d.tpost = -1;
     const int ng = d.ngroups, nr = d.nseg;
     if (ng < 2 || d.gundo != 0 || d.ts_bits > 13) return;
-    const int first_g = reverse ? ng - 1 : 0, last_g = reverse ? 0 : ng - 1;
+    // groups in the order the sweep applies them: forward 0 .. ng-1; reverse
+    // ng-1 .. 0, or with a register group (the last, d.greg) ng-2 .. 0 then it
+    const int first_g = !reverse ? 0 : d.greg >= 0 ? ng - 2 : ng - 1;
+    const int last_g = !reverse ? ng - 1 : d.greg >= 0 ? d.greg : 0;
     std::vector<SplitRun> sp(nr);
     std::vector<int> grp(nr);
     for (int gi = 0; gi < ng; ++gi)

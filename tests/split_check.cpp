@@ -1,7 +1,8 @@
 // CPU check of invariantq_amd/csrc/split.hpp.  A tiled pass's runs (one-qubit
 // unitaries on distinct tile bits, in register groups of consecutive bits)
 // are put in split form and applied the way the tile kernels apply them:
-// groups in the sweep's order (forward 0..ng-1, reverse ng-1..0), the pre
+// groups in the sweep's order (forward 0..ng-1, reverse ng-1..0, or with a
+// register group, the last, ng-2..0 and then it), the pre
 // table's phase on each sub-cube before the first group's runs, each run's v
 // (a real rotation where greal says so), the post table's phase after the
 // last group's runs.  The result must equal every run's full matrix applied
@@ -9,6 +10,7 @@
 // and its inverse, Pauli, Hadamard and phase gates, and products of these.
 // Also checks that ansatz passes of three or more groups are split (the form
 // saves work there).  Prints "ok <passes> <split>" or the first failure.
+#include <algorithm>
 #include <cstdio>
 #include <random>
 
@@ -23,7 +25,7 @@ struct Seg {
 struct Desc {
     int ngroups = 0, nseg = 0, ts_bits = 0;
     uint32_t gundo = 0, gt0[16] = {}, gsize[16] = {}, gfirst[16] = {}, greal = 0;
-    int tpre = -1, tpost = -1;
+    int tpre = -1, tpost = -1, greg = -1;
 };
 
 using M = split_cplx[4];
@@ -121,6 +123,7 @@ int main() {
             i = j;
         }
         d.nseg = (int)segs.size();
+        if (rng() % 2) d.greg = d.ngroups - 1;  // the last group applied in registers, last in both sweeps
         std::vector<double> tabs;
         iqg::split_pass(d, segs.data(), reverse, tabs);
         ++passes;
@@ -144,9 +147,17 @@ int main() {
             const uint32_t h = 128u + (base >> 7);
             return a * split_cplx(t[2 * h], t[2 * h + 1]);
         };
-        const int first_g = reverse ? d.ngroups - 1 : 0, last_g = reverse ? 0 : d.ngroups - 1;
-        for (int k = 0; k < d.ngroups; ++k) {
-            const int g = reverse ? d.ngroups - 1 - k : k;
+        std::vector<int> order;
+        for (int k = 0; k < d.ngroups; ++k) order.push_back(k);
+        if (reverse) {
+            std::reverse(order.begin(), order.end());
+            if (d.greg >= 0) {  // ng-2 .. 0, then the register group
+                order.erase(order.begin());
+                order.push_back(d.greg);
+            }
+        }
+        const int first_g = order.front(), last_g = order.back();
+        for (int g : order) {
             const uint32_t gmask = ((1u << d.gsize[g]) - 1) << d.gt0[g];
             for (uint32_t base = 0; base < TS; ++base) {
                 if (base & gmask) continue;
