@@ -105,9 +105,15 @@ def autotune(runner: BatchRunner, specs: Sequence[Tuple[int, int, np.ndarray]], 
     Stage 1 (default fusion width): tile width in {max, max-1, max-2} (circuits
     wider than the tile take the two-pass tiled path; a width >= n keeps the
     circuit in LDS; widths below n/2 do not fit) x 256 or 512 threads per
-    workgroup; the fastest width per n and the thread count fastest overall.
-    Stage 2: with those, the fusion width (qubits per register pass of the
-    tile sweeps, 2 up to the precision's widest) timed over the whole sample.
+    workgroup, timed per n; for each thread count, the fastest width per n.
+    Stage 2: the thread count, each with its widths (and 256 threads with
+    every width below the widest, so that two workgroups share a compute
+    unit), timed on the whole batch (up to 16384 circuits; beyond, on the
+    per-n samples together): the launch sizes LDS for its widest tile, and
+    the mix of widths and the number of circuits per compute unit, not one
+    qubit count, decide how many workgroups share a compute unit.
+    Stage 3: with those, the fusion width (qubits per register pass of the
+    tile sweeps, 2 up to the precision's widest), timed the same way.
     Returns {"tile_bits": {n: width}, "block_threads": t, "group_width": g,
     "timings": ..., "cached": bool}.  The cache keeps one entry per (device,
     precision): tile widths of qubit counts tuned earlier are kept and new
@@ -125,8 +131,7 @@ def autotune(runner: BatchRunner, specs: Sequence[Tuple[int, int, np.ndarray]], 
                 "group_width": entry["group_width"], "timings": entry.get("timings", {}), "cached": True}
     bmax = runner.max_tile_bits
     timings: Dict[str, Dict[str, float]] = {}
-    best: Dict[int, int] = {}
-    per_threads = {256: 0.0, 512: 0.0}
+    best_by_t: Dict[int, Dict[int, int]] = {256: {}, 512: {}, -256: {}}  # -256: 256 threads, narrow tiles
     samples = {}
     for n in ns:
         sample = [s for s in specs if int(s[0]) == n][:per_n]
@@ -142,16 +147,29 @@ def autotune(runner: BatchRunner, specs: Sequence[Tuple[int, int, np.ndarray]], 
                 _, _, ms = runner.run(sample, hx, {n: w}, t)
                 res[f"{w}/{t}"] = ms
         timings[str(n)] = res
-        w_best = min(cands, key=lambda w: min(res[f"{w}/256"], res[f"{w}/512"]))
-        best[n] = w_best
-        for t in per_threads:
-            per_threads[t] += res[f"{w_best}/{t}"]
-    threads = min(per_threads, key=per_threads.get)
-    everything = [s for n in ns for s in samples[n]]
+        for t in (256, 512):
+            best_by_t[t][n] = min(cands, key=lambda w: res[f"{w}/{t}"])
+        # below the widest tile, two 256-thread workgroups fit a compute unit
+        narrow = [w for w in cands if w < bmax] or cands
+        best_by_t[-256][n] = min(narrow, key=lambda w: res[f"{w}/256"])
+    # stages 2 and 3 time the batch itself (its occupancy is what they tune),
+    # or for very large batches the per-n samples together
+    everything = list(specs) if len(specs) <= 16384 else [s for n in ns for s in samples[n]]
+    def steady(widths, threads, group_width=0):
+        # the second of two back-to-back launches: a launch right after a small
+        # one can run faster than the same launch repeated, as in use
+        runner.run(everything, hx, widths, threads, group_width=group_width)
+        return runner.run(everything, hx, widths, threads, group_width=group_width)[2]
+
+    t_ms = {}
+    for key_t, widths in best_by_t.items():
+        t_ms[key_t] = steady(widths, abs(key_t))
+    timings["block_threads"] = {("256 narrow" if t < 0 else str(t)): ms for t, ms in t_ms.items()}
+    choice = min(t_ms, key=t_ms.get)
+    threads, best = abs(choice), best_by_t[choice]
     gw_ms = {}
     for g in range(2, runner.max_group_width + 1):
-        runner.run(everything[:8], hx, best, threads, group_width=g)  # warm-up
-        _, _, gw_ms[g] = runner.run(everything, hx, best, threads, group_width=g)
+        gw_ms[g] = steady(best, threads, g)
     group_width = min(gw_ms, key=gw_ms.get)
     timings["group_width"] = {str(g): ms for g, ms in gw_ms.items()}
     out = {"tile_bits": best, "block_threads": threads, "group_width": group_width, "timings": timings,

@@ -8,8 +8,8 @@
 // H = sum_{q<n-1} Z_q Z_{q+1} + hx * sum_q X_q.  Parameters of layer l:
 // theta_q = p[2nl + q], phi_q = p[2nl + n + q] (the layout of iq_circuit_*).
 //
-// One workgroup owns one circuit at a time, taken from an atomic queue in the
-// order the host sorted (longest first).  With b = the circuit's tile width:
+// One workgroup owns one circuit at a time: workgroup b starts with circuit b
+// of the host's order (longest first), then takes the next from an atomic queue.  With b = the circuit's tile width:
 //   n <= b  psi and lambda live in LDS for the whole circuit;
 //   n >  b  they live in a per-workgroup HBM scratch and every ansatz layer is
 //           two tiled passes: pass A tiles the b low qubits (contiguous), pass B
@@ -128,9 +128,16 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
     cv<R>* gpsi = scratch + (uint64_t)blockIdx.x * 2 * scratch_elems;
     cv<R>* glam = gpsi + scratch_elems;
 
+    // Workgroup b's first circuit is circuit b (the host sorted them longest
+    // first), later ones come from the queue.  Were the first ones taken from
+    // the queue too, which long circuits end up sharing a compute unit would
+    // depend on the order the workgroups reach it (measured 180 against 211 ms
+    // on one batch of 4096, run to run).
+    bool first_round = true;
     for (;;) {
         __syncthreads();
-        if (tid == 0) s_next = atomicAdd(queue, 1);
+        if (tid == 0) s_next = first_round ? (int)blockIdx.x : (int)gridDim.x + atomicAdd(queue, 1);
+        first_round = false;
         __syncthreads();
         const int ci = s_next;
         if (ci >= ncirc) break;
@@ -155,6 +162,12 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         // elements; the default is 4 for complex64 and 3 for complex128)
         constexpr int GM = GW;
         constexpr int GA = GW;
+        // tile loads: U elements per thread per round (complex128: 8, so
+        // nothing spills); with TS = U * T a round is one sub-cube over tile
+        // bits RB .. RB + RU - 1 (the register group, when the group width is RU)
+        constexpr int U = sizeof(R) == 4 ? 16 : 8, RU = sizeof(R) == 4 ? 4 : 3;
+        const int RB = __builtin_ctz((uint32_t)T);
+        const bool regq = !resident && TS == (uint32_t)U * (uint32_t)T && GW == RU;
         // The layer's diagonal D = CZ ring * prod_q RZ_q(phi_q): amplitude g takes
         // (-1)^cz(g) * e^{i sum_q phi_q (g_q - 1/2)}.  The phase of tile element j
         // (j < TS) is ptab[j & 127] * ptab[128 + (j >> 7)] times the CZ sign,
@@ -175,15 +188,86 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             o.y = v.x * pi + v.y * pr;
             return o;
         };
+        // One sub-cube's reverse step: x (psi) and y (lambda) hold the 2^G
+        // elements that differ in tile bits t0 .. t0 + G - 1 (qubit = tile bit
+        // + qoff); the products go into az, aa, then RY^dagger on both.
+        auto adj_cube = [&](auto Gc, auto& x, auto& y, int t0, int qoff, int l, double* az,
+                            double* aa) __attribute__((always_inline)) {
+            constexpr int G = decltype(Gc)::value;
+            constexpr uint32_t D = 1u << G;
+            R w[D], wall;
+#pragma unroll
+            for (uint32_t m = 0; m < D; ++m) {
+                w[m] = fma(y[m].x, x[m].y, -(y[m].y * x[m].x));
+                wall = m ? wall + w[m] : w[m];
+            }
+#pragma unroll
+            for (int jb = G - 1; jb >= 0; --jb) {
+                const uint32_t bit = 1u << jb;
+                // one fused multiply-add per product term, chained through the sums
+                R w1 = 0, pa = 0;
+                bool first = true;
+#pragma unroll
+                for (uint32_t m = 0; m < D; ++m) {
+                    if (m & bit) continue;
+                    const cv<R>&a0 = x[m], &a1 = x[m | bit], &b0 = y[m], &b1 = y[m | bit];
+                    w1 = first ? w[m | bit] : w1 + w[m | bit];
+                    pa = first ? b1.x * a0.x : fma(b1.x, a0.x, pa);
+                    first = false;
+                    pa = fma(b1.y, a0.y, pa);
+                    pa = fma(-b0.x, a1.x, pa);
+                    pa = fma(-b0.y, a1.y, pa);
+                }
+                az[jb] += (double)(wall - 2 * w1);
+                aa[jb] += (double)pa;
+            }
+#pragma unroll
+            for (int jb = G - 1; jb >= 0; --jb) {
+                const R c = RT[(n * l + (t0 + jb + qoff)) * kRotStride], sn = RT[(n * l + (t0 + jb + qoff)) * kRotStride + 1];
+                const uint32_t bit = 1u << jb;
+#pragma unroll
+                for (uint32_t m = 0; m < D; ++m) {
+                    if (m & bit) continue;
+                    ry<R>(c, -sn, x[m], x[m | bit]);
+                    ry<R>(c, -sn, y[m], y[m | bit]);
+                }
+            }
+        };
+        // wave sums of a group's 2G products (qubit jb: z, a at 2 jb + 0, 1)
+        // into the wave's slots, reduce-scatter: offsets 32, 16 and 8 halve the
+        // values a lane carries (8 -> 4 -> 2 -> 1), offsets 4..1 finish the last
+        auto flush_adj = [&](auto Gc, const double* az, const double* aa, int t0, int qoff) __attribute__((always_inline)) {
+            constexpr int G = decltype(Gc)::value;
+            double acc[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int jb = k / 2;
+                acc[k] = jb >= G ? 0.0 : (k % 2 == 0 ? az[jb < G ? jb : 0] : aa[jb < G ? jb : 0]);
+            }
+            const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0, h3 = (lane & 8) != 0;
+            double v4[4], v2[2];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v4[i] = (h5 ? acc[4 + i] : acc[i]) + __shfl_xor(h5 ? acc[i] : acc[4 + i], 32, 64);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) v2[i] = (h4 ? v4[2 + i] : v4[i]) + __shfl_xor(h4 ? v4[i] : v4[2 + i], 16, 64);
+            double xs = (h3 ? v2[1] : v2[0]) + __shfl_xor(h3 ? v2[0] : v2[1], 8, 64);
+#pragma unroll
+            for (int off = 4; off > 0; off >>= 1) xs += __shfl_xor(xs, off, 64);
+            const int k = (h5 ? 4 : 0) + (h4 ? 2 : 0) + (h3 ? 1 : 0);
+            if ((lane & 7) == 0 && k < 2 * G) wred[wave * kMaxSlots + 2 * (t0 + qoff) + k] += xs;
+        };
         // --- tile transfer helpers: eight or sixteen elements per thread in
         // flight before any is written on, so a tile costs about one memory
         // latency, not one per element
         // psi (and lambda) tile in one round of loads: up to 16 elements per
         // thread per array in flight (64 KiB per CU for a 2^13 complex64 tile;
         // 8 for complex128)
-        // dg: 0 as stored, 1 with D, 2 with D^dagger on the way into LDS
-        auto load2 = [&](int kind, uint64_t t, bool two, int dg) __attribute__((always_inline)) {
-            constexpr int U = sizeof(R) == 4 ? 16 : 8;  // complex128: 8 in flight, so nothing spills
+        // dg: 0 as stored, 1 with D, 2 with D^dagger on the way into LDS.
+        // rop (register group, when TS = U * T: each thread's elements j0 + u T
+        // are one sub-cube over tile bits RB .. RB + RU - 1, RB = log2 T):
+        // 1 applies those qubits' RY (forward), 2 their reverse step (psi and
+        // lambda), in registers before the tile goes to LDS; qubit = tile bit + qoff.
+        auto load2 = [&](int kind, uint64_t t, bool two, int dg, int rop, int qoff, int l) __attribute__((always_inline)) {
             for (uint32_t j0 = tid; j0 < TS; j0 += U * T) {
                 cv<R> v[U], w[U];
 #pragma unroll
@@ -192,16 +276,37 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                     v[u] = gpsi[gi];
                     w[u] = two ? glam[gi] : v[u];
                 }
+                if (dg) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t j = j0 + u * T;
-                    if (j < TS) {
-                        if (dg) {
+                    for (int u = 0; u < U; ++u) {
+                        const uint32_t j = j0 + u * T;
+                        if (j < TS) {
                             R pr, pi;
                             phase_at(kind, t, j, dg == 2, pr, pi);
                             v[u] = cmul(v[u], pr, pi);
                             if (two) w[u] = cmul(w[u], pr, pi);
                         }
+                    }
+                }
+                if (rop == 1) {
+#pragma unroll
+                    for (int i = 0; i < RU; ++i) {
+                        const R c = RT[(n * l + (RB + i + qoff)) * kRotStride], sn = RT[(n * l + (RB + i + qoff)) * kRotStride + 1];
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            if (!(u & (1 << i))) ry<R>(c, sn, v[u], v[u | (1 << i)]);
+                    }
+                } else if (rop == 2) {
+                    double az[RU], aa[RU];
+#pragma unroll
+                    for (int i = 0; i < RU; ++i) az[i] = aa[i] = 0.0;
+                    adj_cube(std::integral_constant<int, RU>{}, v, w, RB, qoff, l, az, aa);
+                    flush_adj(std::integral_constant<int, RU>{}, az, aa, RB, qoff);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t j = j0 + u * T;
+                    if (j < TS) {
                         tp[P(j)] = v[u];
                         if (two) tl[P(j)] = w[u];
                     }
@@ -321,71 +426,14 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                     x[m] = tp[pb0 ^ P(m << t0)];
                     y[m] = tl[pb0 ^ P(m << t0)];
                 }
-                R w[D], wall;
-#pragma unroll
-                for (uint32_t m = 0; m < D; ++m) {
-                    w[m] = fma(y[m].x, x[m].y, -(y[m].y * x[m].x));
-                    wall = m ? wall + w[m] : w[m];
-                }
-#pragma unroll
-                for (int jb = G - 1; jb >= 0; --jb) {
-                    const uint32_t bit = 1u << jb;
-                    // one fused multiply-add per product term, chained through the sums
-                    R w1 = 0, pa = 0;
-                    bool first = true;
-#pragma unroll
-                    for (uint32_t m = 0; m < D; ++m) {
-                        if (m & bit) continue;
-                        const cv<R>&a0 = x[m], &a1 = x[m | bit], &b0 = y[m], &b1 = y[m | bit];
-                        w1 = first ? w[m | bit] : w1 + w[m | bit];
-                        pa = first ? b1.x * a0.x : fma(b1.x, a0.x, pa);
-                        first = false;
-                        pa = fma(b1.y, a0.y, pa);
-                        pa = fma(-b0.x, a1.x, pa);
-                        pa = fma(-b0.y, a1.y, pa);
-                    }
-                    az[jb] += (double)(wall - 2 * w1);
-                    aa[jb] += (double)pa;
-                }
-#pragma unroll
-                for (int jb = G - 1; jb >= 0; --jb) {
-                    const R c = RT[(n * l + (t0 + jb + qoff)) * kRotStride], sn = RT[(n * l + (t0 + jb + qoff)) * kRotStride + 1];
-                    const uint32_t bit = 1u << jb;
-#pragma unroll
-                    for (uint32_t m = 0; m < D; ++m) {
-                        if (m & bit) continue;
-                        ry<R>(c, -sn, x[m], x[m | bit]);
-                        ry<R>(c, -sn, y[m], y[m | bit]);
-                    }
-                }
+                adj_cube(Gc, x, y, t0, qoff, l, az, aa);
 #pragma unroll
                 for (uint32_t m = 0; m < D; ++m) {
                     tp[pb0 ^ P(m << t0)] = x[m];
                     tl[pb0 ^ P(m << t0)] = y[m];
                 }
             }
-            // wave sums of the 2G products (qubit jb: z, a at 2 jb + 0, 1),
-            // reduce-scatter: offsets 32, 16 and 8 halve the values a lane
-            // carries (8 -> 4 -> 2 -> 1), offsets 4..1 finish the last
-            {
-                double acc[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int jb = k / 2;
-                    acc[k] = jb >= G ? 0.0 : (k % 2 == 0 ? az[jb < G ? jb : 0] : aa[jb < G ? jb : 0]);
-                }
-                const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0, h3 = (lane & 8) != 0;
-                double v4[4], v2[2];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) v4[i] = (h5 ? acc[4 + i] : acc[i]) + __shfl_xor(h5 ? acc[i] : acc[4 + i], 32, 64);
-#pragma unroll
-                for (int i = 0; i < 2; ++i) v2[i] = (h4 ? v4[2 + i] : v4[i]) + __shfl_xor(h4 ? v4[i] : v4[2 + i], 16, 64);
-                double xs = (h3 ? v2[1] : v2[0]) + __shfl_xor(h3 ? v2[0] : v2[1], 8, 64);
-#pragma unroll
-                for (int off = 4; off > 0; off >>= 1) xs += __shfl_xor(xs, off, 64);
-                const int k = (h5 ? 4 : 0) + (h4 ? 2 : 0) + (h3 ? 1 : 0);
-                if ((lane & 7) == 0 && k < 2 * G) wred[wave * kMaxSlots + 2 * (t0 + qoff) + k] += xs;
-            }
+            flush_adj(Gc, az, aa, t0, qoff);
             __syncthreads();
         };
         // dispatch on the group width; callers pass 1 <= g <= GA
@@ -405,10 +453,11 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         // diagonal the loads and the stores take (as load2).  (Prefetching the
         // next tile into registers across `body` needs 64 more VGPRs per lane
         // and spilled at 512 threads.)
-        auto tile_pass = [&](int kind, bool two, int dl, int ds, auto&& pre, auto&& body) __attribute__((always_inline)) {
+        auto tile_pass = [&](int kind, bool two, int dl, int ds, int rop, int qoff, int l, auto&& pre,
+                             auto&& body) __attribute__((always_inline)) {
             for (uint64_t t = 0; t < ntA; ++t) {
                 pre(t);  // the last tile's users passed the barrier below
-                load2(kind, t, two, dl);
+                load2(kind, t, two, dl, rop, qoff, l);
                 __syncthreads();
                 body(t);
                 store(kind, t, tp, gpsi, ds);
@@ -497,13 +546,16 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                 __syncthreads();
                 diag(false);
             } else {
-                tile_pass(1, false, 0, 0, no_pre, [&](uint64_t) {  // pass A: RY on qubits 0..b-1
-                    for_groups(0, b, GM, [&](int t0, int g) { rot_group(t0, g, 0, l); });
+                // pass A: RY on qubits 0..b-1 (those on the register group's
+                // tile bits as the tile is loaded)
+                const bool ra = regq, rb = regq && b - h <= RB;
+                tile_pass(1, false, 0, 0, ra ? 1 : 0, 0, l, no_pre, [&](uint64_t) {
+                    for_groups(0, ra ? RB : b, GM, [&](int t0, int g) { rot_group(t0, g, 0, l); });
                 });
                 // pass B: RY on qubits b..n-1, then the RZ set and the CZ ring as
                 // the tile is stored (the loads' barrier orders the tables first)
-                tile_pass(2, false, 0, 1, [&](uint64_t t) { build_phase(2, t, l); }, [&](uint64_t) {
-                    for_groups(b - h, b, GM, [&](int t0, int g) { rot_group(t0, g, h, l); });
+                tile_pass(2, false, 0, 1, rb ? 1 : 0, h, l, [&](uint64_t t) { build_phase(2, t, l); }, [&](uint64_t) {
+                    for_groups(b - h, rb ? RB : b, GM, [&](int t0, int g) { rot_group(t0, g, h, l); });
                 });
             }
         }
@@ -594,14 +646,16 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             } else {
                 // pass B': the CZ ring and RZ set undone as the tile is loaded,
                 // then qubits b..n-1
-                tile_pass(2, true, 2, 0, [&](uint64_t t) {
+                const bool ra = regq, rb = regq && b - h <= RB;
+                tile_pass(2, true, 2, 0, rb ? 2 : 0, h, l, [&](uint64_t t) {
                     build_phase(2, t, l);
                     __syncthreads();
                 }, [&](uint64_t) {
-                    for_groups(b - h, b, GA, [&](int t0, int g) { adj_group(t0, g, h, l); });
+                    for_groups(b - h, rb ? RB : b, GA, [&](int t0, int g) { adj_group(t0, g, h, l); });
                 });
-                tile_pass(1, true, 0, 0, no_pre, [&](uint64_t) {  // pass A': qubits 0..b-1
-                    for_groups(0, b, GA, [&](int t0, int g) { adj_group(t0, g, 0, l); });
+                // pass A': qubits 0..b-1 (the register group's as the tile is loaded)
+                tile_pass(1, true, 0, 0, ra ? 2 : 0, 0, l, no_pre, [&](uint64_t) {
+                    for_groups(0, ra ? RB : b, GA, [&](int t0, int g) { adj_group(t0, g, 0, l); });
                 });
             }
             flush_slots(l);

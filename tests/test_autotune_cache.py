@@ -63,3 +63,24 @@ def test_later_tuning_merges_into_the_entry(tmp_path):
 def test_no_tile_width_fits():
     with pytest.raises(ValueError, match="no tile width fits"):
         batch.autotune(ScriptedRunner(), specs_for([30]), cache=None, per_n=2)
+
+
+class MixRunner(ScriptedRunner):
+    """Per circuit, width 13 is 5% faster than 12 at either thread count; but a
+    256-thread launch whose tiles are all narrower than 13 runs two
+    workgroups per compute unit and takes 0.8 of the time: only the
+    whole-sample stage sees that."""
+
+    def run(self, specs, hx=0.5, tile_bits=None, block_threads=0, num_blocks=0, group_width=0):
+        self.calls += 1
+        widths = [(tile_bits or {}).get(n, self.max_tile_bits) for n, _, _ in specs]
+        ms = sum(l * 2.0 ** n * 1e-6 * (0.95 if w == 13 else 1.0) for (n, l, _), w in zip(specs, widths))
+        if block_threads == 256 and max(widths) < 13:
+            ms *= 0.8
+        return np.zeros(len(specs)), [np.zeros(2 * n * l) for n, l, _ in specs], ms
+
+
+def test_thread_count_chosen_on_the_whole_sample():
+    t = batch.autotune(MixRunner(), specs_for([14, 16, 18]), cache=None, per_n=4)
+    assert t["block_threads"] == 256 and max(t["tile_bits"].values()) < 13
+    assert set(t["timings"]["block_threads"]) == {"256", "512", "256 narrow"}

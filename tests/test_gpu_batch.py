@@ -111,6 +111,28 @@ def test_group_widths_agree(M, precision, widths):
         r.run(specs[:1], group_width=5)
 
 
+@pytest.mark.parametrize("precision,tiles,threads", [("complex64", 12, 256), ("complex64", 13, 512),
+                                                     ("complex128", 11, 256), ("complex128", 12, 512)])
+def test_register_group_launches(M, precision, tiles, threads):
+    """Tiles of exactly (elements per thread) x (threads) amplitudes with the
+    widest fusion width: each thread's loaded elements form the sub-cube of the
+    highest tile bits, whose rotations (forward) and reverse steps run in
+    registers at load.  Circuits with 13-20 qubits (tiled, both passes on the
+    register group, and n = 17 where pass B has no other group) against the
+    single-circuit complex128 adjoint."""
+    adjoint, batch = M
+    rng = np.random.default_rng(tiles * 1000 + threads)
+    shapes = [(13, 3), (15, 2), (17, 2), (18, 1), (20, 1)]
+    specs = [(n, l, rng.uniform(0, 2 * np.pi, 2 * n * l)) for n, l in shapes if n <= 2 * tiles]
+    r = batch.BatchRunner(precision)
+    e, g, _ = r.run(specs, tile_bits={n: tiles for n, _, _ in specs}, block_threads=threads)
+    tol = 1e-10 if precision == "complex128" else 1e-4
+    for (n, l, p), ei, gi in zip(specs, e, g):
+        e_ref, g_ref = _single(adjoint, n, l, p)
+        assert abs(ei - e_ref) < tol * max(1.0, abs(e_ref)), (n, l)
+        assert np.linalg.norm(gi - g_ref) / np.linalg.norm(g_ref) < tol, (n, l)
+
+
 def test_batch_1024_circuits_every_size_stratified(M):
     """A launch of 1024 circuits with n in 10..20 and up to 16 layers (the
     batched workload's distribution), autotuned; two circuits per qubit count
