@@ -107,8 +107,8 @@ def autotune(runner: BatchRunner, specs: Sequence[Tuple[int, int, np.ndarray]], 
     circuit in LDS; widths below n/2 do not fit) x 256 or 512 threads per
     workgroup, timed per n; for each thread count, the fastest width per n.
     Stage 2: the thread count, each with its widths (and 256 threads with
-    every width below the widest, so that two workgroups share a compute
-    unit), timed on the whole batch (up to 16384 circuits; beyond, on the
+    every width below the widest, per n or all at the next-widest, so that
+    two workgroups share a compute unit), timed on the whole batch (up to 16384 circuits; beyond, on the
     per-n samples together): the launch sizes LDS for its widest tile, and
     the mix of widths and the number of circuits per compute unit, not one
     qubit count, decide how many workgroups share a compute unit.
@@ -152,6 +152,8 @@ def autotune(runner: BatchRunner, specs: Sequence[Tuple[int, int, np.ndarray]], 
         # below the widest tile, two 256-thread workgroups fit a compute unit
         narrow = [w for w in cands if w < bmax] or cands
         best_by_t[-256][n] = min(narrow, key=lambda w: res[f"{w}/256"])
+    # and every qubit count at the one width below the widest (where it fits)
+    best_by_t[-257] = {n: (min(n, bmax - 1) if 2 * (bmax - 1) >= n else best_by_t[256][n]) for n in ns}
     # stages 2 and 3 time the batch itself (its occupancy is what they tune),
     # or for very large batches the per-n samples together
     everything = list(specs) if len(specs) <= 16384 else [s for n in ns for s in samples[n]]
@@ -163,10 +165,11 @@ def autotune(runner: BatchRunner, specs: Sequence[Tuple[int, int, np.ndarray]], 
 
     t_ms = {}
     for key_t, widths in best_by_t.items():
-        t_ms[key_t] = steady(widths, abs(key_t))
-    timings["block_threads"] = {("256 narrow" if t < 0 else str(t)): ms for t, ms in t_ms.items()}
+        t_ms[key_t] = steady(widths, 256 if key_t < 0 else key_t)
+    labels = {256: "256", 512: "512", -256: "256 narrow", -257: "256 uniform"}
+    timings["block_threads"] = {labels[t]: ms for t, ms in t_ms.items()}
     choice = min(t_ms, key=t_ms.get)
-    threads, best = abs(choice), best_by_t[choice]
+    threads, best = (256 if choice < 0 else choice), best_by_t[choice]
     gw_ms = {}
     for g in range(2, runner.max_group_width + 1):
         gw_ms[g] = steady(best, threads, g)

@@ -824,8 +824,22 @@ int iq_batch_hea_tfim(void* bp, size_t ncirc, const int32_t* nq, const int32_t* 
                        : gw == 3       ? (const void*)k_batch_hea<float, 3>
                                        : (const void*)k_batch_hea<float, 4>;
     hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    // Workgroups per compute unit, from the kernel's VGPRs and LDS and gfx950's
+    // fixed resources (512 VGPRs per SIMD lane, 4 SIMDs, up to 8 waves per
+    // SIMD).  Not hipOccupancyMaxActiveBlocksPerMultiprocessor: with another
+    // HIP runtime in the process (PyTorch's) the device reports half the
+    // register file and the query returns one workgroup where two fit.
     int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, T, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    {
+        hipFuncAttributes fa{};
+        int smem_cu = 0;
+        hipDeviceGetAttribute(&smem_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+        if (hipFuncGetAttributes(&fa, kern) == hipSuccess && fa.numRegs > 0 && smem_cu > 0) {
+            const int vgprs = (fa.numRegs + 7) / 8 * 8, waves_simd = (T / 64 + 3) / 4;
+            const int by_regs = std::min(512 / vgprs, 8) / waves_simd, by_lds = smem_cu / (int)lds;
+            per_cu = std::max(1, std::min(by_regs, by_lds));
+        }
+    }
     const int nblk = num_blocks > 0 ? num_blocks : std::max(1, cus * per_cu);
     const uint64_t scratch_elems = nmax ? (1ULL << nmax) : 1;
     if (!grow(reinterpret_cast<void**>(&B.circs), B.circ_cap, ncirc * sizeof(BatchCircuit)) ||

@@ -83,4 +83,4 @@ class MixRunner(ScriptedRunner):
 def test_thread_count_chosen_on_the_whole_sample():
     t = batch.autotune(MixRunner(), specs_for([14, 16, 18]), cache=None, per_n=4)
     assert t["block_threads"] == 256 and max(t["tile_bits"].values()) < 13
-    assert set(t["timings"]["block_threads"]) == {"256", "512", "256 narrow"}
+    assert set(t["timings"]["block_threads"]) == {"256", "512", "256 narrow", "256 uniform"}
