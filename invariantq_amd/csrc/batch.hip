@@ -54,24 +54,39 @@ struct BatchCircuit {
 
 constexpr int kMaxSlots = 96;  // gradient slots per layer: 3 per qubit (n <= 30)
 
-// Per qubit and layer the kernel keeps (cos theta/2, sin theta/2) of the RY;
-// the layer's RZ set is applied as one phase per amplitude (see k_batch_hea).
-constexpr int kRotStride = 2;
+// Per qubit and layer the kernel keeps the RY in scaled form.  With c, s the
+// cosine and sine of half the angle, R0(u) = [[1, -u], [u, 1]], X the bit flip
+// and Z = diag(1, -1):
+//   mode 0 (|c| >= |s|): RY = c * R0(t),               t = s / c;
+//   mode 1 (|s| >  |c|): RY = s * (-Z) X R0(-t),       t = c / s;
+// so |t| <= 1 and R0 costs one fused multiply-add per output component (four
+// per pair, where the plain rotation takes a multiply and a fused
+// multiply-add each).  X is no arithmetic: a cube of amplitudes is stored with
+// that bit of its positions flipped.  -Z is diagonal and commutes with every
+// other qubit's rotation, so it joins the layer's diagonal phase.  The inverse
+// is RY^dagger = f X^m R0(-t) (-Z)^m (m = mode, f = c or s): -Z first (with
+// the inverse phase), then R0(-t), then the flip.  The factors f (|f| >= 1/sqrt 2)
+// are not applied: every amplitude meets each qubit's f once, so a layer's
+// rotations leave the whole state scaled by the product of its f, which the
+// layer's phase restores.
+// Entries per (layer, qubit): forward R0 parameter (t or -t), reverse (-t), f, m.
+constexpr int kRotStride = 4;
 // LDS phase tables: tile bits 0..6 and 7..12 (tiles of at most 2^13)
 constexpr int kPhaseLo = 128, kPhaseHi = 64;
 
-// (a, b) <- RY (a, b) with RY = [[c, -s], [s, c]] (a real rotation of a complex pair)
+// (a, b) <- R0(u) (a, b) = (a - u b, u a + b)
 template <typename R>
-__device__ __forceinline__ void ry(R c, R s, cv<R>& a, cv<R>& b) {
-    const cv<R> a0 = a, b0 = b;
-    a.x = fma(c, a0.x, -(s * b0.x));
-    a.y = fma(c, a0.y, -(s * b0.y));
-    b.x = fma(s, a0.x, c * b0.x);
-    b.y = fma(s, a0.y, c * b0.y);
+__device__ __forceinline__ void rot0(R u, cv<R>& a, cv<R>& b) {
+    const cv<R> a0 = a;
+    a.x = fma(-u, b.x, a0.x);
+    a.y = fma(-u, b.y, a0.y);
+    b.x = fma(u, a0.x, b.x);
+    b.y = fma(u, a0.y, b.y);
 }
 
-// One block per circuit: rots[(param_off / 2 + n*l + q) * kRotStride + (0, 1)]
-// = (cos, sin) of half the RY angle of qubit q in layer l, from FP64 angles.
+// One block per circuit: rots[(param_off / 2 + n*l + q) * kRotStride + (0..3)]
+// = (forward parameter, reverse parameter, f, mode) of the RY of qubit q in
+// layer l, from FP64 angles.
 template <typename R>
 __global__ void k_prepare_rots(const BatchCircuit* __restrict__ circs, const double* __restrict__ params,
                                R* __restrict__ rots) {
@@ -80,9 +95,14 @@ __global__ void k_prepare_rots(const BatchCircuit* __restrict__ circs, const dou
     for (int k = threadIdx.x; k < n * C.layers; k += blockDim.x) {
         const int l = k / n, q = k % n;
         const double th = params[C.param_off + 2 * n * l + q];
+        const double c = cos(0.5 * th), s = sin(0.5 * th);
+        const bool m1 = fabs(s) > fabs(c);
+        const double t = m1 ? c / s : s / c;
         R* o = rots + (C.param_off / 2 + k) * kRotStride;
-        o[0] = (R)cos(0.5 * th);
-        o[1] = (R)sin(0.5 * th);
+        o[0] = (R)(m1 ? -t : t);
+        o[1] = (R)(-t);
+        o[2] = (R)(m1 ? s : c);
+        o[3] = m1 ? (R)1 : (R)0;
     }
 }
 
@@ -139,7 +159,9 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         if (tid == 0) s_next = first_round ? (int)blockIdx.x : (int)gridDim.x + atomicAdd(queue, 1);
         first_round = false;
         __syncthreads();
-        const int ci = s_next;
+        // uniform: the circuit, its shape and its rotation table sit in SGPRs,
+        // and the per-qubit rotation mode is a scalar branch
+        const int ci = __builtin_amdgcn_readfirstlane(s_next);
         if (ci >= ncirc) break;
         const BatchCircuit C = circs[ci];
         const int n = C.n, L = C.layers;
@@ -147,12 +169,17 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         const bool resident = (n <= b);
         const uint32_t TS = 1u << b;
         const uint64_t N = 1ULL << n;
-        const R* RT = rots + (C.param_off / 2) * kRotStride;  // [l][q][cos, sin]
+        const R* RT = rots + (C.param_off / 2) * kRotStride;  // [l][q][t, mode, f, f']
         double* G = grads + C.param_off;
         const uint64_t ntA = resident ? 1 : (N >> b);
         const int h = n - b;
         cv<R>* const tp = tiles;       // psi's tile
         cv<R>* const tl = tiles + TS;  // lambda's
+        // Reverse sweep: psi and lambda in memory are the true ones divided by
+        // the f' of the qubits already undone in the layer, so their overlaps
+        // are off by the square of that product: sc_run, by which each group's
+        // sums are multiplied as they are flushed.
+        double sc_run = 1.0;
 
         // LDS position of tile element j: the low four index bits XOR the next
         // four, so sixteen lanes reading sixteen elements of a register group
@@ -223,20 +250,44 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             }
 #pragma unroll
             for (int jb = G - 1; jb >= 0; --jb) {
-                const R c = RT[(n * l + (t0 + jb + qoff)) * kRotStride], sn = RT[(n * l + (t0 + jb + qoff)) * kRotStride + 1];
+                const R u = RT[(n * l + (t0 + jb + qoff)) * kRotStride + 1];
                 const uint32_t bit = 1u << jb;
 #pragma unroll
                 for (uint32_t m = 0; m < D; ++m) {
                     if (m & bit) continue;
-                    ry<R>(c, -sn, x[m], x[m | bit]);
-                    ry<R>(c, -sn, y[m], y[m | bit]);
+                    rot0<R>(u, x[m], x[m | bit]);
+                    rot0<R>(u, y[m], y[m | bit]);
                 }
             }
+        };
+        // product of f^2 over the qubits of a group: the factor by which the
+        // squared scale of psi and lambda grows once the group is undone
+        auto group_scale = [&](int t0, int G, int qoff, int l) __attribute__((always_inline)) -> double {
+            double f2 = 1.0;
+            for (int jb = 0; jb < G; ++jb) {
+                const double f = (double)RT[(n * l + (t0 + jb + qoff)) * kRotStride + 2];
+                f2 *= f * f;
+            }
+            return f2;
+        };
+        // product of the layer's f
+        auto layer_factor = [&](int l) __attribute__((always_inline)) -> double {
+            double f = 1.0;
+            for (int q = 0; q < n; ++q) f *= (double)RT[(n * l + q) * kRotStride + 2];
+            return f;
+        };
+        // the group's mode-1 qubits as bits of the group's local index: the
+        // flip X applied as the group's cube is stored
+        auto group_flip = [&](int t0, int G, int qoff, int l) __attribute__((always_inline)) -> uint32_t {
+            uint32_t jm = 0;
+            for (int jb = 0; jb < G; ++jb)
+                if (RT[(n * l + (t0 + jb + qoff)) * kRotStride + 3] != (R)0) jm |= 1u << jb;
+            return jm;
         };
         // wave sums of a group's 2G products (qubit jb: z, a at 2 jb + 0, 1)
         // into the wave's slots, reduce-scatter: offsets 32, 16 and 8 halve the
         // values a lane carries (8 -> 4 -> 2 -> 1), offsets 4..1 finish the last
-        auto flush_adj = [&](auto Gc, const double* az, const double* aa, int t0, int qoff) __attribute__((always_inline)) {
+        auto flush_adj = [&](auto Gc, const double* az, const double* aa, int t0, int qoff, int l) __attribute__((always_inline)) {
             constexpr int G = decltype(Gc)::value;
             double acc[8];
 #pragma unroll
@@ -254,7 +305,8 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
 #pragma unroll
             for (int off = 4; off > 0; off >>= 1) xs += __shfl_xor(xs, off, 64);
             const int k = (h5 ? 4 : 0) + (h4 ? 2 : 0) + (h3 ? 1 : 0);
-            if ((lane & 7) == 0 && k < 2 * G) wred[wave * kMaxSlots + 2 * (t0 + qoff) + k] += xs;
+            if ((lane & 7) == 0 && k < 2 * G) wred[wave * kMaxSlots + 2 * (t0 + qoff) + k] += xs * sc_run;
+            sc_run *= group_scale(t0, G, qoff, l);
         };
         // --- tile transfer helpers: eight or sixteen elements per thread in
         // flight before any is written on, so a tile costs about one memory
@@ -288,27 +340,29 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                         }
                     }
                 }
+                // the register group's flips: element u goes to u ^ jm
+                const uint32_t pjm = rop ? P(group_flip(RB, RU, qoff, l) << RB) : 0u;
                 if (rop == 1) {
 #pragma unroll
                     for (int i = 0; i < RU; ++i) {
-                        const R c = RT[(n * l + (RB + i + qoff)) * kRotStride], sn = RT[(n * l + (RB + i + qoff)) * kRotStride + 1];
+                        const R u0 = RT[(n * l + (RB + i + qoff)) * kRotStride];
 #pragma unroll
                         for (int u = 0; u < U; ++u)
-                            if (!(u & (1 << i))) ry<R>(c, sn, v[u], v[u | (1 << i)]);
+                            if (!(u & (1 << i))) rot0<R>(u0, v[u], v[u | (1 << i)]);
                     }
                 } else if (rop == 2) {
                     double az[RU], aa[RU];
 #pragma unroll
                     for (int i = 0; i < RU; ++i) az[i] = aa[i] = 0.0;
                     adj_cube(std::integral_constant<int, RU>{}, v, w, RB, qoff, l, az, aa);
-                    flush_adj(std::integral_constant<int, RU>{}, az, aa, RB, qoff);
+                    flush_adj(std::integral_constant<int, RU>{}, az, aa, RB, qoff, l);
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t j = j0 + u * T;
                     if (j < TS) {
-                        tp[P(j)] = v[u];
-                        if (two) tl[P(j)] = w[u];
+                        tp[P(j) ^ pjm] = v[u];
+                        if (two) tl[P(j) ^ pjm] = w[u];
                     }
                 }
             }
@@ -360,6 +414,7 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             constexpr int G = decltype(Gc)::value;
             constexpr uint32_t D = 1u << G;
             const uint32_t cubes = TS >> G, lo = (1u << t0) - 1;
+            const uint32_t pj = P(group_flip(t0, G, qoff, l) << t0);
             for (uint32_t s = tid; s < cubes; s += T) {
                 const uint32_t base = ((s & ~lo) << G) | (s & lo);
                 const uint32_t pb0 = P(base);  // P is linear over disjoint bits
@@ -368,13 +423,14 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                 for (uint32_t m = 0; m < D; ++m) x[m] = tp[pb0 ^ P(m << t0)];
 #pragma unroll
                 for (int jb = 0; jb < G; ++jb) {
-                    const R c = RT[(n * l + (t0 + jb + qoff)) * kRotStride], sn = RT[(n * l + (t0 + jb + qoff)) * kRotStride + 1];
+                    const R u0 = RT[(n * l + (t0 + jb + qoff)) * kRotStride];
 #pragma unroll
                     for (uint32_t m = 0; m < D; ++m)
-                        if (!(m & (1u << jb))) ry<R>(c, sn, x[m], x[m | (1u << jb)]);
+                        if (!(m & (1u << jb))) rot0<R>(u0, x[m], x[m | (1u << jb)]);
                 }
+                const uint32_t pbs = pb0 ^ pj;  // stored with the mode-1 bits flipped
 #pragma unroll
-                for (uint32_t m = 0; m < D; ++m) tp[pb0 ^ P(m << t0)] = x[m];
+                for (uint32_t m = 0; m < D; ++m) tp[pbs ^ P(m << t0)] = x[m];
             }
             __syncthreads();
         };
@@ -417,6 +473,7 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             double az[G], aa[G];
 #pragma unroll
             for (int jb = 0; jb < G; ++jb) az[jb] = aa[jb] = 0.0;
+            const uint32_t pj = P(group_flip(t0, G, qoff, l) << t0);
             for (uint32_t s = tid; s < cubes; s += T) {
                 const uint32_t base = ((s & ~lo) << G) | (s & lo);
                 const uint32_t pb0 = P(base);  // P is linear over disjoint bits
@@ -427,13 +484,14 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                     y[m] = tl[pb0 ^ P(m << t0)];
                 }
                 adj_cube(Gc, x, y, t0, qoff, l, az, aa);
+                const uint32_t pbs = pb0 ^ pj;  // stored with the mode-1 bits flipped
 #pragma unroll
                 for (uint32_t m = 0; m < D; ++m) {
-                    tp[pb0 ^ P(m << t0)] = x[m];
-                    tl[pb0 ^ P(m << t0)] = y[m];
+                    tp[pbs ^ P(m << t0)] = x[m];
+                    tl[pbs ^ P(m << t0)] = y[m];
                 }
             }
-            flush_adj(Gc, az, aa, t0, qoff);
+            flush_adj(Gc, az, aa, t0, qoff, l);
             __syncthreads();
         };
         // dispatch on the group width; callers pass 1 <= g <= GA
@@ -469,24 +527,34 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         // The tables of phase_at, made in FP64 (angle sums, then sincos) for the
         // tile's bit-to-qubit map, the second with the angles of the qubits the
         // tile number fixes folded in.
-        auto build_phase = [&](int kind, uint64_t t, int l) __attribute__((always_inline)) {
+        auto build_phase = [&](int kind, uint64_t t, int l, double scale) __attribute__((always_inline)) {
             const double* ph = params + C.param_off + 2 * n * l + n;
             const int lo = 2 * b - n;  // kind 2: tile bits [0, lo) on qubits [0, lo), the rest on [b, n)
             const int nb = kind == 0 ? n : b;
             auto qubit = [&](int k) { return kind == 0 || k < lo ? k : k - lo + b; };
             const uint32_t na = TS < 128u ? TS : 128u, nbt = TS > 128u ? TS >> 7 : 1u;
             for (uint32_t i = tid; i < na + nbt; i += T) {
+                // qubit q at bit value g: phi_q (g - 1/2), and pi where a mode-1
+                // rotation's -Z meets g = 0
+                auto term = [&](int q, uint32_t g) {
+                    const bool m1 = RT[(n * l + q) * kRotStride + 3] != (R)0;
+                    return ph[q] * ((double)g - 0.5) + (m1 && !g ? M_PI : 0.0);
+                };
                 double ang = 0.0;
                 if (i < na) {
-                    for (int k = 0; k < 7 && k < nb; ++k) ang += ph[qubit(k)] * ((double)((i >> k) & 1u) - 0.5);
+                    for (int k = 0; k < 7 && k < nb; ++k) ang += term(qubit(k), (i >> k) & 1u);
                 } else {
                     const uint32_t e = i - na;
-                    for (int k = 7; k < nb; ++k) ang += ph[qubit(k)] * ((double)((e >> (k - 7)) & 1u) - 0.5);
+                    for (int k = 7; k < nb; ++k) ang += term(qubit(k), (e >> (k - 7)) & 1u);
                     if (kind == 2)  // tile number bit k on qubit lo + k
-                        for (int k = 0; k < n - b; ++k) ang += ph[lo + k] * ((double)((t >> k) & 1u) - 0.5);
+                        for (int k = 0; k < n - b; ++k) ang += term(lo + k, (uint32_t)(t >> k) & 1u);
                 }
                 double sv, cvl;
                 sincos(ang, &sv, &cvl);
+                if (i < na) {  // the state's pending scale, once per phase
+                    sv *= scale;
+                    cvl *= scale;
+                }
                 cv<R> v;
                 v.x = (R)cvl;
                 v.y = (R)sv;
@@ -516,7 +584,8 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                     va += wred[w * kMaxSlots + 2 * q + 1];
                 }
                 G[2 * n * l + n + q] = z;
-                G[2 * n * l + q] = va;
+                // a mode-1 rotation's -Z, applied before the overlaps, turns Y into -Y
+                G[2 * n * l + q] = RT[(n * l + q) * kRotStride + 3] != (R)0 ? -va : va;
             }
             __syncthreads();
         };
@@ -540,9 +609,12 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             __syncthreads();
         }
         for (int l = 0; l < L; ++l) {
+            // the layer's rotations leave the state divided by ff, restored
+            // with the diagonal's phase
+            const double ff = layer_factor(l);
             if (resident) {
                 for_groups(0, n, GM, [&](int t0, int g) { rot_group(t0, g, 0, l); });
-                build_phase(0, 0, l);
+                build_phase(0, 0, l, ff);
                 __syncthreads();
                 diag(false);
             } else {
@@ -554,7 +626,7 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                 });
                 // pass B: RY on qubits b..n-1, then the RZ set and the CZ ring as
                 // the tile is stored (the loads' barrier orders the tables first)
-                tile_pass(2, false, 0, 1, rb ? 1 : 0, h, l, [&](uint64_t t) { build_phase(2, t, l); }, [&](uint64_t) {
+                tile_pass(2, false, 0, 1, rb ? 1 : 0, h, l, [&](uint64_t t) { build_phase(2, t, l, ff); }, [&](uint64_t) {
                     for_groups(b - h, rb ? RB : b, GM, [&](int t0, int g) { rot_group(t0, g, h, l); });
                 });
             }
@@ -638,23 +710,32 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         // --- reverse sweep
         for (int l = L - 1; l >= 0; --l) {
             clear_slots();
+            // psi and lambda come divided by the later layer's product of f',
+            // restored with D^dagger's phase
+            const double fpend = l + 1 < L ? layer_factor(l + 1) : 1.0;
             if (resident) {
-                build_phase(0, 0, l);
+                build_phase(0, 0, l, fpend);
                 __syncthreads();
                 diag(true);
+                sc_run = 1.0;
                 for_groups(0, n, GA, [&](int t0, int g) { adj_group(t0, g, 0, l); });
             } else {
                 // pass B': the CZ ring and RZ set undone as the tile is loaded,
                 // then qubits b..n-1
                 const bool ra = regq, rb = regq && b - h <= RB;
+                // every tile runs the same sequence of groups: sc_run restarts
+                // at the pass's value for each tile
+                double sc0 = 1.0;
                 tile_pass(2, true, 2, 0, rb ? 2 : 0, h, l, [&](uint64_t t) {
-                    build_phase(2, t, l);
+                    build_phase(2, t, l, fpend);
                     __syncthreads();
+                    sc_run = sc0;
                 }, [&](uint64_t) {
                     for_groups(b - h, rb ? RB : b, GA, [&](int t0, int g) { adj_group(t0, g, h, l); });
                 });
                 // pass A': qubits 0..b-1 (the register group's as the tile is loaded)
-                tile_pass(1, true, 0, 0, ra ? 2 : 0, 0, l, no_pre, [&](uint64_t) {
+                sc0 = sc_run;
+                tile_pass(1, true, 0, 0, ra ? 2 : 0, 0, l, [&](uint64_t) { sc_run = sc0; }, [&](uint64_t) {
                     for_groups(0, ra ? RB : b, GA, [&](int t0, int g) { adj_group(t0, g, 0, l); });
                 });
             }

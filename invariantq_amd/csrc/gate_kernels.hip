@@ -597,9 +597,9 @@ inline bool mfma_whole_lines(const MfmaArgs& a, bool f64) {
 // more waves share a SIMD.  Sets with three or more targets inside a line
 // (not streamed) measured 6.34 ms with four batches, 131072 waves, rotation 2
 // when a target lies at qubit 10 or above and 0 otherwise, against 7.09 ms
-// with the streaming defaults, so they keep those.  complex64: 262144 waves,
-// two batches when a target lies at qubit 10 or above, else four, no
-// rotation (any rotation lost, 4.75 vs 5.28 TB/s).  complex64
+// with the streaming defaults, so they keep those.  complex64: two batches
+// and 524288 waves when a target lies at qubit 10 or above, else four
+// batches and 262144 waves, no rotation (any rotation lost, 4.75 vs 5.28 TB/s).  complex64
 // lanes moving two adjacent amplitudes per 16-byte access (the even and odd
 // groups of two batches, where qubit 0 is not a target) halve the memory
 // instructions but measured 5.28 against 5.38 TB/s at every batch count, grid
@@ -611,10 +611,11 @@ struct MfmaDefaults {
 };
 
 inline MfmaDefaults mfma_defaults(const MfmaArgs& a, bool f64) {
-    // complex64 at 30 qubits: sets with a target at qubit 10 or above 5.44
-    // TB/s with two batches against 5.30 with four; sets within qubits 0-9
-    // 5.33 with four against 5.30 (262144 waves either way)
-    if (!f64) return a.pos[4] >= 10 ? MfmaDefaults{2, 262144} : MfmaDefaults{4, 262144};
+    // complex64 at 30 qubits: sets with a target at qubit 10 or above 5.49
+    // TB/s with two batches and 524288 waves, 5.42 with 262144 and 5.35 with
+    // four batches; sets within qubits 0-9 5.47 with four batches and 262144
+    // waves, 5.42 with two
+    if (!f64) return a.pos[4] >= 10 ? MfmaDefaults{2, 524288} : MfmaDefaults{4, 262144};
     return mfma_whole_lines(a, f64) ? MfmaDefaults{2, 524288} : MfmaDefaults{4, 131072};
 }
 
