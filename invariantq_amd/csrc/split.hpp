@@ -144,23 +144,29 @@ void split_pass(Desc& d, Seg* ps, bool reverse, std::vector<double>& tabs) {
     for (int gi = 0; gi < ng; ++gi)
         if (gi == first_g ? first_real : gi == last_g ? last_real : true) d.greal |= 1u << gi;
     // tables: entry i of the low part is the product over runs (outside the
-    // group that applies it) on tile bits 0-6 set in i; the high part, bits 7-12
+    // group that applies it) on tile bits 0-6 set in i; the high part, bits
+    // 7-12.  With fb[b] the product of those runs' factors on tile bit b, an
+    // entry is the entry without its lowest set bit times that bit's fb: one
+    // complex product per entry.
     auto table = [&](bool post) {
         const size_t at = tabs.size() / 2;
         tabs.resize(tabs.size() + 2 * kPhaseTab, 0.0);
         double* t = tabs.data() + 2 * at;
+        split_cplx fb[13];
+        for (auto& f : fb) f = 1.0;
+        for (int r = 0; r < nr; ++r) {
+            if (grp[r] == (post ? last_g : first_g)) continue;
+            const int tb = ps[r].tbit;
+            if (tb >= 0 && tb < 13) fb[tb] *= post ? sp[r].p : sp[r].q;
+        }
+        split_cplx e[kPhaseTab];
+        e[0] = (post ? g_in_post : g_in_pre) ? gall : split_cplx(1.0);
+        for (int i = 1; i < 128; ++i) e[i] = e[i & (i - 1)] * fb[__builtin_ctz((unsigned)i)];
+        e[128] = 1.0;
+        for (int i = 1; i < 64; ++i) e[128 + i] = e[128 + (i & (i - 1))] * fb[7 + __builtin_ctz((unsigned)i)];
         for (int i = 0; i < kPhaseTab; ++i) {
-            const bool hi = i >= 128;
-            const uint32_t idx = hi ? (uint32_t)(i - 128) : (uint32_t)i;
-            split_cplx v = !hi && (post ? g_in_post : g_in_pre) ? gall : split_cplx(1.0);
-            for (int r = 0; r < nr; ++r) {
-                if (grp[r] == (post ? last_g : first_g)) continue;
-                const int tb = ps[r].tbit - (hi ? 7 : 0);
-                if (tb < 0 || tb >= (hi ? 6 : 7) || !((idx >> tb) & 1u)) continue;
-                v *= post ? sp[r].p : sp[r].q;
-            }
-            t[2 * i] = v.real();
-            t[2 * i + 1] = v.imag();
+            t[2 * i] = e[i].real();
+            t[2 * i + 1] = e[i].imag();
         }
         return (int)at;
     };
