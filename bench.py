@@ -779,7 +779,8 @@ def batch_workload(torch, count: int, cpu: bool):
     # autotuner picks for that launch (a class alone has fewer circuits than
     # the whole batch, so its best occupancy differs).  Circuits that fit the
     # tile width live in LDS (bound: LDS, 128 B/clk/CU); wider ones stream psi
-    # and lambda through HBM, two tiled passes per layer each way (bound: HBM).
+    # and lambda through HBM in L + 1 tiled passes each way, each pass serving
+    # half of two layers (bound: HBM).
     lds_peak = 256 * 128 * 2.4e9
     gm, elem = 4, 8  # complex64: 4 qubits per register group, 8-byte amplitudes
     classes = {}
@@ -797,7 +798,10 @@ def batch_workload(torch, count: int, cpu: bool):
             byts = L * (sweeps_f * 2 + sweeps_r * 4) * N * elem
             bound, peak = "lds", lds_peak
         else:
-            byts = (96 * L + 48 * len(cls)) * N  # per layer 96 N B, per circuit 48 N B (init, H)
+            # forward: L + 1 passes reading and writing psi (16 N B each); reverse:
+            # L + 1 passes over psi and lambda (32 N B); per circuit 48 N B more
+            # (the start state, lambda = H psi)
+            byts = (48 * L + 96 * len(cls)) * N
             bound, peak = "hbm", HBM_PEAK_TBPS * 1e12
         if bound == "hbm":
             launch_bytes += byts

@@ -9,22 +9,28 @@
 // theta_q = p[2nl + q], phi_q = p[2nl + n + q] (the layout of iq_circuit_*).
 //
 // One workgroup owns one circuit at a time: workgroup b starts with circuit b
-// of the host's order (longest first), then takes the next from an atomic queue.  With b = the circuit's tile width:
+// of the host's order (longest first), then takes the next from an atomic
+// queue.  With b = the circuit's tile width:
 //   n <= b  psi and lambda live in LDS for the whole circuit;
-//   n >  b  they live in a per-workgroup HBM scratch and every ansatz layer is
-//           two tiled passes: pass A tiles the b low qubits (contiguous), pass B
-//           tiles the n-b high qubits plus the low 2b-n ones (runs of 2^(2b-n)
-//           amplitudes) and also applies the layer's diagonal, which depends
-//           only on the global index.  The reverse sweep runs B then A per layer.
+//   n >  b  they live in a per-workgroup HBM scratch, and a sweep is L + 1
+//           tiled passes that alternate between kind A (the b low qubits,
+//           contiguous) and kind B (the n-b high qubits plus the low 2b-n
+//           ones, runs of 2^(2b-n) amplitudes).  Layer l's rotations go half
+//           in pass l (the qubits of its kind) and half in pass l + 1, so pass
+//           p applies layer p-1's second half, that layer's diagonal and layer
+//           p's first half.  The reverse sweep runs the passes backwards.
 // A layer is RY on every qubit (real rotations), then its diagonal D = CZ ring
 // times the RZ set: one phase per amplitude, e^{i sum_q phi_q (g_q - 1/2)} with
-// the CZ sign, applied as pass B's tile is stored (forward) or loaded (D^dagger,
-// reverse).  Undoing D first leaves psi and lambda at the RY outputs, where
-// dE/dtheta_q = Im<lambda|Y_q psi> and dE/dphi_q = Im<lambda|Z_q psi>.  Inside a
-// tile, rotations go in register groups of GW qubits, one LDS round trip per
-// group.  Gradient inner products accumulate in FP64 per lane, reduce across
-// the wave with shuffles into per-wave slots, and are summed over waves in a
-// fixed order at the end of each layer: deterministic, no atomics on results.
+// the CZ sign, applied in the pass that completes the layer's rotations
+// (D^dagger in the reverse sweep).  Undoing D first leaves psi and lambda at
+// the RY outputs, where dE/dtheta_q = Im<lambda|Y_q psi> and dE/dphi_q =
+// Im<lambda|Z_q psi>; rotations on the other qubits, applied to both, leave a
+// qubit's overlaps unchanged, so each half-layer's overlaps are taken where it
+// is undone.  Inside a tile, rotations go in register groups of GW qubits, one
+// LDS round trip per group.  Gradient inner products accumulate in FP64 per
+// lane, reduce across the wave with shuffles into per-wave slots (one bank per
+// layer parity), and are summed over waves in a fixed order once a layer is
+// undone: deterministic, no atomics on results.
 
 #include <hip/hip_runtime.h>
 
@@ -52,7 +58,11 @@ struct BatchCircuit {
     int64_t out_index;  // energy slot
 };
 
-constexpr int kMaxSlots = 96;  // gradient slots per layer: 3 per qubit (n <= 30)
+// Gradient slots per wave: two banks (layer parity) of 64, two per qubit
+// (2q: dE/dphi_q, 2q + 1: dE/dtheta_q, n <= 30).  One tiled pass can undo the
+// rotations of two layers, each into its own bank.
+constexpr int kSlotBank = 64;
+constexpr int kMaxSlots = 2 * kSlotBank;
 
 // Per qubit and layer the kernel keeps the RY in scaled form.  With c, s the
 // cosine and sine of half the angle, R0(u) = [[1, -u], [u, 1]], X the bit flip
@@ -305,7 +315,7 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
 #pragma unroll
             for (int off = 4; off > 0; off >>= 1) xs += __shfl_xor(xs, off, 64);
             const int k = (h5 ? 4 : 0) + (h4 ? 2 : 0) + (h3 ? 1 : 0);
-            if ((lane & 7) == 0 && k < 2 * G) wred[wave * kMaxSlots + 2 * (t0 + qoff) + k] += xs * sc_run;
+            if ((lane & 7) == 0 && k < 2 * G) wred[wave * kMaxSlots + (l & 1) * kSlotBank + 2 * (t0 + qoff) + k] += xs * sc_run;
             sc_run *= group_scale(t0, G, qoff, l);
         };
         // --- tile transfer helpers: eight or sixteen elements per thread in
@@ -523,15 +533,16 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                 __syncthreads();
             }
         };
-        auto no_pre = [](uint64_t) __attribute__((always_inline)) {};
         // The tables of phase_at, made in FP64 (angle sums, then sincos) for the
         // tile's bit-to-qubit map, the second with the angles of the qubits the
         // tile number fixes folded in.
         auto build_phase = [&](int kind, uint64_t t, int l, double scale) __attribute__((always_inline)) {
             const double* ph = params + C.param_off + 2 * n * l + n;
-            const int lo = 2 * b - n;  // kind 2: tile bits [0, lo) on qubits [0, lo), the rest on [b, n)
+            // kind 2: tile bits [0, lo) on qubits [0, lo), the rest on [b, n);
+            // kind 1: tile bit k on qubit k; kind 0: the whole state
+            const int lo = 2 * b - n;
             const int nb = kind == 0 ? n : b;
-            auto qubit = [&](int k) { return kind == 0 || k < lo ? k : k - lo + b; };
+            auto qubit = [&](int k) { return kind == 2 && k >= lo ? k - lo + b : k; };
             const uint32_t na = TS < 128u ? TS : 128u, nbt = TS > 128u ? TS >> 7 : 1u;
             for (uint32_t i = tid; i < na + nbt; i += T) {
                 // qubit q at bit value g: phi_q (g - 1/2), and pi where a mode-1
@@ -548,6 +559,8 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                     for (int k = 7; k < nb; ++k) ang += term(qubit(k), (e >> (k - 7)) & 1u);
                     if (kind == 2)  // tile number bit k on qubit lo + k
                         for (int k = 0; k < n - b; ++k) ang += term(lo + k, (uint32_t)(t >> k) & 1u);
+                    if (kind == 1)  // tile number bit k on qubit b + k
+                        for (int k = 0; k < n - b; ++k) ang += term(b + k, (uint32_t)(t >> k) & 1u);
                 }
                 double sv, cvl;
                 sincos(ang, &sv, &cvl);
@@ -561,11 +574,12 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
                 ptab[i < na ? i : kPhaseLo + (i - na)] = v;
             }
         };
-        // D (forward) or D^dagger (reverse, on psi and lambda) over an LDS-resident state
-        auto diag = [&](bool both) __attribute__((always_inline)) {
+        // D (forward) or D^dagger (reverse, on psi and lambda) over the tile
+        // in LDS (kind 0: the resident state)
+        auto diag = [&](int kind, uint64_t t, bool both) __attribute__((always_inline)) {
             for (uint32_t j = tid; j < TS; j += T) {
                 R pr, pi;
-                phase_at(0, 0, j, both, pr, pi);
+                phase_at(kind, t, j, both, pr, pi);
                 tp[P(j)] = cmul(tp[P(j)], pr, pi);
                 if (both) tl[P(j)] = cmul(tl[P(j)], pr, pi);
             }
@@ -575,18 +589,23 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             for (int i = tid; i < nw * kMaxSlots; i += T) wred[i] = 0.0;
             __syncthreads();
         };
-        // layer gradients from the slots (2q: dE/dphi_q, 2q+1: dE/dtheta_q)
+        // layer gradients from the layer's bank of slots (2q: dE/dphi_q,
+        // 2q+1: dE/dtheta_q), summed over waves in a fixed order; the bank is
+        // cleared for layer l - 2
         auto flush_slots = [&](int l) __attribute__((always_inline)) {
+            const int bank = (l & 1) * kSlotBank;
             for (int q = tid; q < n; q += T) {
                 double z = 0.0, va = 0.0;
                 for (int w = 0; w < nw; ++w) {
-                    z += wred[w * kMaxSlots + 2 * q];
-                    va += wred[w * kMaxSlots + 2 * q + 1];
+                    z += wred[w * kMaxSlots + bank + 2 * q];
+                    va += wred[w * kMaxSlots + bank + 2 * q + 1];
                 }
                 G[2 * n * l + n + q] = z;
                 // a mode-1 rotation's -Z, applied before the overlaps, turns Y into -Y
                 G[2 * n * l + q] = RT[(n * l + q) * kRotStride + 3] != (R)0 ? -va : va;
             }
+            __syncthreads();
+            for (int i = tid; i < nw * kSlotBank; i += T) wred[(i / kSlotBank) * kMaxSlots + bank + i % kSlotBank] = 0.0;
             __syncthreads();
         };
 
@@ -608,26 +627,41 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
             }
             __syncthreads();
         }
-        for (int l = 0; l < L; ++l) {
-            // the layer's rotations leave the state divided by ff, restored
-            // with the diagonal's phase
-            const double ff = layer_factor(l);
-            if (resident) {
+        if (resident) {
+            for (int l = 0; l < L; ++l) {
+                // the layer's rotations leave the state divided by ff, restored
+                // with the diagonal's phase
+                const double ff = layer_factor(l);
                 for_groups(0, n, GM, [&](int t0, int g) { rot_group(t0, g, 0, l); });
                 build_phase(0, 0, l, ff);
                 __syncthreads();
-                diag(false);
-            } else {
-                // pass A: RY on qubits 0..b-1 (those on the register group's
-                // tile bits as the tile is loaded)
-                const bool ra = regq, rb = regq && b - h <= RB;
-                tile_pass(1, false, 0, 0, ra ? 1 : 0, 0, l, no_pre, [&](uint64_t) {
-                    for_groups(0, ra ? RB : b, GM, [&](int t0, int g) { rot_group(t0, g, 0, l); });
-                });
-                // pass B: RY on qubits b..n-1, then the RZ set and the CZ ring as
-                // the tile is stored (the loads' barrier orders the tables first)
-                tile_pass(2, false, 0, 1, rb ? 1 : 0, h, l, [&](uint64_t t) { build_phase(2, t, l, ff); }, [&](uint64_t) {
-                    for_groups(b - h, rb ? RB : b, GM, [&](int t0, int g) { rot_group(t0, g, h, l); });
+                diag(0, 0, false);
+            }
+        } else {
+            // L + 1 tiled passes: pass p tiles kind A (the b low qubits) for even
+            // p, kind B (the high qubits) for odd p.  Layer l's RY go half in
+            // pass l (the qubits of that pass's kind) and half in pass l + 1,
+            // whose kind holds the other qubits; rotations on different qubits
+            // commute, so pass p applies layer p - 1's second half, then D_{p-1}
+            // (every qubit of layer p - 1 rotated), then layer p's first half:
+            // each pass serves two layers, where one pass per half-layer would
+            // move the state 2L times.
+            for (int p = 0; p <= L; ++p) {
+                const int kind = (p & 1) ? 2 : 1, qo = kind == 2 ? h : 0, g0 = kind == 2 ? b - h : 0;
+                const bool rg = regq && (kind == 1 || b - h <= RB);
+                const bool prev = p >= 1, next = p < L;
+                const int lr = prev ? p - 1 : 0;  // the layer whose rotations the pass starts with
+                const double ff = prev ? layer_factor(p - 1) : 1.0;
+                // the register group's rotations as the tile is loaded; D_{L-1}
+                // (the last pass) as it is stored
+                tile_pass(kind, false, 0, prev && !next ? 1 : 0, rg ? 1 : 0, qo, lr, [&](uint64_t t) {
+                    if (prev) build_phase(kind, t, p - 1, ff);  // the loads' barrier orders it first
+                }, [&](uint64_t t) {
+                    for_groups(g0, rg ? RB : b, GM, [&](int t0, int g) { rot_group(t0, g, qo, lr); });
+                    if (prev && next) {
+                        diag(kind, t, false);
+                        for_groups(g0, b, GM, [&](int t0, int g) { rot_group(t0, g, qo, p); });
+                    }
                 });
             }
         }
@@ -708,38 +742,52 @@ __global__ void __launch_bounds__(512) k_batch_hea(const BatchCircuit* __restric
         __syncthreads();
 
         // --- reverse sweep
-        for (int l = L - 1; l >= 0; --l) {
-            clear_slots();
-            // psi and lambda come divided by the later layer's product of f',
-            // restored with D^dagger's phase
-            const double fpend = l + 1 < L ? layer_factor(l + 1) : 1.0;
-            if (resident) {
+        clear_slots();
+        if (resident) {
+            for (int l = L - 1; l >= 0; --l) {
+                // psi and lambda come divided by the later layer's product of
+                // f', restored with D^dagger's phase
+                const double fpend = l + 1 < L ? layer_factor(l + 1) : 1.0;
                 build_phase(0, 0, l, fpend);
                 __syncthreads();
-                diag(true);
+                diag(0, 0, true);
                 sc_run = 1.0;
                 for_groups(0, n, GA, [&](int t0, int g) { adj_group(t0, g, 0, l); });
-            } else {
-                // pass B': the CZ ring and RZ set undone as the tile is loaded,
-                // then qubits b..n-1
-                const bool ra = regq, rb = regq && b - h <= RB;
-                // every tile runs the same sequence of groups: sc_run restarts
-                // at the pass's value for each tile
-                double sc0 = 1.0;
-                tile_pass(2, true, 2, 0, rb ? 2 : 0, h, l, [&](uint64_t t) {
-                    build_phase(2, t, l, fpend);
-                    __syncthreads();
-                    sc_run = sc0;
-                }, [&](uint64_t) {
-                    for_groups(b - h, rb ? RB : b, GA, [&](int t0, int g) { adj_group(t0, g, h, l); });
-                });
-                // pass A': qubits 0..b-1 (the register group's as the tile is loaded)
-                sc0 = sc_run;
-                tile_pass(1, true, 0, 0, ra ? 2 : 0, 0, l, [&](uint64_t) { sc_run = sc0; }, [&](uint64_t) {
-                    for_groups(0, ra ? RB : b, GA, [&](int t0, int g) { adj_group(t0, g, 0, l); });
-                });
+                flush_slots(l);
             }
-            flush_slots(l);
+        } else {
+            // the forward passes in reverse: pass p undoes layer p's first half
+            // (D_p^dagger was applied by pass p + 1), then D_{p-1}^dagger, then
+            // layer p - 1's second half.  Layer p's overlaps carry the square of
+            // the f' of its qubits already undone: sc_run continues from the end
+            // of pass p + 1, where its second half was undone; every tile runs
+            // the same sequence, so it restarts at the pass's value per tile.
+            double sc_carry = 1.0;
+            for (int p = L; p >= 0; --p) {
+                const int kind = (p & 1) ? 2 : 1, qo = kind == 2 ? h : 0, g0 = kind == 2 ? b - h : 0;
+                const bool rg = regq && (kind == 1 || b - h <= RB);
+                const bool prev = p >= 1, next = p < L;
+                const int lr = next ? p : p - 1;  // the layer whose rotations the pass undoes first
+                // psi and lambda come divided by layer p's product of f' (every
+                // qubit of it undone), restored with D_{p-1}^dagger's phase
+                const double fpend = next ? layer_factor(p) : 1.0;
+                const double sc_in = next ? sc_carry : 1.0;
+                // the last pass's D_{L-1}^dagger as the tile is loaded
+                tile_pass(kind, true, next ? 0 : 2, 0, rg ? 2 : 0, qo, lr, [&](uint64_t t) {
+                    if (prev) build_phase(kind, t, p - 1, fpend);
+                    if (!next) __syncthreads();  // the loads use the table
+                    sc_run = sc_in;
+                }, [&](uint64_t t) {
+                    for_groups(g0, rg ? RB : b, GA, [&](int t0, int g) { adj_group(t0, g, qo, lr); });
+                    if (prev && next) {
+                        diag(kind, t, true);
+                        sc_run = 1.0;
+                        for_groups(g0, b, GA, [&](int t0, int g) { adj_group(t0, g, qo, p - 1); });
+                    }
+                });
+                sc_carry = sc_run;
+                if (next) flush_slots(p);  // both halves of layer p undone
+            }
         }
     }
 }
