@@ -847,6 +847,23 @@ def batch_workload(torch, count: int, cpu: bool):
     return out
 
 
+def batch_class_workload(n: int):
+    """One size class of the 4096-circuit batch launched alone with a fixed
+    configuration (the widest tile, 512 threads, the widest fusion width): a
+    short run for hardware-counter passes."""
+    from invariantq_amd import batch, workloads
+    specs = [s for s in workloads.batch_specs(4096, seed=3) if s[0] == n]
+    if not specs:
+        raise SystemExit(f"the batch has no circuit of {n} qubits")
+    runner = batch.BatchRunner("complex64")
+    cfg = dict(tile_bits={n: runner.max_tile_bits}, block_threads=512, group_width=runner.max_group_width)
+    runner.run(specs, **cfg)
+    _, _, ms = runner.run(specs, **cfg)
+    L = sum(s[1] for s in specs)
+    return {"workload": f"batch size class n={n} alone", "circuits": len(specs), "layers": L, "ms": round(ms, 3),
+            "config": {"tile_bits": runner.max_tile_bits, "block_threads": 512, "group_width": runner.max_group_width}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -862,6 +879,8 @@ def main():
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the circuit20, dm14 and batch sub-objects of the default line")
     ap.add_argument("--only-fused", action="store_true", help="skip every leg but the complex128 fused pass")
+    ap.add_argument("--batch-class", type=int, default=0,
+                    help="with --workload batch4096: launch only the circuits of this qubit count, fixed configuration")
     ap.add_argument("--workload", default="fused",
                     choices=["fused", "circuit20", "dm14", "batch4096", "adjoint24", "bandwidth", "primitives"],
                     help="fused: the headline line; circuit20 / dm14 / batch4096 / adjoint24: the other "
@@ -885,6 +904,9 @@ def main():
         print(json.dumps(circuit_workload(binding, torch, a.steps, a.warmup, not a.no_cpu)), flush=True)
         return
     if a.workload == "batch4096":
+        if a.batch_class:
+            print(json.dumps(batch_class_workload(a.batch_class)), flush=True)
+            return
         print(json.dumps(batch_workload(torch, 4096, not a.no_cpu)), flush=True)
         return
     if a.workload == "dm14":
