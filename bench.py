@@ -433,6 +433,40 @@ def circuit_workload(binding, torch, steps: int, warmup: int, cpu: bool):
                     "circuit_gpu_ms": round(tf_gpu * 1e3, 3), "speedup_gpu_time": round(t_gpu / tf_gpu, 2),
                     "max_abs_amp_diff_vs_unfused": float(np.max(np.abs(sv.get_amplitudes() - unfused_amps))),
                     "samples_equal_unfused": bool(np.array_equal(fs, samples))}
+    # the same circuit through the circuit engine's tile passes (CX as H CZ H:
+    # each layer's one-qubit runs and CZ signs in about two passes), the state
+    # copied into the statevector handle; timed from the first launch to the
+    # copy, wall and GPU (events on the circuit's stream, then the copy)
+    from invariantq_amd import adjoint
+    circ = adjoint.GpuCircuit.from_gates(n, [(op.qubits, op.matrix) for op in ops], "complex128")
+    cstream = torch.cuda.ExternalStream(circ.stream)
+
+    def run_tiled():
+        circ.run()
+        circ.copy_state_to(sv)
+
+    for _ in range(warmup + 1):
+        run_tiled()
+    sv.synchronize()
+    tt, tt_gpu = 0.0, 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        ev[0].record(cstream)
+        circ.run()
+        ev[1].record(cstream)
+        circ.copy_state_to(sv)
+        sv.synchronize()
+        tt += time.perf_counter() - t0
+        tt_gpu += ev[0].elapsed_time(ev[1]) * 1e-3
+    tt, tt_gpu = tt / steps, tt_gpu / steps
+    ts_ = sv.sample(shots, seed)
+    out["tiled"] = {"passes": int(circ.num_segments), "circuit_ms": round(tt * 1e3, 3),
+                    "circuit_gpu_ms": round(tt_gpu * 1e3, 3),
+                    "gate_applications_per_s": round(len(ops) / tt, 1),
+                    "speedup_vs_gate_by_gate_wall": round(t_gates / tt, 2),
+                    "max_abs_amp_diff_vs_unfused": float(np.max(np.abs(sv.get_amplitudes() - unfused_amps))),
+                    "samples_equal_unfused": bool(np.array_equal(ts_, samples))}
+    del circ
     # seed-tree tasks (4 workers, job "cfg0", master 42): one call on one state,
     # against the reference's way, a re-simulation plus sample() per task
     from invariantq_amd import tasks as T

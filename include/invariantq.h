@@ -224,6 +224,14 @@ int    iq_circuit_parameter_shift(void* c, const double* params, size_t nparams,
  * shift; after a gradient, the output when pass outputs were stored (see below), else |0..0>
  * up to rounding (the reverse sweep takes psi back in place). */
 void   iq_circuit_get_amplitudes(void* c, double* real_out, double* imag_out);
+/* The forward sweep alone (no observable needed): the circuit's state becomes U(params)|0..0>,
+ * asynchronously on iq_circuit_stream(c).  A gate list with no parameters runs with params NULL
+ * and nparams 0.  0 on success. */
+int    iq_circuit_run(void* c, const double* params, size_t nparams);
+/* Copies the circuit's state into a statevector handle of the same qubit count and precision
+ * (device to device, after the circuit's work, ordered on the statevector's stream), so the
+ * statevector entry points (probabilities, sampling, measurement) act on it.  0 on success. */
+int    iq_circuit_copy_state(void* c, void* sv);
 void*  iq_circuit_stream(void* c);
 /* Device memory a gradient may hold for the forward sweep's pass outputs
  * (default 16 GiB).  While they fit, the reverse sweep reads psi back from them

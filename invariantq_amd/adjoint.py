@@ -24,6 +24,9 @@ from .core import binding
 from .workloads import Ansatz
 
 RX, RY, RZ, U3, CZ, CX, MATRIX1, MATRIX2 = range(8)
+_H_MATRIX = np.array([[1, 1], [1, -1]], dtype=np.complex128) / np.sqrt(2.0)
+_CX_MATRIX = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]], dtype=np.complex128)
+_CZ_MATRIX = np.diag([1, 1, 1, -1]).astype(np.complex128)
 _KIND = {"rx": RX, "ry": RY, "rz": RZ, "u3": U3, "cz": CZ, "cx": CX, "cnot": CX}
 
 _d = ctypes.POINTER(ctypes.c_double)
@@ -75,6 +78,44 @@ class GpuCircuit:
                                                     _dp(im) if im is not None else None)
         binding.check_error()
         return r
+
+    @classmethod
+    def from_gates(cls, num_qubits: int, gate_list, precision: str = "complex128") -> "GpuCircuit":
+        """A fixed gate list [(qubits, matrix)] (1- and 2-qubit unitaries, the
+        reference's local index: qubits[0] is the high bit) as a tape for the
+        tiled sweep.  One-qubit gates keep their exact matrices; CX becomes
+        H CZ H on its target (CZ is a sign the tile passes apply with the
+        runs around it); CZ stays CZ; other two-qubit gates are fixed 4x4
+        matrices, one pass each."""
+        c = cls(num_qubits, precision)
+        for qubits, m in gate_list:
+            m = np.asarray(m, dtype=np.complex128)
+            if len(qubits) == 1:
+                c.add(MATRIX1, qubits, matrix=m)
+            elif len(qubits) == 2:
+                a, b = int(qubits[0]), int(qubits[1])
+                if np.array_equal(m, _CX_MATRIX):
+                    c.add(MATRIX1, [b], matrix=_H_MATRIX)
+                    c.add(CZ, [a, b])
+                    c.add(MATRIX1, [b], matrix=_H_MATRIX)
+                elif np.array_equal(m, _CZ_MATRIX):
+                    c.add(CZ, [a, b])
+                else:
+                    c.add(MATRIX2, [a, b], matrix=m)
+            else:
+                raise ValueError("the tiled sweep takes one- and two-qubit gates")
+        return c
+
+    def run(self, params=None) -> None:
+        """The forward sweep alone: the circuit's state becomes U(params)|0..0>."""
+        p = self._params(np.zeros(0) if params is None else params)
+        binding._NATIVE_LIB.iq_circuit_run(self._ptr, _dp(p) if p.size else None, p.size)
+        binding.check_error()
+
+    def copy_state_to(self, sv) -> None:
+        """Copies the circuit's state into a NativeStateVector of the same size and precision."""
+        binding._NATIVE_LIB.iq_circuit_copy_state(self._ptr, sv._ptr)
+        binding.check_error()
 
     @classmethod
     def from_ansatz(cls, ansatz: Ansatz, precision: str = "complex64") -> "GpuCircuit":

@@ -160,6 +160,8 @@ SIGNATURES = {
     "iq_circuit_gradient": ([_vp, _d, _sz, _d, _d], ctypes.c_int),
     "iq_circuit_parameter_shift": ([_vp, _d, _sz, ctypes.POINTER(ctypes.c_int64), _sz, _d], ctypes.c_int),
     "iq_circuit_get_amplitudes": ([_vp, _d, _d], None),
+    "iq_circuit_run": ([_vp, _d, _sz], ctypes.c_int),
+    "iq_circuit_copy_state": ([_vp, _vp], ctypes.c_int),
     "iq_circuit_stream": ([_vp], _vp),
     "iq_circuit_set_stored_budget": ([_vp, _sz], ctypes.c_int),
     "iq_circuit_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),

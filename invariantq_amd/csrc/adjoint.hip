@@ -2633,6 +2633,44 @@ double iq_circuit_expectation(void* cp, const double* params, size_t nparams) {
     return e;
 }
 
+// The forward sweep alone: the circuit's state becomes U(params)|0..0>, with
+// no observable.  Asynchronous on the circuit's stream.
+int iq_circuit_run(void* cp, const double* params, size_t nparams) {
+    if (!cp || (!params && nparams)) {
+        set_error(kErrInvalidArgument, "null circuit or parameter array");
+        return -1;
+    }
+    Circuit& c = *circ_of(cp);
+    DeviceGuard dg(c.eng.dev);
+    if (!check_params(c, nparams)) return -1;
+    if (!c.compiled) compile(c);
+    c.tused = 0;
+    const bool ok = c.prec == kC128 ? run_forward<double>(c, params) : run_forward<float>(c, params);
+    return ok ? 0 : -1;
+}
+
+// Copies the circuit's state into statevector handle sv (same qubit count,
+// precision and device), after the circuit's work; ordered on sv's stream.
+int iq_circuit_copy_state(void* cp, void* svp) {
+    if (!cp || !svp) {
+        set_error(kErrInvalidArgument, "null circuit or statevector");
+        return -1;
+    }
+    Circuit& c = *circ_of(cp);
+    Engine& s = *static_cast<Engine*>(svp);
+    if (s.n != c.n || s.prec != c.prec || s.dev != c.eng.dev) {
+        set_error(kErrInvalidArgument, "statevector shape (n=%d, precision %d) differs from the circuit's (n=%d, %d)",
+                  s.n, s.prec, c.n, c.prec);
+        return -1;
+    }
+    DeviceGuard dg(c.eng.dev);
+    if (!c.eng.sync()) return -1;
+    if (!hip_ok(hipMemcpyAsync(s.amp, c.eng.amp, c.eng.dim * c.eng.elem, hipMemcpyDeviceToDevice, s.stream),
+                "state copy"))
+        return -1;
+    return 0;
+}
+
 int iq_circuit_set_lambda_tiles(void* cp, int on) {
     if (!cp) return -1;
     circ_of(cp)->rev_lam_only = on != 0;
