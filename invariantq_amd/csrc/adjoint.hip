@@ -30,6 +30,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -1379,6 +1380,10 @@ struct Circuit {
     // psi from its slot (true), or stage both (false); the same sums either way.
     bool rev_lam_only = true;
     ObsPlan oplan;  // the observable over tile passes (16 to 34 qubits)
+    // The forward sweep's plan (pass descriptors, launches, phase tables) of a
+    // tape with no parametric gate, which depends on the tape alone: made on
+    // the first run, dropped when a gate is added.
+    std::shared_ptr<void> fixed_plan;
     // Per-launch timing (off by default): each launch of an evaluation is
     // bracketed by two events on the circuit's stream; kinds are kLaunch*.
     bool timing = false;
@@ -2301,15 +2306,33 @@ void* pass_out(Circuit& c, bool stored, size_t i, size_t nl) {
 
 // Forward sweep; with `store`, every pass but the last writes its own stored
 // slot (same traffic as in place), for the reverse sweep to read.
+// A tape's forward plan, kept when no gate takes a parameter.
+struct FixedPlan {
+    std::vector<PassSeg> segv;
+    std::vector<Launch> launches;
+    std::vector<double> tabs;
+};
+
 template <typename R>
 bool run_forward_tiled(Circuit& c, const double* params, bool store) {
     Engine& e = c.eng;
     c.stored_valid = false;
     if (!e.reset_basis(0)) return false;
-    std::vector<PassSeg> segv;
-    std::vector<Launch> launches;
-    std::vector<double> tabs;
-    plan_tiles(c, params, false, segv, launches, tabs);
+    bool fixed = true;
+    for (const auto& g : c.gates) fixed = fixed && g.p[0] < 0;
+    if (fixed && !c.fixed_plan) {
+        auto fp = std::make_shared<FixedPlan>();
+        plan_tiles(c, params, false, fp->segv, fp->launches, fp->tabs);
+        c.fixed_plan = fp;
+    }
+    std::vector<PassSeg> seg_local;
+    std::vector<Launch> launch_local;
+    std::vector<double> tab_local;
+    if (!fixed) plan_tiles(c, params, false, seg_local, launch_local, tab_local);
+    const FixedPlan* fp = fixed ? static_cast<const FixedPlan*>(c.fixed_plan.get()) : nullptr;
+    const std::vector<PassSeg>& segv = fp ? fp->segv : seg_local;
+    const std::vector<Launch>& launches = fp ? fp->launches : launch_local;
+    const std::vector<double>& tabs = fp ? fp->tabs : tab_local;
     size_t tab_off = 0;
     Circuit::DescSlot* ds = upload_descs(c, segv, tabs, tab_off);
     if (!ds) return false;
@@ -2536,6 +2559,7 @@ int iq_circuit_add_gate(void* cp, int kind, size_t q0, size_t q1, const int64_t*
     }
     c.gates.push_back(g);
     c.compiled = false;
+    c.fixed_plan.reset();
     return (int)c.gates.size() - 1;
 }
 
