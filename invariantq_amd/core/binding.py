@@ -75,8 +75,10 @@ SIGNATURES = {
     "iq_statevector_create": ([_sz], _vp),
     "iq_statevector_free": ([_vp], None),
     "iq_statevector_reset": ([_vp], None),
-    "iq_statevector_apply_1q": ([_vp, _sz, _d, _d], None),
-    "iq_statevector_apply_2q": ([_vp, _sz, _sz, _d, _d], None),
+    # matrix pointers of the per-gate calls as plain addresses (_vp): a
+    # ctypes pointer object costs more than the launch it feeds
+    "iq_statevector_apply_1q": ([_vp, _sz, _vp, _vp], None),
+    "iq_statevector_apply_2q": ([_vp, _sz, _sz, _vp, _vp], None),
     "iq_statevector_apply_fused": ([_vp, _s, _sz, _d, _d], None),
     "iq_statevector_measure": ([_vp, _sz, ctypes.c_double], ctypes.c_int),
     "iq_statevector_reset_qubit": ([_vp, _sz, ctypes.c_double], None),
@@ -89,8 +91,8 @@ SIGNATURES = {
     "iq_density_matrix_create": ([_sz], _vp),
     "iq_density_matrix_free": ([_vp], None),
     "iq_density_matrix_reset": ([_vp], None),
-    "iq_density_matrix_apply_1q_unitary": ([_vp, _sz, _d, _d], None),
-    "iq_density_matrix_apply_2q_unitary": ([_vp, _sz, _sz, _d, _d], None),
+    "iq_density_matrix_apply_1q_unitary": ([_vp, _sz, _vp, _vp], None),
+    "iq_density_matrix_apply_2q_unitary": ([_vp, _sz, _sz, _vp, _vp], None),
     "iq_density_matrix_apply_kraus_1q": ([_vp, _sz, _sz, _d, _d], None),
     "iq_density_matrix_get_elements": ([_vp, _d, _d], None),
     "iq_density_matrix_trace": ([_vp], ctypes.c_double),
@@ -229,6 +231,19 @@ def _split(matrix, expect: Optional[int] = None):
     return (np.ascontiguousarray(m.real, dtype=np.float64), np.ascontiguousarray(m.imag, dtype=np.float64))
 
 
+def _packed(matrix, expect: int):
+    """One float64 buffer: the real parts, then the imaginary parts (row-major);
+    returns (buffer, address of the real parts, address of the imaginary ones)."""
+    m = np.asarray(matrix, dtype=np.complex128).reshape(-1)
+    if m.size != expect:
+        raise ValueError(f"matrix has {m.size} entries, expected {expect}")
+    buf = np.empty(2 * expect)
+    buf[:expect] = m.real
+    buf[expect:] = m.imag
+    a = buf.ctypes.data
+    return buf, a, a + 8 * expect
+
+
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(_d)
 
@@ -263,13 +278,13 @@ class NativeStateVector:
         _NATIVE_LIB.iq_statevector_reset(self._ptr)
 
     def apply_1q(self, target: int, matrix):
-        re, im = _split(matrix, 4)
-        _NATIVE_LIB.iq_statevector_apply_1q(self._ptr, target, _dp(re), _dp(im))
+        buf, re, im = _packed(matrix, 4)
+        _NATIVE_LIB.iq_statevector_apply_1q(self._ptr, target, re, im)
         check_error()
 
     def apply_2q(self, q0: int, q1: int, matrix):
-        re, im = _split(matrix, 16)
-        _NATIVE_LIB.iq_statevector_apply_2q(self._ptr, q0, q1, _dp(re), _dp(im))
+        buf, re, im = _packed(matrix, 16)
+        _NATIVE_LIB.iq_statevector_apply_2q(self._ptr, q0, q1, re, im)
         check_error()
 
     def apply_fused(self, qubits: Sequence[int], matrix):
@@ -464,13 +479,13 @@ class NativeDensityMatrix:
         _NATIVE_LIB.iq_density_matrix_reset(self._ptr)
 
     def apply_1q_unitary(self, target: int, matrix):
-        re, im = _split(matrix, 4)
-        _NATIVE_LIB.iq_density_matrix_apply_1q_unitary(self._ptr, target, _dp(re), _dp(im))
+        buf, re, im = _packed(matrix, 4)
+        _NATIVE_LIB.iq_density_matrix_apply_1q_unitary(self._ptr, target, re, im)
         check_error()
 
     def apply_2q_unitary(self, q0: int, q1: int, matrix):
-        re, im = _split(matrix, 16)
-        _NATIVE_LIB.iq_density_matrix_apply_2q_unitary(self._ptr, q0, q1, _dp(re), _dp(im))
+        buf, re, im = _packed(matrix, 16)
+        _NATIVE_LIB.iq_density_matrix_apply_2q_unitary(self._ptr, q0, q1, re, im)
         check_error()
 
     def apply_1q_kraus(self, target: int, kraus_ops):

@@ -39,6 +39,19 @@ struct ValMat {
     R v[N];
 };
 
+// XCD-consistent split of a launch's work items, for states that fit the
+// eight XCDs' L2 caches together.  The dispatcher hands workgroup b to XCD
+// b % 8; with xm != 0 the grid is exactly 2^xm items in a multiple of 8
+// blocks, and block b takes the (b % 8)-th eighth of the items.  The top three
+// bits of an item index are the top three bits of its amplitudes' indices
+// whenever no target sits there, so every gate kernel hands the amplitudes
+// with top bits x to XCD x, whose L2 still holds them from the last gate.
+// xm = 0: item blockIdx * blockDim + threadIdx, grid-stride.
+__device__ __forceinline__ uint64_t xcd_item(uint32_t xm) {
+    if (!xm) return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    return ((uint64_t)(blockIdx.x & 7u) << (xm - 3)) | ((uint64_t)(blockIdx.x >> 3) * blockDim.x + threadIdx.x);
+}
+
 // y = U x for one group held in registers.  Rows run in a runtime loop for
 // D >= 8: each iteration streams one matrix row through SGPRs (scalar loads),
 // which bounds SGPR pressure; a fully unrolled 32x32 product would hoist all
@@ -207,11 +220,11 @@ __global__ void __launch_bounds__(256) k_gate_lanes_mixed(cv<R>* __restrict__ ps
 // 128-byte line (each load instruction then covers whole lines).
 template <typename R, int K, bool M3, bool NT = false>
 __global__ void __launch_bounds__(256) k_gate_direct_ptr(cv<R>* __restrict__ psi, uint64_t ngroups, DirectArgs<K> a,
-                                                         const R* __restrict__ mat) {
+                                                         const R* __restrict__ mat, uint32_t xm) {
     constexpr int D = 1 << K;
     // one group per thread; the grid is capped above 2^32 threads
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+    for (uint64_t g = xcd_item(xm); g < ngroups; g += stride) {
         const uint64_t base = insert_zero_bits<K>(g, a.pos);
         R xr[D], xi[D];
 #pragma unroll
@@ -234,11 +247,11 @@ __global__ void __launch_bounds__(256) k_gate_direct_ptr(cv<R>* __restrict__ psi
 // NT as in k_gate_direct_ptr.
 template <typename R, int K, bool M3, bool NT = false>
 __global__ void __launch_bounds__(256) k_gate_direct_val(cv<R>* __restrict__ psi, uint64_t ngroups, DirectArgs<K> a,
-                                                         ValMat<R, (M3 ? 3 : 2) * (1 << (2 * K))> mat) {
+                                                         ValMat<R, (M3 ? 3 : 2) * (1 << (2 * K))> mat, uint32_t xm) {
     constexpr int D = 1 << K;
     // one group per thread; the grid is capped above 2^32 threads
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+    for (uint64_t g = xcd_item(xm); g < ngroups; g += stride) {
         const uint64_t base = insert_zero_bits<K>(g, a.pos);
         R xr[D], xi[D];
 #pragma unroll
@@ -400,9 +413,9 @@ __global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, Mfma
 // (pairs base, base + toff with the control bits set and the target bit clear).
 template <typename R, int K, bool NT>
 __global__ void __launch_bounds__(256) k_gate_controlled(cv<R>* __restrict__ psi, uint64_t npairs, DirectArgs<K> a,
-                                                         uint64_t cbits, uint64_t toff, ValMat<R, 8> mat) {
+                                                         uint64_t cbits, uint64_t toff, ValMat<R, 8> mat, uint32_t xm) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < npairs; g += stride) {
+    for (uint64_t g = xcd_item(xm); g < npairs; g += stride) {
         const uint64_t base = insert_zero_bits<K>(g, a.pos) | cbits;
         R xr[2], xi[2];
 #pragma unroll
@@ -427,7 +440,7 @@ __global__ void __launch_bounds__(256) k_gate_controlled(cv<R>* __restrict__ psi
 // amplitude is touched once).
 template <typename R, int K, bool NT = false>
 __global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t count, DirectArgs<K> a,
-                                              ValMat<double, 2 * (1 << K)> d) {
+                                              ValMat<double, 2 * (1 << K)> d, uint32_t xm) {
     constexpr int D = 1 << K;
     __shared__ double sre[D], sim[D];
     if (threadIdx.x < D) {
@@ -445,8 +458,13 @@ __global__ void __launch_bounds__(256) k_diag(cv<R>* __restrict__ psi, uint64_t 
         o.y = dr * v.y + di * v.x;
         st_stream<R>(psi + i, o, NT);
     };
-    // a block takes spans of 4 * blockDim consecutive amplitudes, so a
-    // wave's four accesses fall in one span
+    if (xm) {  // one amplitude per thread, XCD-consistent
+        const uint64_t i = xcd_item(xm);
+        one(i, ld_stream<R>(psi + i, NT));
+        return;
+    }
+    // a block takes spans of kPassSpan * blockDim consecutive amplitudes, so a
+    // wave's accesses fall in one span
     const uint64_t span = (uint64_t)kPassSpan * blockDim.x;
     for (uint64_t b = (uint64_t)blockIdx.x * span; b < count; b += (uint64_t)gridDim.x * span) {
         const uint64_t i = b + threadIdx.x;
@@ -468,7 +486,26 @@ inline unsigned grid_for(uint64_t items, unsigned block) {
     return (unsigned)std::max<uint64_t>(1, g);
 }
 
-static int g_direct_nt = 1;  // streaming hint in the direct kernel: 1 where lines are whole, 0 never
+static int g_direct_nt = 1;
+
+// States up to this size (half the eight XCDs' L2 caches together) take the
+// XCD-consistent work split (xcd_item).  Measured on a 20-qubit complex128
+// state (16 MiB), one-qubit gates on random targets below the top three
+// qubits: 5.3 against 8.1 us per gate.
+constexpr uint64_t kXcdStateMax = 16ULL << 20;
+constexpr unsigned kXcds = 8;  // XCDs of the MI355X (gfx950)
+
+// xm for a launch of `items` work items (one per thread) in `blocks` blocks of
+// 256, over a state of `state_bytes`: log2(items) when the split applies (a
+// power-of-two item count, exactly one item per thread, a multiple of eight
+// blocks), else 0.
+inline uint32_t xcd_log2(uint64_t items, unsigned blocks, uint64_t state_bytes) {
+    if (state_bytes > kXcdStateMax || (uint64_t)blocks * 256 != items || blocks % kXcds != 0 || (items & (items - 1)))
+        return 0;
+    uint32_t m = 0;
+    while ((1ULL << m) < items) ++m;
+    return m;
+}  // streaming hint in the direct kernel: 1 where lines are whole, 0 never
 constexpr uint32_t kLaneLow = 6;  // lane-exchange forms for targets below this bit (a wave's 64 lanes)
 
 template <typename R, int K, bool M3>
@@ -478,6 +515,7 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
     const uint64_t ngroups = 1ULL << (n - K);
     const unsigned blocks = grid_for(ngroups, 256);
     constexpr int D = 1 << K;
+    const uint32_t xm = xcd_log2(ngroups, blocks, (1ULL << n) * sizeof(cv<R>));
     // streaming when no target sits inside a 128-byte line and the state is
     // well beyond the caches (measured on a 14-qubit density matrix, 2^28
     // complex128 entries: one-qubit Kraus channels 5.43 against 5.27 TB/s,
@@ -541,16 +579,16 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
         pack_gate_matrix(std::is_same<R, double>::value ? kC128 : kC64, K, M3, g.re, g.im, mv.v);
         if (whole)
             hipLaunchKernelGGL((k_gate_direct_val<R, K, M3, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups,
-                               a, mv);
+                               a, mv, xm);
         else
             hipLaunchKernelGGL((k_gate_direct_val<R, K, M3, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups,
-                               a, mv);
+                               a, mv, xm);
     } else if (whole) {
         hipLaunchKernelGGL((k_gate_direct_ptr<R, K, M3, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups,
-                           a, (const R*)g.dmat);
+                           a, (const R*)g.dmat, xm);
     } else {
         hipLaunchKernelGGL((k_gate_direct_ptr<R, K, M3, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, ngroups,
-                           a, (const R*)g.dmat);
+                           a, (const R*)g.dmat, xm);
     }
     return hipGetLastError();
 }
@@ -712,10 +750,11 @@ hipError_t run_diag(void* psi, int n, hipStream_t st, const uint32_t* q, const d
     const uint64_t count = 1ULL << n;
     // one thread per four amplitudes: every load of a thread issued at once
     const unsigned blocks = grid_for((count + kPassSpan - 1) / kPassSpan, 256);
+    const uint32_t xm = kPassSpan == 1 ? xcd_log2(count, blocks, count * sizeof(cv<R>)) : 0u;
     if (count >= kStreamMinCount && g_direct_nt != 0)
-        hipLaunchKernelGGL((k_diag<R, K, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d);
+        hipLaunchKernelGGL((k_diag<R, K, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d, xm);
     else
-        hipLaunchKernelGGL((k_diag<R, K, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d);
+        hipLaunchKernelGGL((k_diag<R, K, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, count, a, d, xm);
     return hipGetLastError();
 }
 
@@ -822,12 +861,13 @@ hipError_t run_controlled(void* psi, int n, hipStream_t st, const uint32_t* c, u
         }
     const uint64_t count = 1ULL << n, npairs = count >> K;
     const unsigned blocks = grid_for(npairs, 256);
+    const uint32_t xm = xcd_log2(npairs, blocks, count * sizeof(cv<R>));
     if (count >= kStreamMinCount && g_direct_nt != 0)
         hipLaunchKernelGGL((k_gate_controlled<R, K, true>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, npairs, a, cbits,
-                           1ULL << t, m);
+                           1ULL << t, m, xm);
     else
         hipLaunchKernelGGL((k_gate_controlled<R, K, false>), dim3(blocks), dim3(256), 0, st, (cv<R>*)psi, npairs, a,
-                           cbits, 1ULL << t, m);
+                           cbits, 1ULL << t, m, xm);
     return hipGetLastError();
 }
 }  // namespace
