@@ -355,7 +355,12 @@ __global__ void __launch_bounds__(256) k_gate_mfma(cv<R>* __restrict__ psi, Mfma
         else return wreg[f];
     };
     const uint64_t xq = a.xq[q], yq = a.yq[q];
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // with a.xm, one super-batch per wave and block b takes the (b % 8)-th
+    // eighth of them (xcd_item's split: the super-batch's top bits are its
+    // amplitudes' top bits)
+    const uint64_t wave = a.xm ? (((uint64_t)(blockIdx.x & 7u) << (a.xm - 3)) |
+                                  ((uint64_t)(blockIdx.x >> 3) * (blockDim.x >> 6) + (threadIdx.x >> 6)))
+                               : ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const uint64_t nsuper = a.nbatches / UB;
     for (uint64_t sb0 = wave; sb0 < nsuper; sb0 += nwaves) {
@@ -679,6 +684,13 @@ hipError_t launch_mfma(void* psi, const MfmaArgs& args, hipStream_t st, const vo
     const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nsuper, target));
     const uint64_t blocks = (waves + 3) / 4;
     if (blocks > 0x7fffffffULL) return hipErrorInvalidValue;
+    // small states (up to kXcdStateMax): one super-batch per wave, in the
+    // XCD-consistent order of the other gate kernels
+    const uint64_t state_bytes = a.nbatches * 16 * 32 * sizeof(cv<R>);  // 2^n amplitudes
+    a.xm = (waves == nsuper && a.rot == 0 && a.bits >= 3 && blocks * 4 == nsuper && blocks % kXcds == 0 &&
+            state_bytes <= kXcdStateMax)
+               ? a.bits
+               : 0u;
     const bool nt = g_mfma_nt >= 0 ? g_mfma_nt != 0 : mfma_whole_lines(a, f64);
     if (nt)
         hipLaunchKernelGGL((k_gate_mfma<R, UB, WLDS, true>), dim3((unsigned)blocks), dim3(256), 0, st, (cv<R>*)psi, a,

@@ -113,6 +113,7 @@ struct MfmaArgs {
     uint64_t yq[4], yk[8];
     uint64_t nbatches;    // 2^(n-5) / 16
     uint32_t rot, bits;   // visit order of super-batches: rotate_index(i, rot, bits)
+    uint32_t xm;          // XCD-consistent split (one super-batch per wave, 2^xm of them), 0: off
 };
 
 // Left rotation of i inside its low `bits` bits: a bijection on [0, 2^bits).
@@ -166,6 +167,7 @@ inline void fill_mfma_args(MfmaArgs& a, int n, const uint32_t* q, bool f64, uint
     a.nbatches = (n >= 9) ? (1ULL << (n - 5)) / 16 : 0;
     a.rot = 0;
     a.bits = 0;
+    a.xm = 0;
 }
 
 // Requires n >= K + 6 and distinct targets < n.
