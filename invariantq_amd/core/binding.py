@@ -130,6 +130,7 @@ SIGNATURES = {
     "iq_backend_name": ([], ctypes.c_char_p),
     "iq_set_gate_path": ([ctypes.c_int], None),
     "iq_set_mfma_options": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], None),
+    "iq_set_entropy_options": ([ctypes.c_int], None),
     "iq_statevector_create_precision": ([_sz, ctypes.c_int], _vp),
     "iq_statevector_get_precision": ([_vp], ctypes.c_int),
     "iq_statevector_set_amplitudes": ([_vp, _d, _d], None),
@@ -572,6 +573,11 @@ def set_mfma_options(batches_per_iteration: int = 0, matrix_in_lds: int = -1, no
                      grid_waves: int = 0, batch_rotation: int = -1) -> None:
     """Matrix-core kernel options (zeros / -1 restore the defaults)."""
     lib().iq_set_mfma_options(batches_per_iteration, matrix_in_lds, nontemporal, grid_waves, batch_rotation)
+
+
+def set_entropy_options(device_jacobi_min_kept: int = 0) -> None:
+    """Kept qubits from which the entropy's Jacobi runs on the device (0: default)."""
+    lib().iq_set_entropy_options(device_jacobi_min_kept)
 
 
 def set_gate_path(path: str) -> None:

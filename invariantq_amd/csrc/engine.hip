@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -39,6 +40,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "jacobi_rows.hpp"
 #include "kernels.hpp"
 
 using namespace iqg;
@@ -1054,11 +1056,12 @@ void iq_statevector_get_cdf(void* sv, double* cdf_out) {
 // on the device and again on the host.
 constexpr size_t kMaxReducedQubits = 15;
 
-// Reduced density matrix of `qubits` (bit b of a row index = qubits[b]) in a
-// host buffer of 2 * 4^k doubles (re, im interleaved; not zero-filled first):
-// the partial trace of reference core/src/statevector.cpp:293-330, on the
-// matrix cores for 4 or more kept and 2 or more traced qubits.
-static bool reduced_matrix(Engine* e, const size_t* qubits, size_t k, std::unique_ptr<double[]>& red) {
+// Reduced density matrix of `qubits` (bit b of a row index = qubits[b]) in
+// the work buffer (kd x kd double2 at its start), with at least
+// `extra_doubles` of the buffer free after it: the partial trace of reference
+// core/src/statevector.cpp:293-330, on the matrix cores for 4 or more kept and
+// 2 or more traced qubits.
+static bool reduced_matrix_device(Engine* e, const size_t* qubits, size_t k, size_t extra_doubles) {
     if (k < 1 || k > kMaxReducedQubits || k > (size_t)e->n) {
         set_error(k > kMaxReducedQubits ? kErrUnsupported : kErrInvalidArgument,
                   "reduced matrices of 1..15 qubits (at most the state's) are supported");
@@ -1079,17 +1082,173 @@ static bool reduced_matrix(Engine* e, const size_t* qubits, size_t k, std::uniqu
     }
     const size_t kd = (size_t)1 << k;
     // reduced matrix, then (matrix-core path) the state reordered to (environment, kept)
-    if (!e->ensure_work(2 * kd * kd + 2 * e->dim)) return false;
+    if (!e->ensure_work(2 * kd * kd + std::max<size_t>(2 * e->dim, extra_doubles))) return false;
+    double2* out = reinterpret_cast<double2*>(e->work);
+    return hip_ok(launch_partial_trace(e->amp, e->n, e->prec, q, (int)k, out, out + kd * kd, e->stream),
+                  "partial trace");
+}
+
+// The same in a host buffer of 2 * 4^k doubles (re, im interleaved; not
+// zero-filled first).
+static bool reduced_matrix(Engine* e, const size_t* qubits, size_t k, std::unique_ptr<double[]>& red) {
+    if (!reduced_matrix_device(e, qubits, k, 0)) return false;
+    const size_t kd = (size_t)1 << k;
     red.reset(new (std::nothrow) double[2 * kd * kd]);
     if (!red) {
         set_error(kErrOutOfMemory, "cannot allocate %zu host doubles for the reduced matrix", 2 * kd * kd);
         return false;
     }
-    double2* out = reinterpret_cast<double2*>(e->work);
-    return hip_ok(launch_partial_trace(e->amp, e->n, e->prec, q, (int)k, out, out + kd * kd, e->stream),
-                  "partial trace") &&
-           e->download(e->work, red.get(), 2 * kd * kd);
+    return e->download(e->work, red.get(), 2 * kd * kd);
 }
+
+// jacobi_rows.hpp's backend on the engine's device: the matrix at the start
+// of the work buffer, the row list, the records and the shared candidate list
+// after it, with the same layout in the pinned staging buffer.  Each rotation
+// is one round trip: the row list up, the rotation and the scans, then one
+// copy down of the block that holds the new columns and the row records (and,
+// when rows have more than one candidate, a second of the shared list).
+struct DeviceRows {
+    static constexpr uint32_t kCap = 1u << 16;  // shared candidate list
+    Engine* e = nullptr;
+    double2* A = nullptr;
+    uint32_t d = 0;
+    unsigned char *dev = nullptr, *host = nullptr;  // same layout
+    // fixed parts: the shared list (t, j, value), the row list; then a block
+    // laid out per call (count of the shared list, row records, columns)
+    size_t off_ev = 0, off_et = 0, off_ej = 0, off_rows = 0, off_blk = 0;
+
+    static size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+    static size_t aux_bytes(size_t d) {
+        return 16 * (size_t)kCap + 8 * (size_t)kCap + align16(4 * d) + 16 + align16(8 * d) + 16 * d + 32 * d + 64;
+    }
+    static size_t aux_doubles(size_t d) { return (aux_bytes(d) + 7) / 8; }
+
+    bool init(Engine* eng, uint32_t dim) {
+        e = eng;
+        d = dim;
+        A = reinterpret_cast<double2*>(e->work);
+        const size_t aux = aux_doubles(d);
+        if (e->work_doubles < 2 * (size_t)d * d + aux) {
+            set_error(kErrOutOfMemory, "Jacobi records do not fit");
+            return false;
+        }
+        if (!e->ensure_pinned_doubles(aux)) return false;
+        dev = reinterpret_cast<unsigned char*>(e->work + 2 * (size_t)d * d);
+        host = reinterpret_cast<unsigned char*>(e->pinned);
+        off_ev = 0;
+        off_et = off_ev + 16 * (size_t)kCap;
+        off_ej = off_et + 4 * (size_t)kCap;
+        off_rows = off_ej + 4 * (size_t)kCap;
+        off_blk = off_rows + align16(4 * (size_t)d);
+        return true;
+    }
+    static uint32_t* u32(unsigned char* base, size_t off) { return reinterpret_cast<uint32_t*>(base + off); }
+    static double2* d2(unsigned char* base, size_t off) { return reinterpret_cast<double2*>(base + off); }
+    static const std::complex<double>* cx(unsigned char* base, size_t off) {
+        return reinterpret_cast<const std::complex<double>*>(base + off);
+    }
+
+    // the block for nrows records and q column entries: [count of the shared
+    // list | counts | first columns | first values | column p | column q]
+    struct Block {
+        size_t en, cnt, fj, fv, colp, colq, end;
+    };
+    static Block block(size_t nrows, size_t q) {
+        Block b;
+        b.en = 0;
+        b.cnt = 16;
+        b.fj = b.cnt + 4 * nrows;
+        b.fv = align16(b.fj + 4 * nrows);
+        b.colp = b.fv + 16 * nrows;
+        b.colq = b.colp + 16 * q;
+        b.end = b.colq + 16 * q;
+        return b;
+    }
+
+    bool diagonal(double* re) {
+        return hip_ok(hipMemcpy2DAsync(re, sizeof(double), A, ((size_t)d + 1) * sizeof(double2), sizeof(double), d,
+                                       hipMemcpyDeviceToHost, e->stream),
+                      "D2H diagonal") &&
+               e->sync();
+    }
+
+    bool upload_rows(const uint32_t* rows, size_t nrows) {
+        std::memcpy(host + off_rows, rows, nrows * 4);
+        return hip_ok(hipMemcpyAsync(dev + off_rows, host + off_rows, nrows * 4, hipMemcpyHostToDevice, e->stream),
+                      "H2D");
+    }
+
+    // scans of the listed rows into the block, then the block down and the wait
+    bool scans(size_t nrows, const Block& b) {
+        unsigned char* bd = dev + off_blk;
+        return hip_ok(launch_jacobi_scan(A, d, u32(dev, off_rows), (uint32_t)nrows, u32(bd, b.cnt),
+                                         u32(bd, b.fj), d2(bd, b.fv), u32(bd, b.en), kCap,
+                                         u32(dev, off_et), u32(dev, off_ej), d2(dev, off_ev),
+                                         e->stream),
+                      "Jacobi scan") &&
+               hip_ok(hipMemcpyAsync(host + off_blk, bd, b.end, hipMemcpyDeviceToHost, e->stream), "D2H") && e->sync();
+    }
+
+    // the shared list (when used) and the records' host view
+    bool finish(const Block& b, RowScan& out) {
+        unsigned char* bh = host + off_blk;
+        const uint32_t en = *u32(bh, b.en), take = std::min(en, kCap);
+        if (take &&
+            !(hip_ok(hipMemcpyAsync(host + off_ev, dev + off_ev, take * 16, hipMemcpyDeviceToHost, e->stream), "D2H") &&
+              hip_ok(hipMemcpyAsync(host + off_et, dev + off_et, take * 4, hipMemcpyDeviceToHost, e->stream), "D2H") &&
+              hip_ok(hipMemcpyAsync(host + off_ej, dev + off_ej, take * 4, hipMemcpyDeviceToHost, e->stream), "D2H") &&
+              e->sync()))
+            return false;
+        out.count = u32(bh, b.cnt);
+        out.first_j = u32(bh, b.fj);
+        out.first_v = cx(bh, b.fv);
+        out.extra_n = en;
+        out.extra_cap = kCap;
+        out.extra_t = u32(host, off_et);
+        out.extra_j = u32(host, off_ej);
+        out.extra_v = cx(host, off_ev);
+        return true;
+    }
+
+    bool scan(const uint32_t* rows, size_t nrows, RowScan& out) {
+        const Block b = block(nrows, 0);
+        return upload_rows(rows, nrows) &&
+               hip_ok(hipMemsetAsync(dev + off_blk + b.en, 0, 4, e->stream), "memset") && scans(nrows, b) &&
+               finish(b, out);
+    }
+
+    bool rotate_scan(size_t p, size_t q, double c, std::complex<double> sn, double npp, double nqq,
+                     const std::complex<double>*& colp, const std::complex<double>*& colq, const uint32_t* rows,
+                     size_t nrows, RowScan& out) {
+        const Block b = block(nrows, q);
+        unsigned char* bd = dev + off_blk;
+        // the new columns of rows i < q go into the block
+        if (!upload_rows(rows, nrows) ||
+            !hip_ok(launch_jacobi_rotate(A, d, (uint32_t)p, (uint32_t)q, c, sn.real(), sn.imag(), npp, nqq,
+                                         d2(bd, b.colp), d2(bd, b.colq), u32(bd, b.en),
+                                         (uint32_t)q, e->stream),
+                    "Jacobi rotation") ||
+            !scans(nrows, b))
+            return false;
+        colp = cx(host + off_blk, b.colp);
+        colq = cx(host + off_blk, b.colq);
+        return finish(b, out);
+    }
+
+    bool fetch_row(size_t i, std::complex<double>* out) {
+        const size_t len = (size_t)d - i - 1;
+        return hip_ok(hipMemcpyAsync(out, A + i * d + i + 1, len * 16, hipMemcpyDeviceToHost, e->stream), "D2H") &&
+               e->sync();
+    }
+};
+
+// Kept qubits from which the entropy's Jacobi runs on the device (the matrix
+// stays there).  Measured per call: 11 kept of 22 qubits 20 against 42 ms on
+// the host, 15 of 30 2.9 against 4.7 s; at 10 kept the host's row-maximum
+// Jacobi on a downloaded copy (7-35 ms) and the device (10-11 ms, about
+// 100 us per rotation round trip) are even, and below it the host wins.
+constexpr int kDeviceJacobiDefault = 11;
+static std::atomic<int> g_device_jacobi_min{kDeviceJacobiDefault};
 
 // the reduced-matrix scratch is given back once it passes 2 GiB (13 or more
 // kept qubits, or large gathered states), so one entropy call does not hold
@@ -1106,12 +1265,31 @@ double iq_statevector_entropy(void* sv, const size_t* qubits, size_t num_subsys_
     DeviceGuard g(e->dev);
     const size_t k = num_subsys_qubits;
     if (k == 0 || k == (size_t)e->n) return 0.0;
+    if ((int)k >= g_device_jacobi_min.load() && k <= kMaxReducedQubits) {
+        // rho_A stays on the device; the host keeps the row records
+        const uint32_t kd = 1u << k;
+        DeviceRows dev;
+        std::vector<double> ev;
+        const bool ok = reduced_matrix_device(e, qubits, k, DeviceRows::aux_doubles(kd)) && dev.init(e, kd) &&
+                        jacobi_eigenvalues_rows(dev, kd, 100, 1e-12, ev);
+        e->trim_work(kKeepWorkDoubles);
+        if (!ok) return std::numeric_limits<double>::quiet_NaN();
+        return entropy_bits(ev);
+    }
     std::unique_ptr<double[]> red;
     const bool ok = reduced_matrix(e, qubits, k, red);
     e->trim_work(kKeepWorkDoubles);
     if (!ok) return std::numeric_limits<double>::quiet_NaN();
     auto* A = reinterpret_cast<std::complex<double>*>(red.get());
     return entropy_bits(jacobi_eigenvalues_inplace(A, (size_t)1 << k, 100, 1e-12));
+}
+
+// Kept qubits from which the entropy's Jacobi runs on the device (1..16; 16
+// keeps every size on the host; 0 restores the default).  Both give the same
+// result bit for bit.
+void iq_set_entropy_options(int device_jacobi_min_kept) {
+    g_device_jacobi_min.store(device_jacobi_min_kept <= 0 ? kDeviceJacobiDefault
+                                                          : std::min(device_jacobi_min_kept, 16));
 }
 
 // Reduced density matrix export (no C export in the reference: partial_trace

@@ -37,13 +37,30 @@
 
 namespace iqg {
 
+constexpr double kJacobiSlack = 1.0 - 1e-14;  // squared-modulus filter, relative
+constexpr double kJacobiFloor = 1e-280;       // squared moduli below this: no filter
+
+// One rotation's parameters for pivot (p, q) of the capped Jacobi: the
+// cosine c and the complex sine sn that zero A(p, q), and the new diagonal
+// entries, from app = A(p, p), aqq = A(q, q) (real parts) and apq = A(p, q).
+inline void jacobi_rotation(double app, double aqq, std::complex<double> apq, double& c, std::complex<double>& sn,
+                            double& npp, double& nqq) {
+    using cplx = std::complex<double>;
+    const double th = 0.5 * std::atan2(2.0 * std::abs(apq), app - aqq);
+    const cplx ph = apq / std::abs(apq);
+    c = std::cos(th);
+    sn = std::sin(th) * ph;
+    const double cross = 2.0 * (c * std::conj(sn) * apq).real();
+    npp = c * c * app + std::norm(sn) * aqq + cross;
+    nqq = std::norm(sn) * app + c * c * aqq - cross;
+}
+
 // Eigenvalue estimates of the d x d Hermitian matrix A (row-major), which is
 // overwritten.
 inline std::vector<double> jacobi_eigenvalues_inplace(std::complex<double>* A, size_t d, int max_iter, double tol) {
     using cplx = std::complex<double>;
     auto at = [&](size_t i, size_t j) -> cplx& { return A[i * d + j]; };
-    constexpr double kSlack = 1.0 - 1e-14;  // squared-modulus filter, relative
-    constexpr double kFloor = 1e-280;       // squared moduli below this: no filter
+    constexpr double kSlack = kJacobiSlack, kFloor = kJacobiFloor;
     auto sq = [](const cplx& z) { return z.real() * z.real() + z.imag() * z.imag(); };
     std::vector<double> rmax(d, 0.0);  // largest |A(i, j)|, j > i
     std::vector<size_t> rarg(d, 0);    // first j attaining it
@@ -119,12 +136,9 @@ inline std::vector<double> jacobi_eigenvalues_inplace(std::complex<double>* A, s
                 q = rarg[i];
             }
         if (best < tol) break;
-        const double app = at(p, p).real(), aqq = at(q, q).real();
-        const cplx apq = at(p, q);
-        const double th = 0.5 * std::atan2(2.0 * std::abs(apq), app - aqq);
-        const cplx ph = apq / std::abs(apq);
-        const double c = std::cos(th);
-        const cplx sn = std::sin(th) * ph;
+        double c, npp, nqq;
+        cplx sn;
+        jacobi_rotation(at(p, p).real(), at(q, q).real(), at(p, q), c, sn, npp, nqq);
         for (size_t i = 0; i < d; ++i) {
             if (i == p || i == q) continue;
             const cplx aip = at(i, p), aiq = at(i, q);
@@ -133,9 +147,6 @@ inline std::vector<double> jacobi_eigenvalues_inplace(std::complex<double>* A, s
             at(i, q) = -sn * aip + c * aiq;
             at(q, i) = std::conj(at(i, q));
         }
-        const double cross = 2.0 * (c * std::conj(sn) * apq).real();
-        const double npp = c * c * app + std::norm(sn) * aqq + cross;
-        const double nqq = std::norm(sn) * app + c * c * aqq - cross;
         at(p, p) = npp;
         at(q, q) = nqq;
         at(p, q) = 0.0;

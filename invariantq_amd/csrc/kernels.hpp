@@ -95,5 +95,19 @@ hipError_t launch_search(const double* cdf, uint64_t count, const double* u, uin
 // in order on a complex128 state; otherwise one thread per element.
 hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t* keep, int k, double2* out,
                                 double2* scratch, hipStream_t st);
+// The capped Jacobi's device side (jacobi_rows.hpp) on a d x d Hermitian
+// matrix A (row-major double2):
+// rotate: the rotation (p, q) with cosine c and sine (sr, si), A(p, p) = npp,
+//   A(q, q) = nqq, A(p, q) = A(q, p) = 0; colp[i], colq[i] = the new A(i, p),
+//   A(i, q) for i < ncol; zeroes *extra_n for the scan that follows.
+// scan: the candidate entries of rows[t], t < nrows (jacobi_rows.hpp's
+//   RowScan: count, first candidate, the others appended to the shared list
+//   of `cap` entries through *extra_n).
+hipError_t launch_jacobi_rotate(double2* A, uint32_t d, uint32_t p, uint32_t q, double c, double sr, double si,
+                                double npp, double nqq, double2* colp, double2* colq, uint32_t* extra_n, uint32_t ncol,
+                                hipStream_t st);
+hipError_t launch_jacobi_scan(const double2* A, uint32_t d, const uint32_t* rows, uint32_t nrows, uint32_t* count,
+                              uint32_t* first_j, double2* first_v, uint32_t* extra_n, uint32_t cap, uint32_t* extra_t,
+                              uint32_t* extra_j, double2* extra_v, hipStream_t st);
 
 }  // namespace iqg

@@ -4,6 +4,7 @@
 // partial per block, and a single-block finaliser sums the partials in a fixed
 // order: results are bit-reproducible run to run (no atomics on the result path).
 
+#include "jacobi_rows.hpp"
 #include "kernels.hpp"
 #include "memio.hpp"
 #include "reduce.hpp"
@@ -790,6 +791,137 @@ __global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict_
             }
 }
 
+// The capped Jacobi's rotation (jacobi_rows.hpp's backend, the loop of
+// jacobi.hpp): one thread per row i != p, q.  The complex products are
+// rounded as the host's complex arithmetic rounds them (each real product
+// rounded, then added), with no fused multiply-add, so the device's matrix
+// stays bit for bit the host's.
+__global__ void __launch_bounds__(256) k_jacobi_rotate(double2* __restrict__ A, uint32_t d, uint32_t p, uint32_t q,
+                                                       double c, double sr, double si, double npp, double nqq,
+                                                       double2* __restrict__ colp, double2* __restrict__ colq,
+                                                       uint32_t* __restrict__ extra_n, uint32_t ncol) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *extra_n = 0;
+    if (i >= d) return;
+    const size_t D = d;
+    if (i == p) {
+        A[(size_t)p * D + p] = make_double2(npp, 0.0);
+        A[(size_t)p * D + q] = make_double2(0.0, 0.0);
+        A[(size_t)q * D + p] = make_double2(0.0, 0.0);
+        return;
+    }
+    if (i == q) {
+        A[(size_t)q * D + q] = make_double2(nqq, 0.0);
+        return;
+    }
+    const double2 aip = A[(size_t)i * D + p], aiq = A[(size_t)i * D + q];
+    // c aip + conj(sn) aiq
+    double2 np;
+    np.x = c * aip.x + (sr * aiq.x - (-si) * aiq.y);
+    np.y = c * aip.y + (sr * aiq.y + (-si) * aiq.x);
+    // -sn aip + c aiq
+    double2 nq;
+    nq.x = ((-sr) * aip.x - (-si) * aip.y) + c * aiq.x;
+    nq.y = ((-sr) * aip.y + (-si) * aip.x) + c * aiq.y;
+    A[(size_t)i * D + p] = np;
+    A[(size_t)p * D + i] = make_double2(np.x, -np.y);
+    A[(size_t)i * D + q] = nq;
+    A[(size_t)q * D + i] = make_double2(nq.x, -nq.y);
+    if (i < ncol) {
+        colp[i] = np;
+        colq[i] = nq;
+    }
+}
+
+// Candidates of the listed rows, one workgroup per row: the largest squared
+// modulus right of the diagonal (top), then every entry at or above
+// top * (1 - 1e-14) (every non-zero entry when top is below 1e-280), as
+// jacobi_rows.hpp defines them.  More than kRowCands candidates: when they all
+// share one unordered pair (|re|, |im|) (hence one std::abs) the row's answer
+// is the first of them, else the host reads the row.
+__global__ void __launch_bounds__(256) k_jacobi_scan(const double2* __restrict__ A, uint32_t d,
+                                                     const uint32_t* __restrict__ rows, uint32_t nrows,
+                                                     uint32_t* __restrict__ count, uint32_t* __restrict__ first_j,
+                                                     double2* __restrict__ first_v, uint32_t* __restrict__ extra_n,
+                                                     uint32_t cap, uint32_t* __restrict__ extra_t,
+                                                     uint32_t* __restrict__ extra_j, double2* __restrict__ extra_v) {
+#pragma clang fp contract(off)
+    __shared__ double s_top[4];
+    __shared__ uint32_t s_n, s_first, s_mixed, s_base;
+    __shared__ uint32_t s_j[kRowCands];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t t = blockIdx.x; t < nrows; t += gridDim.x) {
+        const uint32_t i = rows[t];
+        const double2* row = A + (size_t)i * d;
+        double top = 0.0;
+        for (uint32_t j = i + 1 + tid; j < d; j += 256) {
+            const double2 z = row[j];
+            top = fmax(top, z.x * z.x + z.y * z.y);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) top = fmax(top, __shfl_xor(top, off, 64));
+        if (lane == 0) s_top[wave] = top;
+        if (tid == 0) {
+            s_n = 0;
+            s_first = 0xffffffffu;
+            s_mixed = 0;
+        }
+        __syncthreads();
+        top = fmax(fmax(s_top[0], s_top[1]), fmax(s_top[2], s_top[3]));
+        const double bound = top >= kJacobiFloor ? top * kJacobiSlack : -1.0;
+        for (uint32_t j = i + 1 + tid; j < d; j += 256) {
+            const double2 z = row[j];
+            if (z.x * z.x + z.y * z.y >= bound && (z.x != 0.0 || z.y != 0.0)) {
+                const uint32_t k = atomicAdd(&s_n, 1u);
+                if (k < kRowCands) s_j[k] = j;
+                atomicMin(&s_first, j);
+            }
+        }
+        __syncthreads();
+        const uint32_t n = s_n;
+        if (n > kRowCands) {
+            const double2 f = row[s_first];
+            const double fa = fmax(fabs(f.x), fabs(f.y)), fb = fmin(fabs(f.x), fabs(f.y));
+            for (uint32_t j = i + 1 + tid; j < d; j += 256) {
+                const double2 z = row[j];
+                if (z.x * z.x + z.y * z.y >= bound && (z.x != 0.0 || z.y != 0.0) &&
+                    (fmax(fabs(z.x), fabs(z.y)) != fa || fmin(fabs(z.x), fabs(z.y)) != fb))
+                    atomicOr(&s_mixed, 1u);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                if (s_mixed) {
+                    count[t] = kRowFetch;
+                } else {
+                    count[t] = 1;
+                    first_j[t] = s_first;
+                    first_v[t] = f;
+                }
+            }
+        } else {
+            if (tid == 0) {
+                count[t] = n;
+                if (n) {
+                    first_j[t] = s_j[0];
+                    first_v[t] = row[s_j[0]];
+                }
+                s_base = n > 1 ? atomicAdd(extra_n, n - 1) : 0u;
+            }
+            __syncthreads();
+            if (tid >= 1 && tid < n) {
+                const uint32_t k = s_base + tid - 1;
+                if (k < cap) {
+                    extra_t[k] = t;
+                    extra_j[k] = s_j[tid];
+                    extra_v[k] = row[s_j[tid]];
+                }
+            }
+        }
+        __syncthreads();  // the shared values are reset for the next row
+    }
+}
+
 // Gate-matrix upload on the compute stream: reads pinned (coherent, uncached)
 // host memory over PCIe and writes device memory, so the gate kernel that
 // follows needs no copy-engine round trip.  One 16-byte element per thread
@@ -1068,6 +1200,25 @@ hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t
     if (prec == kC128)
         hipLaunchKernelGGL(k_partial_trace<double>, dim3(b), dim3(256), 0, st, (const double2*)psi, kl, k, envmask, out);
     else hipLaunchKernelGGL(k_partial_trace<float>, dim3(b), dim3(256), 0, st, (const float2*)psi, kl, k, envmask, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_rotate(double2* A, uint32_t d, uint32_t p, uint32_t q, double c, double sr, double si,
+                                double npp, double nqq, double2* colp, double2* colq, uint32_t* extra_n, uint32_t ncol,
+                                hipStream_t st) {
+    if (d < 2 || p >= d || q >= d || p == q || ncol > d) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_jacobi_rotate, dim3((d + 255) / 256), dim3(256), 0, st, A, d, p, q, c, sr, si, npp, nqq, colp,
+                       colq, extra_n, ncol);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_scan(const double2* A, uint32_t d, const uint32_t* rows, uint32_t nrows, uint32_t* count,
+                              uint32_t* first_j, double2* first_v, uint32_t* extra_n, uint32_t cap, uint32_t* extra_t,
+                              uint32_t* extra_j, double2* extra_v, hipStream_t st) {
+    if (nrows == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>(nrows, 16384u);
+    hipLaunchKernelGGL(k_jacobi_scan, dim3(blocks), dim3(256), 0, st, A, d, rows, nrows, count, first_j, first_v,
+                       extra_n, cap, extra_t, extra_j, extra_v);
     return hipGetLastError();
 }
 

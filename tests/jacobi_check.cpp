@@ -4,8 +4,13 @@
 // matrices, rank-deficient density matrices, matrices with exactly tied and
 // 1-ulp-apart off-diagonal moduli, matrices whose squared moduli underflow,
 // pairs with equal std::abs but differently ordered squares (the edge of the
-// squared-modulus filter), and a 1024 x 1024 half-cut reduced state.  Prints
-// "ok" or the first mismatch.
+// squared-modulus filter), and a 1024 x 1024 half-cut reduced state.  The same
+// cases go through jacobi_rows.hpp's driver (the device form: the host holds
+// per-row records, a backend rotates and scans rows) with a host backend that
+// does what the device kernels do: candidate lists capped per row, one shared
+// list of further candidates (with a small cap, so that it overflows), rows
+// of one repeated modulus pair reduced to their first entry, fetched rows.
+// Prints "ok" or the first mismatch.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,9 +18,126 @@
 #include <vector>
 
 #include "jacobi.hpp"
+#include "jacobi_rows.hpp"
+
+// Host stand-in for the device's rotate and scan kernels (the semantics of
+// k_jacobi_rotate / k_jacobi_scan), on a private copy of the matrix.
+struct HostRows {
+    using cplx = std::complex<double>;
+    std::vector<cplx> A;
+    size_t d;
+    uint32_t cap;
+    std::vector<uint32_t> cnt, fj, et, ej;
+    std::vector<cplx> fv, ev_, colp, colq;
+    size_t fetches = 0, uniform = 0, overflowed = 0;
+    HostRows(const std::vector<cplx>& M, size_t d_, uint32_t cap_) : A(M), d(d_), cap(cap_) {}
+    cplx& at(size_t i, size_t j) { return A[i * d + j]; }
+    bool diagonal(double* re) {
+        for (size_t i = 0; i < d; ++i) re[i] = at(i, i).real();
+        return true;
+    }
+    bool scan(const uint32_t* rows, size_t nrows, iqg::RowScan& out) {
+        cnt.assign(nrows, 0);
+        fj.assign(nrows, 0);
+        fv.assign(nrows, 0.0);
+        et.clear();
+        ej.clear();
+        ev_.clear();
+        uint32_t en = 0;
+        for (size_t t = 0; t < nrows; ++t) {
+            const size_t i = rows[t];
+            double top = 0.0;
+            for (size_t j = i + 1; j < d; ++j) {
+                const cplx z = at(i, j);
+                top = std::max(top, z.real() * z.real() + z.imag() * z.imag());
+            }
+            const double bound = top >= iqg::kJacobiFloor ? top * iqg::kJacobiSlack : -1.0;
+            std::vector<size_t> c;
+            for (size_t j = i + 1; j < d; ++j) {
+                const cplx z = at(i, j);
+                const double s2 = z.real() * z.real() + z.imag() * z.imag();
+                if (s2 >= bound && (z.real() != 0.0 || z.imag() != 0.0)) c.push_back(j);
+            }
+            // the device's candidates arrive in no particular order
+            std::reverse(c.begin(), c.end());
+            if (c.size() > iqg::kRowCands) {
+                size_t first = d;
+                for (size_t j : c) first = std::min(first, j);
+                auto key = [&](size_t j) {
+                    const double a = std::fabs(at(i, j).real()), b = std::fabs(at(i, j).imag());
+                    return std::make_pair(std::max(a, b), std::min(a, b));
+                };
+                bool same = true;
+                for (size_t j : c) same &= key(j) == key(first);
+                if (same) {
+                    cnt[t] = 1;
+                    fj[t] = (uint32_t)first;
+                    fv[t] = at(i, first);
+                    ++uniform;
+                } else {
+                    cnt[t] = iqg::kRowFetch;
+                }
+                continue;
+            }
+            cnt[t] = (uint32_t)c.size();
+            for (size_t k = 0; k < c.size(); ++k) {
+                if (k == 0) {
+                    fj[t] = (uint32_t)c[k];
+                    fv[t] = at(i, c[k]);
+                } else {
+                    if (en < cap) {
+                        et.push_back((uint32_t)t);
+                        ej.push_back((uint32_t)c[k]);
+                        ev_.push_back(at(i, c[k]));
+                    }
+                    ++en;
+                }
+            }
+        }
+        out.count = cnt.data();
+        out.first_j = fj.data();
+        out.first_v = fv.data();
+        if (en > cap) ++overflowed;
+        out.extra_n = en;
+        out.extra_cap = cap;
+        out.extra_t = et.data();
+        out.extra_j = ej.data();
+        out.extra_v = ev_.data();
+        return true;
+    }
+    bool rotate_scan(size_t p, size_t q, double c, cplx sn, double npp, double nqq, const cplx*& cp, const cplx*& cq,
+                     const uint32_t* rows, size_t nrows, iqg::RowScan& out) {
+        colp.assign(d, 0.0);
+        colq.assign(d, 0.0);
+        for (size_t i = 0; i < d; ++i) {
+            if (i == p || i == q) continue;
+            const cplx aip = at(i, p), aiq = at(i, q);
+            at(i, p) = c * aip + std::conj(sn) * aiq;
+            at(p, i) = std::conj(at(i, p));
+            at(i, q) = -sn * aip + c * aiq;
+            at(q, i) = std::conj(at(i, q));
+            colp[i] = at(i, p);
+            colq[i] = at(i, q);
+        }
+        at(p, p) = npp;
+        at(q, q) = nqq;
+        at(p, q) = 0.0;
+        at(q, p) = 0.0;
+        cp = colp.data();
+        cq = colq.data();
+        return scan(rows, nrows, out);
+    }
+    bool fetch_row(size_t i, cplx* out) {
+        ++fetches;
+        for (size_t j = i + 1; j < d; ++j) out[j - i - 1] = at(i, j);
+        return true;
+    }
+};
 
 extern "C" void iqo_hermitian_eigenvalues(const double* re, const double* im, size_t d, int max_iter, double tol,
                                           double* out);
+
+static size_t g_fetched = 0, g_uniform = 0, g_overflowed = 0;
 
 static int check(const std::vector<std::complex<double>>& A, size_t d, int max_iter, double tol, const char* what) {
     std::vector<double> re(d * d), im(d * d), ref(d);
@@ -25,15 +147,32 @@ static int check(const std::vector<std::complex<double>>& A, size_t d, int max_i
     }
     iqo_hermitian_eigenvalues(re.data(), im.data(), d, max_iter, tol, ref.data());
     const auto got = iqg::jacobi_eigenvalues(A, d, max_iter, tol);
-    if (std::memcmp(got.data(), ref.data(), d * sizeof(double)) != 0) {
+    auto same = [&](const std::vector<double>& v, const char* form) {
+        if (v.size() == d && std::memcmp(v.data(), ref.data(), d * sizeof(double)) == 0) return 0;
         for (size_t i = 0; i < d; ++i)
-            if (got[i] != ref[i]) {
-                printf("mismatch %s d=%zu iters=%d at %zu: %.17g vs %.17g\n", what, d, max_iter, i, got[i], ref[i]);
+            if (i >= v.size() || v[i] != ref[i]) {
+                printf("mismatch %s (%s) d=%zu iters=%d at %zu: %.17g vs %.17g\n", what, form, d, max_iter, i,
+                       i < v.size() ? v[i] : NAN, ref[i]);
                 break;
             }
         return 1;
+    };
+    int bad = same(got, "row maxima");
+    // the device form, with a shared candidate list of 40 (overflows on tied
+    // matrices) and of 1 << 20
+    for (uint32_t cap : {40u, 1u << 20}) {
+        HostRows dev(A, d, cap);
+        std::vector<double> rows_ev;
+        if (!iqg::jacobi_eigenvalues_rows(dev, d, max_iter, tol, rows_ev)) {
+            printf("device form failed %s d=%zu\n", what, d);
+            return 1;
+        }
+        bad |= same(rows_ev, cap == 40u ? "rows, cap 40" : "rows");
+        g_fetched += dev.fetches;
+        g_uniform += dev.uniform;
+        g_overflowed += dev.overflowed;
     }
-    return 0;
+    return bad;
 }
 
 int main() {
@@ -195,6 +334,13 @@ int main() {
         printf("%d of %d cases differ\n", bad, cases);
         return 1;
     }
-    printf("ok %d cases\n", cases);
+    // each path of the device form was taken
+    if (!g_fetched || !g_uniform || !g_overflowed) {
+        printf("device form: fetched %zu, uniform %zu, overflowed %zu: a path was not exercised\n", g_fetched, g_uniform,
+               g_overflowed);
+        return 1;
+    }
+    printf("ok %d cases (device form: %zu rows fetched, %zu tied rows reduced, %zu overflowed lists)\n", cases,
+           g_fetched, g_uniform, g_overflowed);
     return 0;
 }

@@ -135,3 +135,58 @@ def test_reduced_matrix_blocked_kernel_off_diagonal_blocks(oracle_lib):
     got, want = sv.reduced_density_matrix(keep), ref.partial_trace(keep)
     assert np.max(np.abs(got - want)) < 1e-13
     assert np.max(np.abs(got - got.conj().T)) < 1e-15
+
+
+def _entropy_both_ways(sv, keep):
+    """The entropy with the Jacobi on the device (rho_A never leaves it) and
+    on the host (rho_A downloaded), same state, same rho_A."""
+    from invariantq_amd.core import binding
+    try:
+        binding.set_entropy_options(1)
+        dev = sv.von_neumann_entropy(keep)
+        binding.set_entropy_options(16)
+        host = sv.von_neumann_entropy(keep)
+    finally:
+        binding.set_entropy_options(0)
+    return dev, host
+
+
+@pytest.mark.parametrize("case", ["brick", "deep", "plus", "ghz", "sparse", "complex64"])
+def test_device_jacobi_matches_host_bit_for_bit(oracle_lib, case):
+    """The device Jacobi (rows rotated and scanned on the GPU, pivots and
+    moduli on the host) against the host's row-maximum Jacobi, which
+    tests/jacobi_check.cpp pins bit for bit to the oracle's full scan: the
+    same entropy to the last bit.  The states cover a low-rank rho_A (most
+    rows rescanned per rotation), a near-maximally mixed one, rows of one
+    repeated modulus (|+>^n: every candidate ties), zero rows (GHZ, sparse),
+    and complex64 amplitudes gathered into FP64."""
+    from invariantq_amd.core.binding import NativeStateVector
+    rng = np.random.default_rng(77)
+    if case in ("brick", "deep", "complex64"):
+        n = 18
+        sv, _ = states(oracle_lib, n, "complex64" if case == "complex64" else "complex128",
+                       seed=5 if case == "brick" else 6)
+        if case == "deep":
+            for op in workloads.random_u3_cx(n, 14, seed=66):
+                if len(op.qubits) == 1:
+                    sv.apply_1q(op.qubits[0], op.matrix)
+                else:
+                    sv.apply_2q(op.qubits[0], op.qubits[1], op.matrix)
+        keeps = [list(range(9)), [int(q) for q in rng.permutation(n)[:12]]]
+    else:
+        n = 16
+        if case == "plus":
+            psi = np.full(1 << n, 2.0 ** (-n / 2), dtype=complex)
+        elif case == "ghz":
+            psi = np.zeros(1 << n, dtype=complex)
+            psi[0] = psi[-1] = np.sqrt(0.5)
+        else:
+            psi, _ = _sparse_state(n, 64, rng)
+        sv = NativeStateVector(n)
+        sv.set_amplitudes(psi)
+        keeps = [[3, 0, 9, 12, 5, 14, 1, 7], list(range(12))]
+    for keep in keeps:
+        print(f"device Jacobi {case} n={n} k={len(keep)}", flush=True)
+        dev, host = _entropy_both_ways(sv, keep)
+        assert np.isfinite(dev)
+        assert dev == host, (dev, host)
