@@ -173,7 +173,7 @@ void   iq_statevector_sample_tasks(void* sv, size_t ntasks, const uint64_t* seed
 void   iq_statevector_reduced_density_matrix(void* sv, const size_t* qubits, size_t k, double* re_out,
                                              double* im_out);
 /* Where the entropy's capped Jacobi runs: on the device from `device_jacobi_min_kept` kept
- * qubits (1..16, 16 = always on the host; 0 restores the default, 11).  No reference
+ * qubits (1..16, 16 = always on the host; 0 restores the default, 10).  No reference
  * counterpart (its Jacobi is serial, core/src/statevector.cpp:333-393); both placements give
  * the same entropy bit for bit. */
 void   iq_set_entropy_options(int device_jacobi_min_kept);

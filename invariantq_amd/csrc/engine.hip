@@ -1165,11 +1165,16 @@ struct DeviceRows {
         return b;
     }
 
+    // through the block area (a strided 8-byte copy of the diagonal took 57 ms
+    // at d = 8192)
     bool diagonal(double* re) {
-        return hip_ok(hipMemcpy2DAsync(re, sizeof(double), A, ((size_t)d + 1) * sizeof(double2), sizeof(double), d,
-                                       hipMemcpyDeviceToHost, e->stream),
-                      "D2H diagonal") &&
-               e->sync();
+        if (!hip_ok(launch_diag_real(A, d, reinterpret_cast<double*>(dev + off_blk), e->stream), "diagonal") ||
+            !hip_ok(hipMemcpyAsync(host + off_blk, dev + off_blk, (size_t)d * 8, hipMemcpyDeviceToHost, e->stream),
+                    "D2H") ||
+            !e->sync())
+            return false;
+        std::memcpy(re, host + off_blk, (size_t)d * 8);
+        return true;
     }
 
     bool upload_rows(const uint32_t* rows, size_t nrows) {
@@ -1243,11 +1248,11 @@ struct DeviceRows {
 };
 
 // Kept qubits from which the entropy's Jacobi runs on the device (the matrix
-// stays there).  Measured per call: 11 kept of 22 qubits 20 against 42 ms on
-// the host, 15 of 30 2.9 against 4.7 s; at 10 kept the host's row-maximum
-// Jacobi on a downloaded copy (7-35 ms) and the device (10-11 ms, about
-// 100 us per rotation round trip) are even, and below it the host wins.
-constexpr int kDeviceJacobiDefault = 11;
+// stays there).  Measured per call: 10 kept of 20 qubits 3.0-3.8 ms against
+// 7.3-37 ms for the host's row-maximum Jacobi on a downloaded copy, 13 of 26
+// 55 against 301 ms, 15 of 30 2.6 against 4.6 s; at 8 kept the host is ahead
+// (2.8 against 3.3 ms: a rotation's device round trip is about 100 us).
+constexpr int kDeviceJacobiDefault = 10;
 static std::atomic<int> g_device_jacobi_min{kDeviceJacobiDefault};
 
 // the reduced-matrix scratch is given back once it passes 2 GiB (13 or more

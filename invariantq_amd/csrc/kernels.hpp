@@ -106,6 +106,8 @@ hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t
 hipError_t launch_jacobi_rotate(double2* A, uint32_t d, uint32_t p, uint32_t q, double c, double sr, double si,
                                 double npp, double nqq, double2* colp, double2* colq, uint32_t* extra_n, uint32_t ncol,
                                 hipStream_t st);
+// out[i] = Re A(i, i) for a d x d row-major double2 matrix.
+hipError_t launch_diag_real(const double2* A, uint32_t d, double* out, hipStream_t st);
 hipError_t launch_jacobi_scan(const double2* A, uint32_t d, const uint32_t* rows, uint32_t nrows, uint32_t* count,
                               uint32_t* first_j, double2* first_v, uint32_t* extra_n, uint32_t cap, uint32_t* extra_t,
                               uint32_t* extra_j, double2* extra_v, hipStream_t st);

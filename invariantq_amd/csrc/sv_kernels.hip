@@ -724,16 +724,39 @@ __global__ void __launch_bounds__(64) k_herk_mfma(const double2* __restrict__ ph
 // contiguous load), and each wave accumulates a 32x32 quarter of the block
 // as 2x2 tiles of v_mfma_f64_16x16x4 with the operand and result layouts of
 // k_herk_mfma.  A strip element is read from memory once per block instead
-// of once per 16x16 tile.  Blocks on the diagonal compute all four quarters
-// (the lower half agrees with the conjugate of the upper to rounding: its
-// imaginary parts interleave the two product sets in the other order); the
-// others also write their conjugate transpose.
+// of once per 16x16 tile.  Off-diagonal blocks also write their conjugate
+// transpose; in a diagonal block only the entries on or above the diagonal
+// are kept, each written with its conjugate transpose, and the diagonal's
+// imaginary parts (zero in exact arithmetic) as zero, so the matrix is
+// exactly Hermitian.
+// Block order (XT, from 8 x 8 blocks): workgroup b runs on XCD b % 8, and
+// XCD x takes the super-tiles x, x + 8, ... of 8 x 8 blocks of the upper
+// triangle, one at a time, so the blocks that run together on one XCD share
+// 16 strips through its L2 (26 qubits, 13 kept: 41 against 48 ms).  The
+// three-multiply form (Re = Xr^T Xr + Xi^T Xi, Im = (Xr + Xi)^T (Xr - Xi) -
+// Xr^T Xr + Xi^T Xi) was measured no faster: its third accumulator set costs
+// a wave per SIMD, and the kernel is not bound by the matrix cores.
 constexpr uint32_t kHerkBlock = 64, kHerkRows = 16;
 
+template <bool XT>
 __global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict__ phi, uint64_t m, uint32_t d,
-                                                      double2* __restrict__ out) {
+                                                      double2* __restrict__ out, uint32_t ns) {
     typedef double v4 __attribute__((ext_vector_type(4)));
-    const uint32_t bR = blockIdx.y, bC = blockIdx.x;
+    uint32_t bR, bC;
+    if (XT) {
+        const uint32_t b = blockIdx.x, x = b & 7u, s = b >> 3, w = s & 63u;
+        uint32_t g = (s >> 6) * 8u + x, sr = 0;  // super-tile g of the upper triangle, row by row
+        while (sr < ns && g >= ns - sr) {
+            g -= ns - sr;
+            ++sr;
+        }
+        if (sr >= ns) return;
+        bR = sr * 8u + (w >> 3);
+        bC = (sr + g) * 8u + (w & 7u);
+    } else {
+        bR = blockIdx.y;
+        bC = blockIdx.x;
+    }
     if (bC < bR) return;
     __shared__ double2 sa[kHerkRows][kHerkBlock], sb[kHerkRows][kHerkBlock];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -786,9 +809,17 @@ __global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict_
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) {
                 const uint64_t row = r0 + wr * 32 + ta * 16 + kk + 4 * jj, col = c0 + wc * 32 + tb * 16 + j;
-                out[row * d + col] = make_double2(re[ta][tb][jj], im[ta][tb][jj]);
-                if (!diag) out[col * d + row] = make_double2(re[ta][tb][jj], -im[ta][tb][jj]);
+                if (diag && row > col) continue;  // written by the transposed entry
+                const double vr = re[ta][tb][jj], vi = row == col ? 0.0 : im[ta][tb][jj];
+                out[row * d + col] = make_double2(vr, vi);
+                if (row != col) out[col * d + row] = make_double2(vr, -vi);
             }
+}
+
+// Real parts of the diagonal of a d x d matrix (double2, row-major) into out.
+__global__ void __launch_bounds__(256) k_diag_real(const double2* __restrict__ A, uint32_t d, double* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < d) out[i] = A[(size_t)i * d + i].x;
 }
 
 // The capped Jacobi's rotation (jacobi_rows.hpp's backend, the loop of
@@ -1188,7 +1219,13 @@ hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t
         }
         if (k >= 6 && n - k >= 4) {  // d a multiple of 64, m of 16
             const uint32_t nb = d / kHerkBlock;
-            hipLaunchKernelGGL(k_herk_blocked, dim3(nb, nb), dim3(256), 0, st, phi, m, d, out);
+            if (nb >= 8) {
+                const uint32_t ns = nb / 8, tiles = ns * (ns + 1) / 2;
+                const uint32_t blocks = ((tiles + 7) / 8) * 8 * 64;
+                hipLaunchKernelGGL(k_herk_blocked<true>, dim3(blocks), dim3(256), 0, st, phi, m, d, out, ns);
+            } else {
+                hipLaunchKernelGGL(k_herk_blocked<false>, dim3(nb, nb), dim3(256), 0, st, phi, m, d, out, 0u);
+            }
         } else {
             const uint32_t nt = d / 16;
             hipLaunchKernelGGL(k_herk_mfma, dim3(nt, nt), dim3(64), 0, st, phi, m, d, out);
@@ -1219,6 +1256,12 @@ hipError_t launch_jacobi_scan(const double2* A, uint32_t d, const uint32_t* rows
     const uint32_t blocks = std::min<uint32_t>(nrows, 16384u);
     hipLaunchKernelGGL(k_jacobi_scan, dim3(blocks), dim3(256), 0, st, A, d, rows, nrows, count, first_j, first_v,
                        extra_n, cap, extra_t, extra_j, extra_v);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_real(const double2* A, uint32_t d, double* out, hipStream_t st) {
+    if (d == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_diag_real, dim3((d + 255) / 256), dim3(256), 0, st, A, d, out);
     return hipGetLastError();
 }
 
