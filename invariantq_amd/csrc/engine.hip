@@ -1081,10 +1081,21 @@ static bool reduced_matrix_device(Engine* e, const size_t* qubits, size_t k, siz
             }
     }
     const size_t kd = (size_t)1 << k;
-    // reduced matrix, then (matrix-core path) the state reordered to (environment, kept)
-    if (!e->ensure_work(2 * kd * kd + std::max<size_t>(2 * e->dim, extra_doubles))) return false;
+    // reduced matrix, then the partial sums of the environment chunks and
+    // (matrix-core path, unless the kept qubits are 0..k-1 of a complex128
+    // state) the state reordered to (environment, kept)
+    bool in_order = e->prec == kC128;
+    for (size_t i = 0; i < k; ++i) in_order &= q[i] == (uint32_t)i;
+    const bool gather = k >= 4 && (size_t)e->n - k >= 2 && !in_order;
+    const size_t parts = k <= 3   ? 2048 * (kd * (kd + 1) / 2)
+                         : k <= 5  ? 2048 * kd * kd
+                         : k <= 10 ? std::min<size_t>(1u << 22, 512 * kd * kd)
+                                   : 0;
+    const size_t after = 2 * ((gather ? e->dim : 0) + parts);
+    if (!e->ensure_work(2 * kd * kd + std::max<size_t>(after, extra_doubles))) return false;
     double2* out = reinterpret_cast<double2*>(e->work);
-    return hip_ok(launch_partial_trace(e->amp, e->n, e->prec, q, (int)k, out, out + kd * kd, e->stream),
+    return hip_ok(launch_partial_trace(e->amp, e->n, e->prec, q, (int)k, out, out + kd * kd,
+                                       (e->work_doubles - 2 * kd * kd) / 2, e->stream),
                   "partial trace");
 }
 

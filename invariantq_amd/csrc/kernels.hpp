@@ -88,13 +88,16 @@ hipError_t launch_sample_chunks(const void* psi, uint64_t count, int prec, const
 hipError_t launch_cdf(const void* psi, uint64_t count, int prec, double* cdf, double* scratch, hipStream_t st);
 hipError_t launch_search(const double* cdf, uint64_t count, const double* u, uint64_t shots, uint64_t* out, hipStream_t st);
 
-// Reduced density matrix of `keep` (LSB-first local index), complex128 out [kd * kd].
 // Reduced density matrix of the kept qubits (bit b of a row index = qubit
-// keep[b]), d x d row-major.  With k >= 4 kept and >= 2 traced qubits it runs
-// on the matrix cores and needs `scratch` (2^n double2) unless keep is 0..k-1
-// in order on a complex128 state; otherwise one thread per element.
+// keep[b]), d x d row-major, exactly Hermitian.  `scratch` (scratch_elems
+// double2) holds the workgroups' partial sums, and for 4 or more kept qubits
+// the state reordered to (environment, kept) (2^n double2) unless keep is
+// 0..k-1 in order on a complex128 state.  1-3 kept: the environment sum over
+// many workgroups; 4 or more (and 2 or more traced): the FP64 matrix cores,
+// split over environment chunks when the matrix alone gives few workgroups;
+// otherwise one thread per element.
 hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t* keep, int k, double2* out,
-                                double2* scratch, hipStream_t st);
+                                double2* scratch, uint64_t scratch_elems, hipStream_t st);
 // The capped Jacobi's device side (jacobi_rows.hpp) on a d x d Hermitian
 // matrix A (row-major double2):
 // rotate: the rotation (p, q) with cosine c and sine (sr, si), A(p, p) = npp,

@@ -692,17 +692,23 @@ __global__ void __launch_bounds__(256) k_gather_keep(const cv<R>* __restrict__ p
 //   Re += Xr^T Xr + Xi^T Xi,  Im += Xi^T Xr - Xr^T Xi.
 // Operand lanes: A holds row (lane % 16), k (lane / 16); B holds k (lane / 16),
 // column (lane % 16); result value jj of a lane is row lane/16 + 4 jj, column
-// lane % 16.  Off-diagonal tiles also write their conjugate transpose.
+// lane % 16.  Off-diagonal tiles also write their conjugate transpose; a
+// diagonal tile keeps its entries on or above the diagonal, each written with
+// its conjugate transpose, and zero imaginary parts on the diagonal.
+// Environment chunk blockIdx.z (rows [z * mchunk, (z + 1) * mchunk)) writes
+// its partial matrix at out + z * slice.
 __global__ void __launch_bounds__(64) k_herk_mfma(const double2* __restrict__ phi, uint64_t m, uint32_t d,
-                                                  double2* __restrict__ out) {
+                                                  double2* __restrict__ out, uint64_t mchunk, uint64_t slice) {
     typedef double v4 __attribute__((ext_vector_type(4)));
     const uint32_t tr = blockIdx.y, tc = blockIdx.x;
     if (tc < tr) return;
+    const uint64_t ebeg = (uint64_t)blockIdx.z * mchunk, eend = ebeg + mchunk < m ? ebeg + mchunk : m;
+    out += (uint64_t)blockIdx.z * slice;
     const uint32_t lane = threadIdx.x & 63u, kk = lane >> 4, j = lane & 15u;
     const uint64_t r0 = (uint64_t)tr * 16 + j, c0 = (uint64_t)tc * 16 + j;
     v4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
 #pragma unroll 4
-    for (uint64_t e0 = 0; e0 < m; e0 += 4) {
+    for (uint64_t e0 = ebeg; e0 < eend; e0 += 4) {
         const double2 a = phi[(e0 + kk) * d + r0];
         const double2 b = phi[(e0 + kk) * d + c0];
         re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, re, 0, 0, 0);
@@ -713,8 +719,10 @@ __global__ void __launch_bounds__(64) k_herk_mfma(const double2* __restrict__ ph
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
         const uint64_t row = (uint64_t)tr * 16 + kk + 4 * jj, col = (uint64_t)tc * 16 + j;
-        out[row * d + col] = make_double2(re[jj], im[jj]);
-        if (tr != tc) out[col * d + row] = make_double2(re[jj], -im[jj]);
+        if (row > col) continue;  // (diagonal tile) written by the transposed entry
+        const double vi = row == col ? 0.0 : im[jj];
+        out[row * d + col] = make_double2(re[jj], vi);
+        if (row != col) out[col * d + row] = make_double2(re[jj], -vi);
     }
 }
 
@@ -729,7 +737,7 @@ __global__ void __launch_bounds__(64) k_herk_mfma(const double2* __restrict__ ph
 // are kept, each written with its conjugate transpose, and the diagonal's
 // imaginary parts (zero in exact arithmetic) as zero, so the matrix is
 // exactly Hermitian.
-// Block order (XT, from 8 x 8 blocks): workgroup b runs on XCD b % 8, and
+// Block order (XT, from 64 blocks a side, d >= 4096): workgroup b runs on XCD b % 8, and
 // XCD x takes the super-tiles x, x + 8, ... of 8 x 8 blocks of the upper
 // triangle, one at a time, so the blocks that run together on one XCD share
 // 16 strips through its L2 (26 qubits, 13 kept: 41 against 48 ms).  The
@@ -740,7 +748,8 @@ constexpr uint32_t kHerkBlock = 64, kHerkRows = 16;
 
 template <bool XT>
 __global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict__ phi, uint64_t m, uint32_t d,
-                                                      double2* __restrict__ out, uint32_t ns) {
+                                                      double2* __restrict__ out, uint32_t ns, uint64_t mchunk,
+                                                      uint64_t slice) {
     typedef double v4 __attribute__((ext_vector_type(4)));
     uint32_t bR, bC;
     if (XT) {
@@ -758,6 +767,10 @@ __global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict_
         bC = blockIdx.x;
     }
     if (bC < bR) return;
+    // environment chunk blockIdx.z: rows [z * mchunk, (z + 1) * mchunk), its
+    // partial matrix at out + z * slice
+    const uint64_t ebeg = (uint64_t)blockIdx.z * mchunk, eend = ebeg + mchunk < m ? ebeg + mchunk : m;
+    out += (uint64_t)blockIdx.z * slice;
     __shared__ double2 sa[kHerkRows][kHerkBlock], sb[kHerkRows][kHerkBlock];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t wr = wave >> 1, wc = wave & 1u, kk = lane >> 4, j = lane & 15u;
@@ -771,7 +784,7 @@ __global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict_
             re[a][b] = v4{0, 0, 0, 0};
             im[a][b] = v4{0, 0, 0, 0};
         }
-    for (uint64_t e0 = 0; e0 < m; e0 += kHerkRows) {
+    for (uint64_t e0 = ebeg; e0 < eend; e0 += kHerkRows) {
         // 16 rows x 64 of each strip: four elements per thread per strip
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -820,6 +833,118 @@ __global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict_
 __global__ void __launch_bounds__(256) k_diag_real(const double2* __restrict__ A, uint32_t d, double* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < d) out[i] = A[(size_t)i * d + i].x;
+}
+
+// Reduced matrices of 1-3 kept qubits: the environment sum split over the
+// grid, chunk blockIdx.x taking environment indices [x * chunk, (x + 1) *
+// chunk).  A lane takes one environment index at a time with its 2^K kept
+// amplitudes in registers and adds the FP64 products of the upper triangle
+// (entry u: row r, column c >= r, row by row); the lanes' sums are reduced
+// in a fixed order (waves, then the workgroup) into part[x * U + u].
+template <typename R, int K>
+__global__ void __launch_bounds__(256) k_rho_small(const cv<R>* __restrict__ psi, KeepList keep, uint64_t envmask,
+                                                   uint64_t m, uint64_t chunk, double2* __restrict__ part) {
+    constexpr int D = 1 << K, U = D * (D + 1) / 2;
+    __shared__ double s_acc[4][2 * U];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint64_t off[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        off[r] = 0;
+#pragma unroll
+        for (int b = 0; b < K; ++b)
+            if ((r >> b) & 1) off[r] |= 1ULL << keep.q[b];
+    }
+    double acc[2 * U];
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) acc[u] = 0.0;
+    const uint64_t e0 = (uint64_t)blockIdx.x * chunk, e1 = e0 + chunk < m ? e0 + chunk : m;
+    for (uint64_t e = e0 + tid; e < e1; e += 256) {
+        const uint64_t base = deposit_bits(e, envmask);
+        double ar[D], ai[D];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            const cv<R> a = psi[base | off[r]];
+            ar[r] = (double)a.x;
+            ai[r] = (double)a.y;
+        }
+        int u = 0;
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int c = r; c < D; ++c, ++u) {
+                acc[2 * u] += ar[r] * ar[c] + ai[r] * ai[c];  // a_r conj(a_c)
+                acc[2 * u + 1] += ai[r] * ar[c] - ar[r] * ai[c];
+            }
+    }
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) {
+        double v = acc[u];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) s_acc[wave][u] = v;
+    }
+    __syncthreads();
+    if (tid < (uint32_t)(2 * U)) {
+        const double v = ((s_acc[0][tid] + s_acc[1][tid]) + s_acc[2][tid]) + s_acc[3][tid];
+        double* pd = reinterpret_cast<double*>(part + (uint64_t)blockIdx.x * U);
+        pd[tid] = v;
+    }
+}
+
+// The d x d matrix from `nparts` partial upper triangles (U = d (d + 1) / 2
+// entries each, part[x * U + u]): one workgroup per entry, the partials summed
+// in a fixed order; the entry and its conjugate transpose written, with a zero
+// imaginary part on the diagonal.
+__global__ void __launch_bounds__(256) k_rho_reduce(const double2* __restrict__ part, uint32_t nparts, uint32_t d,
+                                                    double2* __restrict__ out) {
+    __shared__ double s_re[4], s_im[4];
+    const uint32_t u = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t U = d * (d + 1) / 2;
+    double sr = 0.0, si = 0.0;
+    for (uint32_t x = tid; x < nparts; x += 256) {
+        const double2 v = part[(uint64_t)x * U + u];
+        sr += v.x;
+        si += v.y;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_down(sr, o, 64);
+        si += __shfl_down(si, o, 64);
+    }
+    if (lane == 0) {
+        s_re[wave] = sr;
+        s_im[wave] = si;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        sr = ((s_re[0] + s_re[1]) + s_re[2]) + s_re[3];
+        si = ((s_im[0] + s_im[1]) + s_im[2]) + s_im[3];
+        uint32_t r = 0, rest = u;
+        while (rest >= d - r) {
+            rest -= d - r;
+            ++r;
+        }
+        const uint32_t c = r + rest;
+        out[(uint64_t)r * d + c] = make_double2(sr, r == c ? 0.0 : si);
+        if (r != c) out[(uint64_t)c * d + r] = make_double2(sr, -si);
+    }
+}
+
+// out[i] = sum over s of part[s * count + i], s in order (the environment
+// chunks of a split matrix-core product).
+__global__ void __launch_bounds__(256) k_sum_slices(const double2* __restrict__ part, uint32_t nslices, uint64_t count,
+                                                    double2* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    double sr = 0.0, si = 0.0;
+#pragma unroll 8
+    for (uint32_t x = 0; x < nslices; ++x) {
+        const double2 v = part[(uint64_t)x * count + i];
+        sr += v.x;
+        si += v.y;
+    }
+    out[i] = make_double2(sr, si);
 }
 
 // The capped Jacobi's rotation (jacobi_rows.hpp's backend, the loop of
@@ -1189,8 +1314,25 @@ hipError_t launch_sample_chunks(const void* psi, uint64_t count, int prec, const
     return hipGetLastError();
 }
 
+// Environment chunks for a product that would launch `blocks` workgroups over
+// m environment rows: enough chunks for about `target` workgroups, at least
+// 64 rows each (a multiple of 16), and no more partial slices of `slice`
+// elements than `cap` holds.  Returns the chunk count (1: no split).
+static uint32_t env_chunks(uint64_t blocks, uint64_t target, uint64_t m, uint64_t slice, uint64_t cap,
+                           uint64_t& mchunk) {
+    uint64_t S = blocks >= target ? 1 : (target + blocks - 1) / blocks;
+    S = std::min<uint64_t>(S, std::max<uint64_t>(1, m / 64));
+    S = std::min<uint64_t>(S, slice ? cap / slice : S);
+    if (S <= 1) {
+        mchunk = m;
+        return 1;
+    }
+    mchunk = ((m + S - 1) / S + 15) / 16 * 16;
+    return (uint32_t)((m + mchunk - 1) / mchunk);
+}
+
 hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t* keep, int k, double2* out,
-                                double2* scratch, hipStream_t st) {
+                                double2* scratch, uint64_t scratch_elems, hipStream_t st) {
     if (k < 1 || k > 16 || k > n) return hipErrorInvalidValue;
     KeepList kl{};
     uint64_t kmask = 0;
@@ -1203,10 +1345,36 @@ hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t
     const uint64_t envmask = ((1ULL << n) - 1) & ~kmask;
     const uint32_t d = 1u << k;
     const uint64_t m = 1ULL << (n - k);
-    if (k >= 4 && n - k >= 2 && (scratch || (in_order && prec == kC128))) {
+    if (k <= 3 && m >= 2 && scratch) {
+        // the environment sum over many workgroups, partial triangles in scratch
+        const uint32_t U = d * (d + 1) / 2;
+        uint64_t S = std::min<uint64_t>(2048, std::max<uint64_t>(1, m / 256));
+        S = std::min<uint64_t>(S, std::max<uint64_t>(1, scratch_elems / U));
+        const uint64_t chunk = (m + S - 1) / S;
+        S = (m + chunk - 1) / chunk;
+#define IQ_RHO_SMALL(RT, KK)                                                                                     \
+    hipLaunchKernelGGL((k_rho_small<RT, KK>), dim3((unsigned)S), dim3(256), 0, st, (const cv<RT>*)psi, kl, envmask, m, \
+                       chunk, scratch)
+        if (prec == kC128) {
+            if (k == 1) IQ_RHO_SMALL(double, 1);
+            else if (k == 2) IQ_RHO_SMALL(double, 2);
+            else IQ_RHO_SMALL(double, 3);
+        } else {
+            if (k == 1) IQ_RHO_SMALL(float, 1);
+            else if (k == 2) IQ_RHO_SMALL(float, 2);
+            else IQ_RHO_SMALL(float, 3);
+        }
+#undef IQ_RHO_SMALL
+        hipLaunchKernelGGL(k_rho_reduce, dim3(U), dim3(256), 0, st, (const double2*)scratch, (uint32_t)S, d, out);
+        return hipGetLastError();
+    }
+    const bool direct = in_order && prec == kC128;
+    if (k >= 4 && n - k >= 2 && (direct || (scratch && scratch_elems >= (1ULL << n)))) {
         // matrix-core path: reorder to (environment, kept) unless already so
         const double2* phi = (const double2*)psi;
-        if (!(in_order && prec == kC128)) {
+        double2* parts = scratch;  // environment-chunk partials after the reordered state
+        uint64_t parts_cap = scratch ? scratch_elems : 0;
+        if (!direct) {
             const uint64_t total = 1ULL << n;
             const unsigned b = blocks_for(total, 256, 65536);
             if (prec == kC128)
@@ -1216,20 +1384,38 @@ hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t
                 hipLaunchKernelGGL(k_gather_keep<float>, dim3(b), dim3(256), 0, st, (const float2*)psi, kl, k, envmask,
                                    total, scratch);
             phi = scratch;
+            parts = scratch + total;
+            parts_cap = scratch_elems - total;
         }
+        const uint64_t slice = (uint64_t)d * d;
+        uint64_t mchunk = m;
+        uint32_t S = 1;
         if (k >= 6 && n - k >= 4) {  // d a multiple of 64, m of 16
             const uint32_t nb = d / kHerkBlock;
-            if (nb >= 8) {
+            S = env_chunks((uint64_t)nb * (nb + 1) / 2, 512, m, slice, parts_cap, mchunk);
+            double2* dst = S > 1 ? parts : out;
+            const uint64_t sl = S > 1 ? slice : 0;
+            // the XCD order from 8 super-tiles a side (every XCD gets several;
+            // with fewer the work would sit on a few XCDs)
+            if (nb >= 64) {
                 const uint32_t ns = nb / 8, tiles = ns * (ns + 1) / 2;
                 const uint32_t blocks = ((tiles + 7) / 8) * 8 * 64;
-                hipLaunchKernelGGL(k_herk_blocked<true>, dim3(blocks), dim3(256), 0, st, phi, m, d, out, ns);
+                hipLaunchKernelGGL(k_herk_blocked<true>, dim3(blocks, 1, S), dim3(256), 0, st, phi, m, d, dst, ns, mchunk,
+                                   sl);
             } else {
-                hipLaunchKernelGGL(k_herk_blocked<false>, dim3(nb, nb), dim3(256), 0, st, phi, m, d, out, 0u);
+                hipLaunchKernelGGL(k_herk_blocked<false>, dim3(nb, nb, S), dim3(256), 0, st, phi, m, d, dst, 0u, mchunk,
+                                   sl);
             }
         } else {
             const uint32_t nt = d / 16;
-            hipLaunchKernelGGL(k_herk_mfma, dim3(nt, nt), dim3(64), 0, st, phi, m, d, out);
+            // one wave per tile: chunks for about 2048 waves
+            S = env_chunks((uint64_t)nt * (nt + 1) / 2, 2048, m, slice, parts_cap, mchunk);
+            double2* dst = S > 1 ? parts : out;
+            hipLaunchKernelGGL(k_herk_mfma, dim3(nt, nt, S), dim3(64), 0, st, phi, m, d, dst, mchunk, S > 1 ? slice : 0);
         }
+        if (S > 1)
+            hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((slice + 255) / 256)), dim3(256), 0, st, (const double2*)parts,
+                               S, slice, out);
         return hipGetLastError();
     }
     const uint64_t total = (uint64_t)d * d;

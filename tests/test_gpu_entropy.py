@@ -190,3 +190,24 @@ def test_device_jacobi_matches_host_bit_for_bit(oracle_lib, case):
         dev, host = _entropy_both_ways(sv, keep)
         assert np.isfinite(dev)
         assert dev == host, (dev, host)
+
+
+@pytest.mark.parametrize("n,keep,precision", [
+    (20, [7], "complex128"), (20, [19, 0], "complex128"), (20, [3, 17, 9], "complex128"),
+    (19, [2, 11, 5], "complex64"), (20, [0, 1, 2, 3], "complex128"), (20, [18, 2, 9, 0, 13], "complex128"),
+    (18, [4, 0, 12, 7, 1, 15, 9], "complex128"), (20, list(range(9)), "complex128"),
+    (20, [8, 1, 19, 3, 11, 0, 6, 15, 4], "complex64")])
+def test_reduced_matrix_environment_chunks(oracle_lib, n, keep, precision):
+    """Few kept qubits of a larger state, where the environment sum is split
+    over many workgroups: 1-3 kept (lane per environment index), 4-5 kept
+    (one wave per 16 x 16 tile, chunked), 7 and 9 kept (64 x 64 blocks in
+    both block orders, chunked), against the oracle element by element;
+    the matrix is exactly Hermitian and the entropy matches."""
+    sv, ref = states(oracle_lib, n, precision, seed=n + len(keep))
+    print(f"reduced matrix n={n} keep={keep} {precision}", flush=True)
+    got, want = sv.reduced_density_matrix(keep), ref.partial_trace(keep)
+    tol = 1e-13 if precision == "complex128" else 2e-6
+    assert np.max(np.abs(got - want)) < tol
+    assert np.array_equal(got, got.conj().T)
+    assert abs(np.trace(got).real - 1.0) < (1e-12 if precision == "complex128" else 1e-5)
+    assert abs(sv.von_neumann_entropy(keep) - ref.von_neumann_entropy(keep)) < (1e-9 if precision == "complex128" else 1e-4)

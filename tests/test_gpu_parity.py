@@ -61,6 +61,13 @@ def test_golden_u3cx_n8(B, golden):
             assert abs(sv.measure_prob(q, o) - g["measure_prob"][q][o]) < TOL128
     for sub, e in zip(g["entropy_subsets"], g["entropy"]):
         assert abs(sv.von_neumann_entropy([int(v) for v in sub if v >= 0]) - e) < 1e-10
+    # the same subsets with the Jacobi on the device (rho_A never leaves it)
+    try:
+        B.set_entropy_options(1)
+        for sub, e in zip(g["entropy_subsets"], g["entropy"]):
+            assert abs(sv.von_neumann_entropy([int(v) for v in sub if v >= 0]) - e) < 1e-10
+    finally:
+        B.set_entropy_options(0)
 
 
 def test_golden_sampling_n10(B, golden):
