@@ -129,6 +129,11 @@ void        iq_set_mfma_options(int batches_per_iteration, int matrix_in_lds, in
 void*  iq_statevector_create_precision(size_t num_qubits, int precision);
 int    iq_statevector_get_precision(void* sv);
 void   iq_statevector_set_amplitudes(void* sv, const double* real_in, const double* imag_in);
+/* The amplitudes as interleaved (re, im) FP64 pairs (2 * 2^n doubles, e.g. a NumPy complex128
+ * array), for callers that hold complex arrays.  No reference counterpart: get_amplitudes
+ * (c_api.h:20) exports split arrays; these copy straight into or out of the caller's array. */
+void   iq_statevector_get_amplitudes_interleaved(void* sv, double* out);
+void   iq_statevector_set_amplitudes_interleaved(void* sv, const double* in);
 double iq_statevector_norm_squared(void* sv);     /* StateVector::norm_squared, statevector.cpp:206-213 */
 void   iq_statevector_normalize(void* sv);        /* StateVector::normalize, statevector.cpp:215-224 */
 void   iq_statevector_apply_diagonal(void* sv, const size_t* qubits, size_t k, const double* d_real,
@@ -182,6 +187,10 @@ void*  iq_statevector_device_ptr(void* sv);  /* device address of the amplitudes
 void*  iq_statevector_stream(void* sv);      /* hipStream_t the handle's work is ordered on */
 
 void   iq_density_matrix_set_elements(void* dm, const double* real_in, const double* imag_in);
+/* rho as interleaved (re, im) FP64 pairs, row-major (2 * 4^n doubles); no reference counterpart
+ * (get_elements, c_api.h:37, exports split arrays). */
+void   iq_density_matrix_get_elements_interleaved(void* dm, double* out);
+void   iq_density_matrix_set_elements_interleaved(void* dm, const double* in);
 /* 1- or 2-qubit Kraus channel (class-only in the reference, density_matrix.cpp:165-191). */
 void   iq_density_matrix_apply_kraus(void* dm, const size_t* qubits, size_t k, size_t num_kraus,
                                      const double* k_real, const double* k_imag);

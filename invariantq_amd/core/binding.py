@@ -134,6 +134,8 @@ SIGNATURES = {
     "iq_statevector_create_precision": ([_sz, ctypes.c_int], _vp),
     "iq_statevector_get_precision": ([_vp], ctypes.c_int),
     "iq_statevector_set_amplitudes": ([_vp, _d, _d], None),
+    "iq_statevector_get_amplitudes_interleaved": ([_vp, _d], None),
+    "iq_statevector_set_amplitudes_interleaved": ([_vp, _d], None),
     "iq_statevector_norm_squared": ([_vp], ctypes.c_double),
     "iq_statevector_normalize": ([_vp], None),
     "iq_statevector_apply_diagonal": ([_vp, _s, _sz, _d, _d], None),
@@ -148,6 +150,8 @@ SIGNATURES = {
     "iq_statevector_device_ptr": ([_vp], _vp),
     "iq_statevector_stream": ([_vp], _vp),
     "iq_density_matrix_set_elements": ([_vp, _d, _d], None),
+    "iq_density_matrix_get_elements_interleaved": ([_vp, _d], None),
+    "iq_density_matrix_set_elements_interleaved": ([_vp, _d], None),
     "iq_density_matrix_apply_kraus": ([_vp, _s, _sz, _sz, _d, _d], None),
     "iq_density_matrix_apply_superop": ([_vp, _s, _sz, _d, _d], None),
     "iq_density_matrix_expval_z": ([_vp, _d], None),
@@ -332,18 +336,18 @@ class NativeStateVector:
         return out
 
     def get_amplitudes(self) -> np.ndarray:
-        re = np.empty(self.dim, dtype=np.float64)
-        im = np.empty(self.dim, dtype=np.float64)
-        _NATIVE_LIB.iq_statevector_get_amplitudes(self._ptr, _dp(re), _dp(im))
+        # straight into a complex128 array (the split export, c_api.h:20, then
+        # re + 1j * im costs two more passes over fresh host memory)
+        out = np.empty(self.dim, dtype=np.complex128)
+        _NATIVE_LIB.iq_statevector_get_amplitudes_interleaved(self._ptr, _dp(out))
         check_error()
-        return re + 1j * im
+        return out
 
     def set_amplitudes(self, amplitudes):
-        a = np.asarray(amplitudes, dtype=np.complex128).reshape(-1)
+        a = np.ascontiguousarray(np.asarray(amplitudes, dtype=np.complex128).reshape(-1))
         if a.size != self.dim:
             raise ValueError("amplitude count does not match the statevector dimension")
-        re, im = _split(a)
-        _NATIVE_LIB.iq_statevector_set_amplitudes(self._ptr, _dp(re), _dp(im))
+        _NATIVE_LIB.iq_statevector_set_amplitudes_interleaved(self._ptr, _dp(a))
         check_error()
 
     def get_probabilities(self) -> np.ndarray:
@@ -515,19 +519,16 @@ class NativeDensityMatrix:
         check_error()
 
     def get_matrix(self) -> np.ndarray:
-        total = self.dim * self.dim
-        re = np.empty(total, dtype=np.float64)
-        im = np.empty(total, dtype=np.float64)
-        _NATIVE_LIB.iq_density_matrix_get_elements(self._ptr, _dp(re), _dp(im))
+        out = np.empty((self.dim, self.dim), dtype=np.complex128)
+        _NATIVE_LIB.iq_density_matrix_get_elements_interleaved(self._ptr, _dp(out))
         check_error()
-        return (re + 1j * im).reshape((self.dim, self.dim))
+        return out
 
     def set_matrix(self, rho):
-        a = np.asarray(rho, dtype=np.complex128).reshape(-1)
+        a = np.ascontiguousarray(np.asarray(rho, dtype=np.complex128).reshape(-1))
         if a.size != self.dim * self.dim:
             raise ValueError("matrix size does not match the density-matrix dimension")
-        re, im = _split(a)
-        _NATIVE_LIB.iq_density_matrix_set_elements(self._ptr, _dp(re), _dp(im))
+        _NATIVE_LIB.iq_density_matrix_set_elements_interleaved(self._ptr, _dp(a))
         check_error()
 
     def trace(self) -> float:

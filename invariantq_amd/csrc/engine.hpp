@@ -119,6 +119,9 @@ struct Engine {
     bool download(const double* dsrc, double* hdst, uint64_t count);
     bool export_split(double* re, double* im);
     bool import_split(const double* re, const double* im);
+    // interleaved FP64 (re, im) pairs, 2 * dim doubles
+    bool export_interleaved(double* out);
+    bool import_interleaved(const double* in);
     bool export_probs(double* out);
     bool fetch(int m, double* out);
     bool norm2(double* out);

@@ -55,6 +55,9 @@ hipError_t launch_collapse(void* psi, int n, int prec, int target, int outcome, 
 hipError_t launch_scale(void* psi, uint64_t count, int prec, double factor, hipStream_t st);
 hipError_t launch_split(const void* psi, uint64_t first, uint64_t count, int prec, double* re, double* im, hipStream_t st);
 hipError_t launch_join(void* psi, uint64_t first, uint64_t count, int prec, const double* re, const double* im, hipStream_t st);
+// Amplitudes [first, first + count) as interleaved FP64 (re, im) pairs, and back.
+hipError_t launch_interleave(const void* psi, uint64_t first, uint64_t count, int prec, double* out, hipStream_t st);
+hipError_t launch_deinterleave(void* psi, uint64_t first, uint64_t count, int prec, const double* in, hipStream_t st);
 hipError_t launch_probs(const void* psi, uint64_t count, int prec, double* out, hipStream_t st);
 
 // Deterministic FP64 reductions.  Each writes `m` values into `out` (device),

@@ -112,25 +112,27 @@ __global__ void __launch_bounds__(256) k_scale(cv<R>* __restrict__ psi, uint64_t
     });
 }
 
+// FP64 copies of amplitudes: re[i * es], im[i * es] (es = 1: split arrays;
+// es = 2 with im = re + 1: interleaved complex)
 template <typename R>
-__global__ void __launch_bounds__(256) k_split(const cv<R>* __restrict__ psi, uint64_t count, double* __restrict__ re,
-                                               double* __restrict__ im) {
+__global__ void __launch_bounds__(256) k_split(const cv<R>* __restrict__ psi, uint64_t count, double* re, double* im,
+                                               uint32_t es) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         const cv<R> v = psi[i];
-        re[i] = (double)v.x;
-        im[i] = (double)v.y;
+        re[i * es] = (double)v.x;
+        im[i * es] = (double)v.y;
     }
 }
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_join(cv<R>* __restrict__ psi, uint64_t count, const double* __restrict__ re,
-                                              const double* __restrict__ im) {
+__global__ void __launch_bounds__(256) k_join(cv<R>* __restrict__ psi, uint64_t count, const double* re,
+                                              const double* im, uint32_t es) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         cv<R> v;
-        v.x = (R)re[i];
-        v.y = (R)im[i];
+        v.x = (R)re[i * es];
+        v.y = (R)im[i * es];
         psi[i] = v;
     }
 }
@@ -1140,20 +1142,39 @@ hipError_t launch_scale(void* psi, uint64_t count, int prec, double factor, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_split(const void* psi, uint64_t first, uint64_t count, int prec, double* re, double* im, hipStream_t st) {
+static hipError_t split_strided(const void* psi, uint64_t first, uint64_t count, int prec, double* re, double* im,
+                                uint32_t es, hipStream_t st) {
     const unsigned b = blocks_for(count, 256, 65536);
     if (prec == kC128)
-        hipLaunchKernelGGL(k_split<double>, dim3(b), dim3(256), 0, st, (const double2*)psi + first, count, re, im);
-    else hipLaunchKernelGGL(k_split<float>, dim3(b), dim3(256), 0, st, (const float2*)psi + first, count, re, im);
+        hipLaunchKernelGGL(k_split<double>, dim3(b), dim3(256), 0, st, (const double2*)psi + first, count, re, im, es);
+    else hipLaunchKernelGGL(k_split<float>, dim3(b), dim3(256), 0, st, (const float2*)psi + first, count, re, im, es);
     return hipGetLastError();
+}
+
+static hipError_t join_strided(void* psi, uint64_t first, uint64_t count, int prec, const double* re, const double* im,
+                               uint32_t es, hipStream_t st) {
+    const unsigned b = blocks_for(count, 256, 65536);
+    if (prec == kC128)
+        hipLaunchKernelGGL(k_join<double>, dim3(b), dim3(256), 0, st, (double2*)psi + first, count, re, im, es);
+    else hipLaunchKernelGGL(k_join<float>, dim3(b), dim3(256), 0, st, (float2*)psi + first, count, re, im, es);
+    return hipGetLastError();
+}
+
+hipError_t launch_split(const void* psi, uint64_t first, uint64_t count, int prec, double* re, double* im, hipStream_t st) {
+    return split_strided(psi, first, count, prec, re, im, 1, st);
 }
 
 hipError_t launch_join(void* psi, uint64_t first, uint64_t count, int prec, const double* re, const double* im,
                        hipStream_t st) {
-    const unsigned b = blocks_for(count, 256, 65536);
-    if (prec == kC128) hipLaunchKernelGGL(k_join<double>, dim3(b), dim3(256), 0, st, (double2*)psi + first, count, re, im);
-    else hipLaunchKernelGGL(k_join<float>, dim3(b), dim3(256), 0, st, (float2*)psi + first, count, re, im);
-    return hipGetLastError();
+    return join_strided(psi, first, count, prec, re, im, 1, st);
+}
+
+hipError_t launch_interleave(const void* psi, uint64_t first, uint64_t count, int prec, double* out, hipStream_t st) {
+    return split_strided(psi, first, count, prec, out, out + 1, 2, st);
+}
+
+hipError_t launch_deinterleave(void* psi, uint64_t first, uint64_t count, int prec, const double* in, hipStream_t st) {
+    return join_strided(psi, first, count, prec, in, in + 1, 2, st);
 }
 
 hipError_t launch_probs(const void* psi, uint64_t count, int prec, double* out, hipStream_t st) {

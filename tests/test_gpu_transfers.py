@@ -34,3 +34,41 @@ def test_round_trip_and_probabilities(n, prec):
     lo, hi = stored[:half], stored[half:]
     want = np.concatenate([(lo + hi) / np.sqrt(2), (lo - hi) / np.sqrt(2)])
     assert np.max(np.abs(got2 - want)) < (1e-6 if prec == "complex64" else 1e-14)
+
+
+@pytest.mark.parametrize("n,prec", [(9, "complex128"), (23, "complex128"), (23, "complex64")])
+def test_split_and_interleaved_transfers_agree(n, prec):
+    """The reference's split-array entry points (c_api.h:19-20, what the
+    reference binding calls) against the interleaved ones the package binding
+    uses: the same values bit for bit both ways, statevector and density
+    matrix."""
+    print(f"split vs interleaved n={n} {prec}", flush=True)
+    lib = B.lib()
+    rng = np.random.default_rng(n + 7)
+    a = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    sv = B.NativeStateVector(n, prec)
+    re, im = np.ascontiguousarray(a.real), np.ascontiguousarray(a.imag)
+    lib.iq_statevector_set_amplitudes(sv._ptr, B._dp(re), B._dp(im))
+    B.check_error()
+    r2, i2 = np.empty(1 << n), np.empty(1 << n)
+    lib.iq_statevector_get_amplitudes(sv._ptr, B._dp(r2), B._dp(i2))
+    B.check_error()
+    got = sv.get_amplitudes()  # interleaved
+    assert np.array_equal(got.real, r2) and np.array_equal(got.imag, i2)
+    sv2 = B.NativeStateVector(n, prec)
+    sv2.set_amplitudes(a)  # interleaved
+    lib.iq_statevector_get_amplitudes(sv2._ptr, B._dp(r2), B._dp(i2))
+    B.check_error()
+    assert np.array_equal(got.real, r2) and np.array_equal(got.imag, i2)
+    if n <= 12:
+        nq = n // 2
+        rho = rng.standard_normal((1 << nq, 1 << nq)) + 1j * rng.standard_normal((1 << nq, 1 << nq))
+        dm = B.NativeDensityMatrix(nq)
+        rr, ri = np.ascontiguousarray(rho.real.reshape(-1)), np.ascontiguousarray(rho.imag.reshape(-1))
+        lib.iq_density_matrix_set_elements(dm._ptr, B._dp(rr), B._dp(ri))
+        B.check_error()
+        assert np.array_equal(dm.get_matrix(), rho)
+        dm.set_matrix(rho.T.copy())
+        lib.iq_density_matrix_get_elements(dm._ptr, B._dp(rr), B._dp(ri))
+        B.check_error()
+        assert np.array_equal(rr.reshape(rho.shape) + 1j * ri.reshape(rho.shape), rho.T)
