@@ -92,15 +92,34 @@ def _normalise(op):
 
 
 def _counts(samples, n):
-    counts: Dict[str, int] = {}
-    for s in samples:
-        key = f"{int(s):0{n}b}"
-        counts[key] = counts.get(key, 0) + 1
-    return counts
+    """Bitstring counts keyed in order of first appearance, as the per-shot
+    loop of invariantq/simulator/statevector.py:104-108 builds them (the
+    distinct outcomes and their counts from np.unique, then one key each)."""
+    s = np.asarray(samples, dtype=np.uint64).reshape(-1)
+    if s.size == 0:
+        return {}
+    uniq, first, cnt = np.unique(s, return_index=True, return_counts=True)
+    order = np.argsort(first, kind="stable")
+    return dict(zip(_bitstrings(uniq[order], n), cnt[order].tolist()))
+
+
+def _bitstrings(idx, n):
+    """f"{i:0{n}b}" of every index (n <= 64), the digits made in one array:
+    the big-endian bytes of each index unpacked to bits, the last n kept."""
+    if idx.size == 0:
+        return []
+    b = np.unpackbits(np.asarray(idx, dtype=">u8").view(np.uint8).reshape(-1, 8), axis=1)[:, 64 - n:]
+    raw = np.ascontiguousarray(b + np.uint8(48)).tobytes().decode("ascii")
+    return [raw[i:i + n] for i in range(0, len(raw), n)]
 
 
 def _probabilities(probs, n):
-    return {f"{i:0{n}b}": float(p) for i, p in enumerate(probs) if p > 1e-7}
+    """Probabilities above 1e-7 keyed by bitstring, in index order
+    (statevector.py:110-113); the threshold is applied on the whole array
+    first, so only the kept entries are formatted."""
+    p = np.asarray(probs, dtype=np.float64).reshape(-1)
+    idx = np.flatnonzero(p > 1e-7)
+    return dict(zip(_bitstrings(idx, n), p[idx].tolist()))
 
 
 class StateVectorSimulator:

@@ -28,3 +28,25 @@ def test_memory_limits():
     assert checked_power_of_two(3, 16, copies=2) == 256
     with pytest.raises(MemoryError):
         checked_power_of_two(62, 16)
+
+
+def test_counts_and_probabilities_match_the_reference_loops():
+    """The vectorised result formatting against the per-shot and per-entry
+    loops of statevector.py:104-113: same keys, values and key order."""
+    import numpy as np
+    from invariantq_amd.simulator import _counts, _probabilities
+    rng = np.random.default_rng(5)
+    for n, shots in ((1, 7), (5, 1000), (12, 3000)):
+        samples = rng.integers(0, 1 << n, size=shots, dtype=np.uint64)
+        want = {}
+        for s in samples:
+            key = f"{int(s):0{n}b}"
+            want[key] = want.get(key, 0) + 1
+        got = _counts(samples, n)
+        assert got == want and list(got) == list(want)
+        probs = rng.random(1 << n) * 3e-7
+        probs[::7] = 1e-7  # the threshold is strict
+        wantp = {f"{i:0{n}b}": float(p) for i, p in enumerate(probs) if p > 1e-7}
+        gotp = _probabilities(probs, n)
+        assert gotp == wantp and list(gotp) == list(wantp)
+    assert _counts(np.array([], dtype=np.uint64), 3) == {}
