@@ -10,6 +10,7 @@
 #include "reduce.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace iqg {
 
@@ -831,6 +832,114 @@ __global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict_
             }
 }
 
+// The blocked product with its strips moved by the direct global-to-LDS
+// loads of gfx950 (global_load_lds_dwordx4: each lane's 16 bytes land at
+// the wave's LDS base + 16 * lane, no registers in between), double
+// buffered: step s computes from one buffer while step s + 1's rows arrive
+// in the other, one barrier per step.  The two buffers are distinct arrays
+// used in a fixed alternation (and a diagonal block loads its strip into
+// both), so the compiler can see that a step's LDS reads do not touch the
+// loads in flight and issues no memory wait before them.  RW rows per step;
+// the XCD block order, operand layout and results of k_herk_blocked, the
+// same sums in the same order.  64 KiB of LDS per block at RW = 16.
+template <int RW>
+__global__ void __launch_bounds__(256) k_herk_dma(const double2* __restrict__ phi, uint64_t m, uint32_t d,
+                                                  double2* __restrict__ out, uint32_t ns, uint64_t mchunk,
+                                                  uint64_t slice) {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    const uint32_t b = blockIdx.x, x = b & 7u, sq = b >> 3, w = sq & 63u;
+    uint32_t g = (sq >> 6) * 8u + x, sr = 0;
+    while (sr < ns && g >= ns - sr) {
+        g -= ns - sr;
+        ++sr;
+    }
+    if (sr >= ns) return;
+    const uint32_t bR = sr * 8u + (w >> 3), bC = (sr + g) * 8u + (w & 7u);
+    if (bC < bR) return;
+    const uint64_t ebeg = (uint64_t)blockIdx.z * mchunk, eend = ebeg + mchunk < m ? ebeg + mchunk : m;
+    out += (uint64_t)blockIdx.z * slice;
+    // two buffers as distinct arrays, used in a fixed alternation, so that the
+    // reads of one are not ordered behind the loads into the other
+    __shared__ double2 sa0[RW][kHerkBlock], sb0[RW][kHerkBlock], sa1[RW][kHerkBlock], sb1[RW][kHerkBlock];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t wr = wave >> 1, wc = wave & 1u, kk = lane >> 4, j = lane & 15u;
+    const uint64_t r0 = (uint64_t)bR * kHerkBlock, c0 = (uint64_t)bC * kHerkBlock;
+    const bool diag = bR == bC;
+    // rows of a step: wave w moves rows w, w + 4, ... of both strips, one
+    // 1 KiB row (64 lanes x 16 B) per instruction
+    auto fetch = [&](uint64_t e0, double2 (*SA)[kHerkBlock], double2 (*SB)[kHerkBlock]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int er = (int)wave; er < RW; er += 4) {
+            const double2* row = phi + (e0 + (uint64_t)er) * d;
+            __builtin_amdgcn_global_load_lds((const void*)(row + r0 + lane),
+                                             (__attribute__((address_space(3))) void*)&SA[er][0], 16, 0, 0);
+            // (a diagonal block loads its strip twice: then every read has
+            // one known buffer)
+            __builtin_amdgcn_global_load_lds((const void*)(row + c0 + lane),
+                                             (__attribute__((address_space(3))) void*)&SB[er][0], 16, 0, 0);
+        }
+    };
+    v4 re[2][2], im[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            re[a][b] = v4{0, 0, 0, 0};
+            im[a][b] = v4{0, 0, 0, 0};
+        }
+    auto compute = [&](const double2 (*A)[kHerkBlock], const double2 (*B)[kHerkBlock]) __attribute__((always_inline)) {
+#pragma unroll
+        for (uint32_t ks = 0; ks < (uint32_t)RW; ks += 4) {
+            double2 a[2], b[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                a[t] = A[ks + kk][wr * 32 + t * 16 + j];
+                b[t] = B[ks + kk][wc * 32 + t * 16 + j];
+            }
+#pragma unroll
+            for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+                for (int tb = 0; tb < 2; ++tb) {
+                    re[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].x, b[tb].x, re[ta][tb], 0, 0, 0);
+                    re[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].y, b[tb].y, re[ta][tb], 0, 0, 0);
+                    im[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].y, b[tb].x, im[ta][tb], 0, 0, 0);
+                    im[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[ta].x, b[tb].y, im[ta][tb], 0, 0, 0);
+                }
+        }
+    };
+    fetch(ebeg, sa0, sb0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's rows are in LDS
+    __syncthreads();
+    // steps in pairs: buffer 0, then buffer 1 (the environment chunk is a
+    // multiple of 16 rows, so a pair's second step exists when RW = 8, and for
+    // RW = 16 it is checked)
+    for (uint64_t e0 = ebeg; e0 < eend; e0 += 2 * RW) {
+        const bool more1 = e0 + RW < eend, more2 = e0 + 2 * RW < eend;
+        if (more1) fetch(e0 + RW, sa1, sb1);
+        compute(sa0, sb0);
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        if (!more1) break;
+        if (more2) fetch(e0 + 2 * RW, sa0, sb0);
+        compute(sa1, sb1);
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const uint64_t row = r0 + wr * 32 + ta * 16 + kk + 4 * jj, col = c0 + wc * 32 + tb * 16 + j;
+                if (diag && row > col) continue;
+                const double vr = re[ta][tb][jj], vi = row == col ? 0.0 : im[ta][tb][jj];
+                out[row * d + col] = make_double2(vr, vi);
+                if (row != col) out[col * d + row] = make_double2(vr, -vi);
+            }
+}
+
 // Real parts of the diagonal of a d x d matrix (double2, row-major) into out.
 __global__ void __launch_bounds__(256) k_diag_real(const double2* __restrict__ A, uint32_t d, double* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1418,7 +1527,13 @@ hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t
             const uint64_t sl = S > 1 ? slice : 0;
             // the XCD order from 8 super-tiles a side (every XCD gets several;
             // with fewer the work would sit on a few XCDs)
-            if (nb >= 64) {
+            if (nb >= 256) {
+                // 14 or more kept: strips by direct global-to-LDS loads,
+                // double buffered (15 of 30 kept: 2.27 against 2.69 s)
+                const uint32_t ns = nb / 8, tiles = ns * (ns + 1) / 2;
+                const uint32_t blocks = ((tiles + 7) / 8) * 8 * 64;
+                hipLaunchKernelGGL((k_herk_dma<16>), dim3(blocks, 1, S), dim3(256), 0, st, phi, m, d, dst, ns, mchunk, sl);
+            } else if (nb >= 64) {
                 const uint32_t ns = nb / 8, tiles = ns * (ns + 1) / 2;
                 const uint32_t blocks = ((tiles + 7) / 8) * 8 * 64;
                 hipLaunchKernelGGL(k_herk_blocked<true>, dim3(blocks, 1, S), dim3(256), 0, st, phi, m, d, dst, ns, mchunk,
