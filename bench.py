@@ -782,9 +782,10 @@ def batch_workload(torch, count: int, cpu: bool):
     t_tune = time.perf_counter() - t0
     gw = tune["group_width"]
     # warm-up: the whole batch, so the timed launch is a steady repeat
-    runner.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"], group_width=gw)
+    runner.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"], group_width=gw, solo=False)
     t0 = time.perf_counter()
-    e, g, ms = runner.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"], group_width=gw)
+    e, g, ms = runner.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"], group_width=gw,
+                          solo=False)
     wall = time.perf_counter() - t0
     nparams = int(sum(2 * n * l for n, l, _ in specs))
     gate_pass_bytes = sum(3 * n * l * (2 + 4) * (1 << n) * 8 for n, l, _ in specs)
@@ -823,8 +824,8 @@ def batch_workload(torch, count: int, cpu: bool):
         cls = [s for s in specs if s[0] == n]
         ct = batch.autotune(runner, cls, cache=None, per_n=64)
         cfg = dict(tile_bits=ct["tile_bits"], block_threads=ct["block_threads"], group_width=ct["group_width"])
-        runner.run(cls, **cfg)
-        _, _, cms = runner.run(cls, **cfg)
+        runner.run(cls, solo=False, **cfg)
+        _, _, cms = runner.run(cls, solo=False, **cfg)
         b, N = ct["tile_bits"][n], 1 << n
         L = sum(s[1] for s in cls)
         if n <= b:
@@ -891,8 +892,8 @@ def batch_class_workload(n: int):
         raise SystemExit(f"the batch has no circuit of {n} qubits")
     runner = batch.BatchRunner("complex64")
     cfg = dict(tile_bits={n: runner.max_tile_bits}, block_threads=512, group_width=runner.max_group_width)
-    runner.run(specs, **cfg)
-    _, _, ms = runner.run(specs, **cfg)
+    runner.run(specs, solo=False, **cfg)
+    _, _, ms = runner.run(specs, solo=False, **cfg)
     L = sum(s[1] for s in specs)
     return {"workload": f"batch size class n={n} alone", "circuits": len(specs), "layers": L, "ms": round(ms, 3),
             "config": {"tile_bits": runner.max_tile_bits, "block_threads": 512, "group_width": runner.max_group_width}}

@@ -54,11 +54,76 @@ class BatchRunner:
             self._free(self._ptr)
             self._ptr = None
 
+    # One circuit per workgroup: a circuit runs on one compute unit, so a
+    # size class with few circuits leaves the GPU idle for its whole sweep
+    # (one 20-qubit, 6-layer circuit: 36 ms, where the single-circuit engine,
+    # every compute unit on one circuit, takes about 0.5 ms).  Per circuit on
+    # one CU (complex64, measured at n = 14, 17, 20): about 36 ms * 2^(n-20)
+    # * layers / 6; on the single-circuit engine about 0.28 ms + 3.5e-8 ms *
+    # 2^n * layers.
+    _CU_MS_20_6 = 36.0
+    _ENGINE_MS_FIXED, _ENGINE_MS_PER_AMP_LAYER = 0.28, 3.5e-8
+
+    def solo_classes(self, specs: Sequence[Tuple[int, int, np.ndarray]]) -> List[int]:
+        """Qubit counts whose circuits the single-circuit engine evaluates
+        in less than half the time their persistent-launch sweep would hold
+        a compute unit (a class with fewer circuits than fill the GPU)."""
+        by_n: Dict[int, List[int]] = {}
+        for n, layers, _ in specs:
+            by_n.setdefault(int(n), []).append(int(layers))
+        out = []
+        for n, ls in by_n.items():
+            if n < 16:
+                continue
+            one_cu = self._CU_MS_20_6 * 2.0 ** (n - 20) * max(ls) / 6.0
+            engine = sum(self._ENGINE_MS_FIXED + self._ENGINE_MS_PER_AMP_LAYER * 2.0 ** n * l for l in ls)
+            if engine < 0.5 * one_cu:
+                out.append(n)
+        return sorted(out)
+
+    def _solo_circuit(self, n: int, layers: int, hx: float):
+        from .adjoint import GpuCircuit
+        from .workloads import Ansatz, tfim_terms
+        cache = self.__dict__.setdefault("_solo", {})
+        key = (n, layers, float(hx))
+        if key not in cache:
+            c = GpuCircuit.from_ansatz(Ansatz(n, layers), self.precision)
+            c.set_observable(tfim_terms(n, hx))
+            cache[key] = c
+        return cache[key]
+
     def run(self, specs: Sequence[Tuple[int, int, np.ndarray]], hx: float = 0.5,
             tile_bits: Optional[Dict[int, int]] = None, block_threads: int = 0, num_blocks: int = 0,
-            group_width: int = 0):
+            group_width: int = 0, solo: bool = True):
         """specs: (n, layers, params) per circuit; group_width 0 = the widest.
-        Returns (energies, [grad per circuit], kernel ms)."""
+        Returns (energies, [grad per circuit], kernel ms).  With `solo`, the
+        circuits of the size classes solo_classes() names go through the
+        single-circuit adjoint engine (same ansatz, observable, precision and
+        parameter layout), the rest through the persistent launch; the kernel
+        ms is the launch's."""
+        for n, layers, params in specs:
+            if np.asarray(params).size != 2 * int(n) * int(layers):
+                raise ValueError("each circuit needs 2 * n * layers parameters")
+        solo_n = set(self.solo_classes(specs)) if solo else set()
+        self.last_solo = sum(1 for sp in specs if int(sp[0]) in solo_n)
+        if solo_n:
+            energies = np.empty(len(specs), dtype=np.float64)
+            grads: List[Optional[np.ndarray]] = [None] * len(specs)
+            rest = [i for i, sp in enumerate(specs) if int(sp[0]) not in solo_n]
+            ms = 0.0
+            if rest:
+                e_r, g_r, ms = self._launch([specs[i] for i in rest], hx, tile_bits, block_threads, num_blocks,
+                                            group_width)
+                for k, i in enumerate(rest):
+                    energies[i], grads[i] = e_r[k], g_r[k]
+            for i, (n, layers, params) in enumerate(specs):
+                if int(n) in solo_n:
+                    e, g = self._solo_circuit(int(n), int(layers), hx).gradient(np.asarray(params, dtype=np.float64))
+                    energies[i], grads[i] = e, np.asarray(g, dtype=np.float64)
+            return energies, grads, ms
+        return self._launch(specs, hx, tile_bits, block_threads, num_blocks, group_width)
+
+    def _launch(self, specs, hx, tile_bits, block_threads, num_blocks, group_width):
         if binding._NATIVE_LIB.iq_batch_set_group_width(self._ptr, int(group_width)) != 0:
             binding.check_error()
         nq = np.ascontiguousarray([s[0] for s in specs], dtype=np.int32)
@@ -143,8 +208,8 @@ def autotune(runner: BatchRunner, specs: Sequence[Tuple[int, int, np.ndarray]], 
         res = {}
         for w in cands:
             for t in (256, 512):
-                runner.run(sample[:8], hx, {n: w}, t)  # warm-up
-                _, _, ms = runner.run(sample, hx, {n: w}, t)
+                runner.run(sample[:8], hx, {n: w}, t, solo=False)  # warm-up
+                _, _, ms = runner.run(sample, hx, {n: w}, t, solo=False)
                 res[f"{w}/{t}"] = ms
         timings[str(n)] = res
         for t in (256, 512):
@@ -160,8 +225,8 @@ def autotune(runner: BatchRunner, specs: Sequence[Tuple[int, int, np.ndarray]], 
     def steady(widths, threads, group_width=0):
         # the second of two back-to-back launches: a launch right after a small
         # one can run faster than the same launch repeated, as in use
-        runner.run(everything, hx, widths, threads, group_width=group_width)
-        return runner.run(everything, hx, widths, threads, group_width=group_width)[2]
+        runner.run(everything, hx, widths, threads, group_width=group_width, solo=False)
+        return runner.run(everything, hx, widths, threads, group_width=group_width, solo=False)[2]
 
     t_ms = {}
     for key_t, widths in best_by_t.items():

@@ -21,7 +21,8 @@ class ScriptedRunner:
     def __init__(self):
         self.calls = 0
 
-    def run(self, specs, hx=0.5, tile_bits=None, block_threads=0, num_blocks=0, group_width=0):
+    def run(self, specs, hx=0.5, tile_bits=None, block_threads=0, num_blocks=0, group_width=0, solo=True):
+        assert not solo  # the autotuner times the launch alone
         self.calls += 1
         ms = 0.0
         for n, l, _ in specs:
@@ -71,7 +72,8 @@ class MixRunner(ScriptedRunner):
     workgroups per compute unit and takes 0.8 of the time: only the
     whole-sample stage sees that."""
 
-    def run(self, specs, hx=0.5, tile_bits=None, block_threads=0, num_blocks=0, group_width=0):
+    def run(self, specs, hx=0.5, tile_bits=None, block_threads=0, num_blocks=0, group_width=0, solo=True):
+        assert not solo  # the autotuner times the launch alone
         self.calls += 1
         widths = [(tile_bits or {}).get(n, self.max_tile_bits) for n, _, _ in specs]
         ms = sum(l * 2.0 ** n * 1e-6 * (0.95 if w == 13 else 1.0) for (n, l, _), w in zip(specs, widths))

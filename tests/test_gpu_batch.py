@@ -36,7 +36,7 @@ def test_batch_matches_single_circuit_c128(M):
     adjoint, batch = M
     specs = _specs(1)
     r = batch.BatchRunner("complex128")
-    e, g, ms = r.run(specs, tile_bits=TILES, block_threads=256)
+    e, g, ms = r.run(specs, tile_bits=TILES, block_threads=256, solo=False)
     assert ms > 0
     for (n, l, p), ei, gi in zip(specs, e, g):
         e_ref, g_ref = _single(adjoint, n, l, p)
@@ -49,7 +49,7 @@ def test_batch_small_against_oracle(M, oracle_lib):
     n, l = 5, 3
     rng = np.random.default_rng(4)
     p = rng.uniform(0, 2 * np.pi, 2 * n * l)
-    e, g, _ = batch.BatchRunner("complex128").run([(n, l, p)])
+    e, g, _ = batch.BatchRunner("complex128").run([(n, l, p)], solo=False)
     terms = orc.tfim_terms(n)
     assert abs(e[0] - orc.energy(oracle_lib, n, orc.hea_ops(n, l, p), terms)) < 1e-12
     ps = orc.grad_parameter_shift(oracle_lib, n, lambda x: orc.hea_ops(n, l, x), terms, p)
@@ -60,8 +60,8 @@ def test_batch_complex64_error_and_determinism(M):
     adjoint, batch = M
     specs = _specs(2)
     r = batch.BatchRunner("complex64")
-    e1, g1, _ = r.run(specs, tile_bits=TILES, block_threads=512)
-    e2, g2, _ = r.run(list(reversed(specs)), tile_bits=TILES, block_threads=512)
+    e1, g1, _ = r.run(specs, tile_bits=TILES, block_threads=512, solo=False)
+    e2, g2, _ = r.run(list(reversed(specs)), tile_bits=TILES, block_threads=512, solo=False)
     for (n, l, p), a, b, ga, gb in zip(specs, e1, reversed(e2), g1, reversed(g2)):
         assert a == b and np.array_equal(ga, gb)  # independent of queue order
         e_ref, g_ref = _single(adjoint, n, l, p)
@@ -73,9 +73,9 @@ def test_batch_validation(M):
     _, batch = M
     r = batch.BatchRunner("complex64")
     with pytest.raises(ValueError):
-        r.run([(20, 2, np.zeros(80))], tile_bits={20: 9})  # 2 * width < n
+        r.run([(20, 2, np.zeros(80))], tile_bits={20: 9}, solo=False)  # 2 * width < n
     with pytest.raises(ValueError):
-        r.run([(4, 2, np.zeros(3))])
+        r.run([(4, 2, np.zeros(3))], solo=False)
 
 
 def test_autotuner_table(M, tmp_path):
@@ -88,7 +88,8 @@ def test_autotuner_table(M, tmp_path):
     assert t["group_width"] in (2, 3, 4) and set(t["timings"]["group_width"]) == {"2", "3", "4"}
     again = batch.autotune(r, specs, cache=tmp_path / "tune.json", per_n=16)
     assert again["cached"] and again["tile_bits"] == t["tile_bits"] and again["group_width"] == t["group_width"]
-    e, g, _ = r.run(specs, tile_bits=t["tile_bits"], block_threads=t["block_threads"], group_width=t["group_width"])
+    e, g, _ = r.run(specs, tile_bits=t["tile_bits"], block_threads=t["block_threads"], group_width=t["group_width"],
+                    solo=False)
     assert np.all(np.isfinite(e))
 
 
@@ -100,15 +101,15 @@ def test_group_widths_agree(M, precision, widths):
     _, batch = M
     specs = _specs(7)
     r = batch.BatchRunner(precision)
-    ref_e, ref_g, _ = r.run(specs, tile_bits=TILES, block_threads=512)
+    ref_e, ref_g, _ = r.run(specs, tile_bits=TILES, block_threads=512, solo=False)
     tol = 1e-12 if precision == "complex128" else 2e-5
     for w in widths:
-        e, g, _ = r.run(specs, tile_bits=TILES, block_threads=512, group_width=w)
+        e, g, _ = r.run(specs, tile_bits=TILES, block_threads=512, group_width=w, solo=False)
         for (n, l, _), a, b, ga, gb in zip(specs, e, ref_e, g, ref_g):
             assert abs(a - b) < tol, (w, n, l)
             assert np.max(np.abs(ga - gb)) < tol * max(1.0, np.max(np.abs(gb))), (w, n, l)
     with pytest.raises(ValueError):
-        r.run(specs[:1], group_width=5)
+        r.run(specs[:1], group_width=5, solo=False)
 
 
 @pytest.mark.parametrize("precision,tiles,threads", [("complex64", 12, 256), ("complex64", 13, 512),
@@ -125,7 +126,7 @@ def test_register_group_launches(M, precision, tiles, threads):
     shapes = [(13, 3), (15, 2), (17, 2), (18, 1), (20, 1)]
     specs = [(n, l, rng.uniform(0, 2 * np.pi, 2 * n * l)) for n, l in shapes if n <= 2 * tiles]
     r = batch.BatchRunner(precision)
-    e, g, _ = r.run(specs, tile_bits={n: tiles for n, _, _ in specs}, block_threads=threads)
+    e, g, _ = r.run(specs, tile_bits={n: tiles for n, _, _ in specs}, block_threads=threads, solo=False)
     tol = 1e-10 if precision == "complex128" else 1e-4
     for (n, l, p), ei, gi in zip(specs, e, g):
         e_ref, g_ref = _single(adjoint, n, l, p)
@@ -147,7 +148,7 @@ def test_batch_1024_circuits_every_size_stratified(M):
     print(f"1024-circuit batch, tiles {tune['tile_bits']}, {tune['block_threads']} threads, "
           f"fusion width {tune['group_width']}", flush=True)
     e, g, ms = r.run(specs, tile_bits=tune["tile_bits"], block_threads=tune["block_threads"],
-                     group_width=tune["group_width"])
+                     group_width=tune["group_width"], solo=False)
     assert ms > 0 and np.all(np.isfinite(e))
     rng = np.random.default_rng(12)
     for n in ns:
@@ -159,3 +160,32 @@ def test_batch_1024_circuits_every_size_stratified(M):
             e_ref, g_ref = _single(adjoint, n, l, p)
             assert abs(e[i] - e_ref) < 1e-4 * max(1.0, abs(e_ref)), (n, l)
             assert np.linalg.norm(g[i] - g_ref) / np.linalg.norm(g_ref) < 1e-4, (n, l)
+
+
+def test_few_large_circuits_take_the_single_circuit_engine(M):
+    """A batch whose large size classes hold a few circuits each: those go
+    through the single-circuit engine (a circuit of the persistent launch
+    holds one compute unit), the small ones through the launch.  Every
+    energy and gradient against the single-circuit complex128 adjoint, and
+    the launch's share bit for bit as a launch of those circuits alone."""
+    adjoint, batch = M
+    rng = np.random.default_rng(31)
+    specs = [(20, 5, rng.uniform(0, 2 * np.pi, 200)), (18, 3, rng.uniform(0, 2 * np.pi, 108))]
+    specs += [(10, 4, rng.uniform(0, 2 * np.pi, 80)) for _ in range(20)]
+    specs += [(20, 2, rng.uniform(0, 2 * np.pi, 80))]
+    r = batch.BatchRunner("complex64")
+    assert r.solo_classes(specs) == [18, 20]
+    e, g, ms = r.run(specs)
+    assert r.last_solo == 3 and ms > 0
+    small = [s for s in specs if s[0] == 10]
+    e_b, g_b, _ = r.run(small, solo=False)
+    k = 0
+    for i, (n, l, p) in enumerate(specs):
+        e_ref, g_ref = _single(adjoint, n, l, p)
+        assert abs(e[i] - e_ref) < 1e-5
+        assert np.linalg.norm(g[i] - g_ref) <= 1e-4 * max(1.0, np.linalg.norm(g_ref))
+        if n == 10:
+            assert e[i] == e_b[k] and np.array_equal(g[i], g_b[k])
+            k += 1
+    with pytest.raises(ValueError):
+        r.run([(20, 2, np.zeros(79))])
