@@ -10,7 +10,6 @@
 #include "reduce.hpp"
 
 #include <algorithm>
-#include <cstdlib>
 
 namespace iqg {
 
@@ -739,36 +738,14 @@ __global__ void __launch_bounds__(64) k_herk_mfma(const double2* __restrict__ ph
 // transpose; in a diagonal block only the entries on or above the diagonal
 // are kept, each written with its conjugate transpose, and the diagonal's
 // imaginary parts (zero in exact arithmetic) as zero, so the matrix is
-// exactly Hermitian.
-// Block order (XT, from 64 blocks a side, d >= 4096): workgroup b runs on XCD b % 8, and
-// XCD x takes the super-tiles x, x + 8, ... of 8 x 8 blocks of the upper
-// triangle, one at a time, so the blocks that run together on one XCD share
-// 16 strips through its L2 (26 qubits, 13 kept: 41 against 48 ms).  The
-// three-multiply form (Re = Xr^T Xr + Xi^T Xi, Im = (Xr + Xi)^T (Xr - Xi) -
-// Xr^T Xr + Xi^T Xi) was measured no faster: its third accumulator set costs
-// a wave per SIMD, and the kernel is not bound by the matrix cores.
+// exactly Hermitian.  Used below 64 blocks a side (fewer than 12 kept
+// qubits); from there k_herk_dma takes over.
 constexpr uint32_t kHerkBlock = 64, kHerkRows = 16;
 
-template <bool XT>
 __global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict__ phi, uint64_t m, uint32_t d,
-                                                      double2* __restrict__ out, uint32_t ns, uint64_t mchunk,
-                                                      uint64_t slice) {
+                                                      double2* __restrict__ out, uint64_t mchunk, uint64_t slice) {
     typedef double v4 __attribute__((ext_vector_type(4)));
-    uint32_t bR, bC;
-    if (XT) {
-        const uint32_t b = blockIdx.x, x = b & 7u, s = b >> 3, w = s & 63u;
-        uint32_t g = (s >> 6) * 8u + x, sr = 0;  // super-tile g of the upper triangle, row by row
-        while (sr < ns && g >= ns - sr) {
-            g -= ns - sr;
-            ++sr;
-        }
-        if (sr >= ns) return;
-        bR = sr * 8u + (w >> 3);
-        bC = (sr + g) * 8u + (w & 7u);
-    } else {
-        bR = blockIdx.y;
-        bC = blockIdx.x;
-    }
+    const uint32_t bR = blockIdx.y, bC = blockIdx.x;
     if (bC < bR) return;
     // environment chunk blockIdx.z: rows [z * mchunk, (z + 1) * mchunk), its
     // partial matrix at out + z * slice
@@ -839,11 +816,22 @@ __global__ void __launch_bounds__(256) k_herk_blocked(const double2* __restrict_
 // in the other, one barrier per step.  The two buffers are distinct arrays
 // used in a fixed alternation (and a diagonal block loads its strip into
 // both), so the compiler can see that a step's LDS reads do not touch the
-// loads in flight and issues no memory wait before them.  RW rows per step;
-// the XCD block order, operand layout and results of k_herk_blocked, the
-// same sums in the same order.  64 KiB of LDS per block at RW = 16.
+// loads in flight and issues no memory wait before them.  RW rows per step.
+// Block order: workgroup b runs on XCD b % 8, and XCD x takes the
+// super-tiles x, x + 8, ... of 8 x 8 blocks of the upper triangle, one at a
+// time, so the blocks that run together on one XCD share 16 strips through
+// its L2 (used from 8 super-tiles a side: with fewer, the work would sit on a
+// few XCDs).  The strips come in under the products, so the kernel is bound
+// by the matrix cores, and it takes three real products per tile where
+// k_herk_blocked takes four:
+//   re = Xr^T Xr,  im = Xi^T Xi,  t3 = (Xr + Xi)^T (Xr - Xi),
+//   Re = re + im,  Im = Xi^T Xr - Xr^T Xi = t3 - re + im.
+// 64 KiB of LDS per block at RW = 16; two blocks per CU are asked of the
+// register allocator (226 VGPRs; unconstrained it kept two copies of the
+// accumulators and fitted one wave per SIMD).  Environment chunks as
+// k_herk_blocked.
 template <int RW>
-__global__ void __launch_bounds__(256) k_herk_dma(const double2* __restrict__ phi, uint64_t m, uint32_t d,
+__global__ void __launch_bounds__(256, 2) k_herk_dma(const double2* __restrict__ phi, uint64_t m, uint32_t d,
                                                   double2* __restrict__ out, uint32_t ns, uint64_t mchunk,
                                                   uint64_t slice) {
     typedef double v4 __attribute__((ext_vector_type(4)));
@@ -880,13 +868,14 @@ __global__ void __launch_bounds__(256) k_herk_dma(const double2* __restrict__ ph
                                              (__attribute__((address_space(3))) void*)&SB[er][0], 16, 0, 0);
         }
     };
-    v4 re[2][2], im[2][2];
+    v4 re[2][2], im[2][2], t3[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             re[a][b] = v4{0, 0, 0, 0};
             im[a][b] = v4{0, 0, 0, 0};
+            t3[a][b] = v4{0, 0, 0, 0};
         }
     auto compute = [&](const double2 (*A)[kHerkBlock], const double2 (*B)[kHerkBlock]) __attribute__((always_inline)) {
 #pragma unroll
@@ -902,9 +891,9 @@ __global__ void __launch_bounds__(256) k_herk_dma(const double2* __restrict__ ph
 #pragma unroll
                 for (int tb = 0; tb < 2; ++tb) {
                     re[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].x, b[tb].x, re[ta][tb], 0, 0, 0);
-                    re[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].y, b[tb].y, re[ta][tb], 0, 0, 0);
-                    im[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].y, b[tb].x, im[ta][tb], 0, 0, 0);
-                    im[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[ta].x, b[tb].y, im[ta][tb], 0, 0, 0);
+                    im[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].y, b[tb].y, im[ta][tb], 0, 0, 0);
+                    t3[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].x + a[ta].y, b[tb].x - b[tb].y, t3[ta][tb],
+                                                                      0, 0, 0);
                 }
         }
     };
@@ -934,7 +923,8 @@ __global__ void __launch_bounds__(256) k_herk_dma(const double2* __restrict__ ph
             for (int jj = 0; jj < 4; ++jj) {
                 const uint64_t row = r0 + wr * 32 + ta * 16 + kk + 4 * jj, col = c0 + wc * 32 + tb * 16 + j;
                 if (diag && row > col) continue;
-                const double vr = re[ta][tb][jj], vi = row == col ? 0.0 : im[ta][tb][jj];
+                const double vr = re[ta][tb][jj] + im[ta][tb][jj];
+                const double vi = row == col ? 0.0 : (t3[ta][tb][jj] - re[ta][tb][jj]) + im[ta][tb][jj];
                 out[row * d + col] = make_double2(vr, vi);
                 if (row != col) out[col * d + row] = make_double2(vr, -vi);
             }
@@ -1527,20 +1517,16 @@ hipError_t launch_partial_trace(const void* psi, int n, int prec, const uint32_t
             const uint64_t sl = S > 1 ? slice : 0;
             // the XCD order from 8 super-tiles a side (every XCD gets several;
             // with fewer the work would sit on a few XCDs)
-            if (nb >= 256) {
-                // 14 or more kept: strips by direct global-to-LDS loads,
-                // double buffered (15 of 30 kept: 2.27 against 2.69 s)
+            if (nb >= 64) {
+                // 12 or more kept: strips by direct global-to-LDS loads,
+                // double buffered, three products per tile, in the XCD order
+                // (15 of 30 kept: 1.81 against 2.69 s; 13 of 26: 44 against
+                // 55 ms)
                 const uint32_t ns = nb / 8, tiles = ns * (ns + 1) / 2;
                 const uint32_t blocks = ((tiles + 7) / 8) * 8 * 64;
                 hipLaunchKernelGGL((k_herk_dma<16>), dim3(blocks, 1, S), dim3(256), 0, st, phi, m, d, dst, ns, mchunk, sl);
-            } else if (nb >= 64) {
-                const uint32_t ns = nb / 8, tiles = ns * (ns + 1) / 2;
-                const uint32_t blocks = ((tiles + 7) / 8) * 8 * 64;
-                hipLaunchKernelGGL(k_herk_blocked<true>, dim3(blocks, 1, S), dim3(256), 0, st, phi, m, d, dst, ns, mchunk,
-                                   sl);
             } else {
-                hipLaunchKernelGGL(k_herk_blocked<false>, dim3(nb, nb, S), dim3(256), 0, st, phi, m, d, dst, 0u, mchunk,
-                                   sl);
+                hipLaunchKernelGGL(k_herk_blocked, dim3(nb, nb, S), dim3(256), 0, st, phi, m, d, dst, mchunk, sl);
             }
         } else {
             const uint32_t nt = d / 16;

@@ -211,3 +211,20 @@ def test_reduced_matrix_environment_chunks(oracle_lib, n, keep, precision):
     assert np.array_equal(got, got.conj().T)
     assert abs(np.trace(got).real - 1.0) < (1e-12 if precision == "complex128" else 1e-5)
     assert abs(sv.von_neumann_entropy(keep) - ref.von_neumann_entropy(keep)) < (1e-9 if precision == "complex128" else 1e-4)
+
+
+def test_reduced_matrix_14_kept_matrix_core_direct_loads(oracle_lib):
+    """14 kept qubits of 18 (d = 16384): the product whose strips arrive by
+    direct loads into LDS, in the three-product form, against the oracle
+    element by element (rows compared in chunks), exactly Hermitian."""
+    n, keep = 18, [13, 0, 17, 2, 9, 4, 15, 6, 11, 8, 1, 10, 3, 12]
+    sv, ref = states(oracle_lib, n, seed=41)
+    print(f"reduced matrix n={n} k={len(keep)} (direct loads)", flush=True)
+    got = sv.reduced_density_matrix(keep)
+    want = ref.partial_trace(keep)
+    step = 1024
+    for r in range(0, got.shape[0], step):
+        assert np.max(np.abs(got[r:r + step] - want[r:r + step])) < 1e-13
+    assert abs(np.trace(got).real - 1.0) < 1e-12
+    for r in range(0, got.shape[0], step):
+        assert np.array_equal(got[r:r + step], got[:, r:r + step].conj().T)
