@@ -122,6 +122,36 @@ def _probabilities(probs, n):
     return dict(zip(_bitstrings(idx, n), p[idx].tolist()))
 
 
+def _unitary_flags(norm) -> List[bool]:
+    """For each normalised op: a gate whose matrix is a finite unitary of its
+    size, by the test of statevector.py:55-65 (np.allclose(U^H U, I,
+    atol=1e-10, rtol=1e-10) on a finite 2^k x 2^k matrix), made for all gates
+    of one size at once; True for the other ops."""
+    flags = [True] * len(norm)
+    by_d: Dict[int, List[int]] = {}
+    mats: Dict[int, np.ndarray] = {}
+    for i, (kind, _, qubits, matrix) in enumerate(norm):
+        if kind != "gate":
+            continue
+        d = 1 << len(qubits)
+        m = np.asarray(matrix, dtype=np.complex128)
+        if m.shape != (d, d):
+            flags[i] = False
+            continue
+        by_d.setdefault(d, []).append(i)
+        mats[i] = m
+    for d, idx in by_d.items():
+        M = np.stack([mats[i] for i in idx])
+        finite = np.isfinite(M).all(axis=(1, 2))
+        with np.errstate(invalid="ignore", over="ignore"):
+            P = np.conj(np.swapaxes(M, 1, 2)) @ M
+            eye = np.eye(d)
+            close = (np.abs(P - eye) <= 1e-10 + 1e-10 * np.abs(eye)).all(axis=(1, 2))
+        for k, i in enumerate(idx):
+            flags[i] = bool(finite[k] and close[k])
+    return flags
+
+
 class StateVectorSimulator:
     """GPU counterpart of invariantq/simulator/statevector.py:16-144."""
 
@@ -161,33 +191,27 @@ class StateVectorSimulator:
             pending.clear()
 
         norm = [_normalise(raw) for raw in ops]
+        unitary = _unitary_flags(norm)
         if (self.tiled and not allow_non_unitary and n >= 16
                 and all(k == "gate" and len(q) <= 2 for k, _, q, _ in norm)):
             gl = []
-            for _, name, qubits, matrix in norm:
-                m = np.asarray(matrix, dtype=np.complex128)
-                d = 1 << len(qubits)
-                ok = (m.shape == (d, d) and np.isfinite(m).all()
-                      and np.allclose(m.conj().T @ m, np.eye(d), atol=1e-10, rtol=1e-10))
+            for (_, name, qubits, matrix), ok in zip(norm, unitary):
                 if not ok:
                     raise ValueError(f"Gate '{name}' is not a finite unitary matrix")
-                gl.append((list(qubits), m))
+                gl.append((list(qubits), np.asarray(matrix, dtype=np.complex128)))
             from .adjoint import GpuCircuit
             circ = GpuCircuit.from_gates(n, gl, self.precision)
             circ.run()
             circ.copy_state_to(sv)
             return sv, circ.num_segments
-        for kind, name, qubits, matrix in norm:
+        for (kind, name, qubits, matrix), ok in zip(norm, unitary):
             if kind == "reset":
                 drain()
                 sv.reset_qubit(qubits[0])
                 continue
             if kind != "gate":
                 raise ValueError("noise channels need the density-matrix simulator")
-            d = 1 << len(qubits)
             m = np.asarray(matrix, dtype=np.complex128)
-            ok = (m.shape == (d, d) and np.isfinite(m).all()
-                  and np.allclose(m.conj().T @ m, np.eye(d), atol=1e-10, rtol=1e-10))
             if not ok and not allow_non_unitary:
                 raise ValueError(f"Gate '{name}' is not a finite unitary matrix")
             if self.fusion_width:

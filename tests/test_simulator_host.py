@@ -50,3 +50,30 @@ def test_counts_and_probabilities_match_the_reference_loops():
         gotp = _probabilities(probs, n)
         assert gotp == wantp and list(gotp) == list(wantp)
     assert _counts(np.array([], dtype=np.uint64), 3) == {}
+
+
+def test_batched_unitarity_check_matches_per_gate_allclose():
+    """The gates' unitarity test made for all gates of one size at once gives
+    the per-gate decision of statevector.py:55-65 (finite, and
+    np.allclose(U^H U, I, atol=1e-10, rtol=1e-10)), near the tolerance too."""
+    import numpy as np
+    from invariantq_amd import workloads
+    from invariantq_amd.simulator import _unitary_flags
+    rng = np.random.default_rng(1)
+    cases = []
+    for k in range(600):
+        d = (2, 4, 8)[k % 3]
+        U = workloads.haar_unitary(rng, d)
+        U = U + 10.0 ** rng.uniform(-13, -8) * (rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))
+        if k % 97 == 0:
+            U[0, 0] = np.nan
+        cases.append(("gate", "u", list(range(d.bit_length() - 1)), U))
+    cases.insert(5, ("reset", None, [0], None))
+    cases.insert(9, ("gate", "bad", [0], np.eye(4)))  # wrong size for one qubit
+    got = _unitary_flags(cases)
+    want = [True if k != "gate" else bool(np.asarray(m).shape == (1 << len(q), 1 << len(q))
+                                          and np.isfinite(m).all()
+                                          and np.allclose(np.conj(m).T @ m, np.eye(m.shape[0]), atol=1e-10, rtol=1e-10))
+            for k, _, q, m in cases]
+    assert got == want
+    assert 0 < sum(want) < len(want)
