@@ -994,6 +994,9 @@ def main():
                "pct_hbm_peak": round(100 * b64 * a.blocks / wall64 / 1e12 / HBM_PEAK_TBPS, 2),
                "ms_per_step": round(wall64 * 1e3, 3),
                "fp32_TFLOPs": round(flops_block * a.blocks / wall64 / 1e12, 3),
+               # of the FP32 matrix peak, counting the dense complex product
+               # (8 flops per complex multiply-add; the kernel does 6 of them)
+               "fp32_pct_peak": round(100 * flops_block * a.blocks / wall64 / 1e12 / 157.3, 1),
                "avg_kernel_ms": round(float(np.mean(mean64)) * 1e3, 4), "norm": norm64,
                "roofline": roofline("k_gate_mfma<float, ...> (all 200 blocks)", b64, float(np.mean(mean64)),
                                     int(mean64.size))}
@@ -1052,6 +1055,10 @@ def main():
                    "qubits": n, "blocks_per_step": a.blocks, "precision": "complex128", "parallelism": "single GPU"},
         "pct_hbm_peak": round(100 * value / HBM_PEAK_TBPS, 2),
         "fp64_TFLOPs": round(flops_block * a.blocks / wall / 1e12, 3),
+        # of the FP64 matrix peak (78.6 TFLOP/s), counting the dense complex
+        # product (8 flops per complex multiply-add; the kernel's 3-multiply
+        # form does 6)
+        "fp64_pct_peak": round(100 * flops_block * a.blocks / wall / 1e12 / 78.6, 1),
         "gate_applications_per_s": round(a.blocks / wall, 2),
         "roofline": {"bound": "hbm", "achieved": round(bytes_block / dom_d / 1e9, 1), "peak": HBM_PEAK_TBPS * 1000,
                      "unit": "GB/s", "frac": round(bytes_block / dom_d / 1e12 / HBM_PEAK_TBPS, 4),
