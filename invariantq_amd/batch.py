@@ -60,7 +60,7 @@ class BatchRunner:
     # every compute unit on one circuit, takes about 0.5 ms).  Per circuit on
     # one CU (complex64, measured at n = 14, 17, 20): about 36 ms * 2^(n-20)
     # * layers / 6; on the single-circuit engine about 0.28 ms + 3.5e-8 ms *
-    # 2^n * layers.
+    # 2^n * layers (its circuit made once per shape and call).
     _CU_MS_20_6 = 36.0
     _ENGINE_MS_FIXED, _ENGINE_MS_PER_AMP_LAYER = 0.28, 3.5e-8
 
@@ -81,10 +81,12 @@ class BatchRunner:
                 out.append(n)
         return sorted(out)
 
-    def _solo_circuit(self, n: int, layers: int, hx: float):
+    def _solo_circuit(self, cache: Dict, n: int, layers: int, hx: float):
+        """The single-circuit engine's circuit for one (n, layers, hx), made
+        once per run() call: each holds device memory (its state and the
+        stored states of its sweep), which is given back when the call ends."""
         from .adjoint import GpuCircuit
         from .workloads import Ansatz, tfim_terms
-        cache = self.__dict__.setdefault("_solo", {})
         key = (n, layers, float(hx))
         if key not in cache:
             c = GpuCircuit.from_ansatz(Ansatz(n, layers), self.precision)
@@ -116,10 +118,13 @@ class BatchRunner:
                                             group_width)
                 for k, i in enumerate(rest):
                     energies[i], grads[i] = e_r[k], g_r[k]
+            circuits: Dict = {}
             for i, (n, layers, params) in enumerate(specs):
                 if int(n) in solo_n:
-                    e, g = self._solo_circuit(int(n), int(layers), hx).gradient(np.asarray(params, dtype=np.float64))
+                    c = self._solo_circuit(circuits, int(n), int(layers), hx)
+                    e, g = c.gradient(np.asarray(params, dtype=np.float64))
                     energies[i], grads[i] = e, np.asarray(g, dtype=np.float64)
+            circuits.clear()
             return energies, grads, ms
         return self._launch(specs, hx, tile_bits, block_threads, num_blocks, group_width)
 
