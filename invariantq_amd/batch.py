@@ -60,9 +60,13 @@ class BatchRunner:
     # every compute unit on one circuit, takes about 0.5 ms).  Per circuit on
     # one CU (complex64, measured at n = 14, 17, 20): about 36 ms * 2^(n-20)
     # * layers / 6; on the single-circuit engine about 0.28 ms + 3.5e-8 ms *
-    # 2^n * layers (its circuit made once per shape and call).
+    # 2^n * layers, plus making the circuit for a shape not already kept.
     _CU_MS_20_6 = 36.0
     _ENGINE_MS_FIXED, _ENGINE_MS_PER_AMP_LAYER = 0.28, 3.5e-8
+    # making one such circuit (its tape, observable and first plan): about
+    # 5 ms at n = 20, 6 layers
+    _BUILD_MS_FIXED, _BUILD_MS_PER_GATE = 2.0, 0.025
+    _SOLO_CACHE = 8  # circuits kept for later calls (least recently used dropped)
 
     def solo_classes(self, specs: Sequence[Tuple[int, int, np.ndarray]]) -> List[int]:
         """Qubit counts whose circuits the single-circuit engine evaluates
@@ -71,27 +75,36 @@ class BatchRunner:
         by_n: Dict[int, List[int]] = {}
         for n, layers, _ in specs:
             by_n.setdefault(int(n), []).append(int(layers))
+        cached = self.__dict__.get("_solo", {})
         out = []
         for n, ls in by_n.items():
             if n < 16:
                 continue
             one_cu = self._CU_MS_20_6 * 2.0 ** (n - 20) * max(ls) / 6.0
             engine = sum(self._ENGINE_MS_FIXED + self._ENGINE_MS_PER_AMP_LAYER * 2.0 ** n * l for l in ls)
+            engine += sum(self._BUILD_MS_FIXED + self._BUILD_MS_PER_GATE * 2 * n * l
+                          for l in set(ls) if not any(k[0] == n and k[1] == l for k in cached))
             if engine < 0.5 * one_cu:
                 out.append(n)
         return sorted(out)
 
-    def _solo_circuit(self, cache: Dict, n: int, layers: int, hx: float):
-        """The single-circuit engine's circuit for one (n, layers, hx), made
-        once per run() call: each holds device memory (its state and the
-        stored states of its sweep), which is given back when the call ends."""
+    def _solo_circuit(self, n: int, layers: int, hx: float):
+        """The single-circuit engine's circuit for one (n, layers, hx).  Each
+        holds device memory (its state and the stored states of its sweep), so
+        only the _SOLO_CACHE most recently used are kept between calls."""
+        from collections import OrderedDict
         from .adjoint import GpuCircuit
         from .workloads import Ansatz, tfim_terms
+        cache = self.__dict__.setdefault("_solo", OrderedDict())
         key = (n, layers, float(hx))
-        if key not in cache:
+        if key in cache:
+            cache.move_to_end(key)
+        else:
             c = GpuCircuit.from_ansatz(Ansatz(n, layers), self.precision)
             c.set_observable(tfim_terms(n, hx))
             cache[key] = c
+            while len(cache) > self._SOLO_CACHE:
+                cache.popitem(last=False)
         return cache[key]
 
     def run(self, specs: Sequence[Tuple[int, int, np.ndarray]], hx: float = 0.5,
@@ -118,13 +131,11 @@ class BatchRunner:
                                             group_width)
                 for k, i in enumerate(rest):
                     energies[i], grads[i] = e_r[k], g_r[k]
-            circuits: Dict = {}
             for i, (n, layers, params) in enumerate(specs):
                 if int(n) in solo_n:
-                    c = self._solo_circuit(circuits, int(n), int(layers), hx)
+                    c = self._solo_circuit(int(n), int(layers), hx)
                     e, g = c.gradient(np.asarray(params, dtype=np.float64))
                     energies[i], grads[i] = e, np.asarray(g, dtype=np.float64)
-            circuits.clear()
             return energies, grads, ms
         return self._launch(specs, hx, tile_bits, block_threads, num_blocks, group_width)
 

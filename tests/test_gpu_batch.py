@@ -170,13 +170,16 @@ def test_few_large_circuits_take_the_single_circuit_engine(M):
     the launch's share bit for bit as a launch of those circuits alone."""
     adjoint, batch = M
     rng = np.random.default_rng(31)
-    specs = [(20, 5, rng.uniform(0, 2 * np.pi, 200)), (18, 3, rng.uniform(0, 2 * np.pi, 108))]
+    specs = [(20, 5, rng.uniform(0, 2 * np.pi, 200)), (19, 10, rng.uniform(0, 2 * np.pi, 380))]
     specs += [(10, 4, rng.uniform(0, 2 * np.pi, 80)) for _ in range(20)]
-    specs += [(20, 2, rng.uniform(0, 2 * np.pi, 80))]
+    specs += [(20, 2, rng.uniform(0, 2 * np.pi, 80)), (19, 10, rng.uniform(0, 2 * np.pi, 380))]
     r = batch.BatchRunner("complex64")
-    assert r.solo_classes(specs) == [18, 20]
+    assert r.solo_classes(specs) == [19, 20]
+    # one shallow 18-qubit circuit stays in the launch: making its circuit
+    # costs more than the launch would
+    assert r.solo_classes([(18, 3, rng.uniform(0, 2 * np.pi, 108))]) == []
     e, g, ms = r.run(specs)
-    assert r.last_solo == 3 and ms > 0
+    assert r.last_solo == 4 and ms > 0
     small = [s for s in specs if s[0] == 10]
     e_b, g_b, _ = r.run(small, solo=False)
     k = 0
