@@ -67,6 +67,42 @@ def test_embed_matches_brute_force():
     assert np.max(np.abs(full_operator(8, [5, 2], m) - full_operator(8, [2, 7, 5], e))) < 1e-14
 
 
+def test_index_map_embedding_and_left_product():
+    """embed_fast and left_apply (the bucket composition's index maps) against
+    the brute-force operators, on wire lists in every order, DM wires included."""
+    rng = np.random.default_rng(5)
+    for _ in range(60):
+        kt = int(rng.integers(1, 6))
+        target = [int(x) for x in rng.permutation(9)[:kt]]
+        k = int(rng.integers(1, kt + 1))
+        wires = [int(x) for x in rng.permutation(target)[:k]]
+        m = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+        a = rng.normal(size=(1 << kt, 1 << kt)) + 1j * rng.normal(size=(1 << kt, 1 << kt))
+        e = fusion.embed_fast(m, wires, target)
+        assert np.max(np.abs(full_operator(9, wires, m) - full_operator(9, target, e))) < 1e-13
+        assert np.array_equal(e, fusion.embed(m, wires, target))
+        assert np.max(np.abs(fusion.left_apply(m, wires, a, target) - e @ a)) < 1e-12
+    rw = fusion.dm_wires([3, 1])
+    m = rng.normal(size=(4, 4))
+    assert np.array_equal(fusion.embed_fast(m, [("c", 1), ("r", 3)], rw), fusion.embed(m, [("c", 1), ("r", 3)], rw))
+
+
+def test_partial_emission_keeps_the_bucket_that_fits():
+    """A two-qubit gate between a full bucket and a small one emits only the
+    full one and joins the small one: qubits 0-4 fill a bucket, qubits 5-6
+    another; CX(4, 5) cannot join both, so the first is emitted and the CX
+    joins {5, 6}.  Then the 20-qubit brick circuit's pass count (89 when every
+    touched bucket was emitted)."""
+    rng = np.random.default_rng(8)
+    ops = [("gate", "u", [0, 1, 2, 3, 4], haar_unitary(rng, 32)), ("gate", "u", [5, 6], haar_unitary(rng, 4)),
+           ("gate", "cx", [4, 5], gates.gate_matrix("cx"))]
+    fused = fusion.fuse_statevector(ops, 5)
+    assert [sorted(f.qubits) for f in fused] == [[0, 1, 2, 3, 4], [4, 5, 6]]
+    a = product(7, [(o[2], o[3]) for o in ops])
+    assert np.max(np.abs(a - product(7, [(list(f.qubits), f.matrix) for f in fused]))) < 1e-13
+    assert len(fusion.fuse_statevector(workloads.random_u3_cx(20, 20, seed=2024), 5)) <= 81
+
+
 def test_fuser_rejects_bad_ops():
     with pytest.raises(ValueError):
         fusion.fuse_statevector([("gate", "u", [1, 1], np.eye(4))], 5)
