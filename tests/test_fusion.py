@@ -91,8 +91,10 @@ def test_partial_emission_keeps_the_bucket_that_fits():
     """A two-qubit gate between a full bucket and a small one emits only the
     full one and joins the small one: qubits 0-4 fill a bucket, qubits 5-6
     another; CX(4, 5) cannot join both, so the first is emitted and the CX
-    joins {5, 6}.  Then the 20-qubit brick circuit's pass count (89 when every
-    touched bucket was emitted)."""
+    joins {5, 6}.  Then pass counts: the 20-qubit brick circuit (89 when
+    every touched bucket was emitted and a gate touching none joined the
+    fullest open bucket), and a lone U3 layer on 20 qubits, whose 20
+    one-qubit buckets are packed into four when flushed."""
     rng = np.random.default_rng(8)
     ops = [("gate", "u", [0, 1, 2, 3, 4], haar_unitary(rng, 32)), ("gate", "u", [5, 6], haar_unitary(rng, 4)),
            ("gate", "cx", [4, 5], gates.gate_matrix("cx"))]
@@ -100,7 +102,14 @@ def test_partial_emission_keeps_the_bucket_that_fits():
     assert [sorted(f.qubits) for f in fused] == [[0, 1, 2, 3, 4], [4, 5, 6]]
     a = product(7, [(o[2], o[3]) for o in ops])
     assert np.max(np.abs(a - product(7, [(list(f.qubits), f.matrix) for f in fused]))) < 1e-13
-    assert len(fusion.fuse_statevector(workloads.random_u3_cx(20, 20, seed=2024), 5)) <= 81
+    assert len(fusion.fuse_statevector(workloads.random_u3_cx(20, 20, seed=2024), 5)) <= 46
+    layer = workloads.u3_layer(20, seed=3)
+    fused = fusion.fuse_statevector(layer, 5)
+    assert len(fused) == 4 and sorted(q for f in fused for q in f.qubits) == list(range(20))
+    a = product(10, [(op.qubits, op.matrix) for op in layer if op.qubits[0] < 10])
+    small = fusion.fuse_statevector([op for op in layer if op.qubits[0] < 10], 5)
+    assert len(small) == 2
+    assert np.max(np.abs(a - product(10, [(list(f.qubits), f.matrix) for f in small]))) < 1e-13
 
 
 def test_fuser_rejects_bad_ops():
