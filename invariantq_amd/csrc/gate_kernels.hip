@@ -499,6 +499,8 @@ static int g_direct_nt = 1;
 // qubits: 5.3 against 8.1 us per gate.
 constexpr uint64_t kXcdStateMax = 16ULL << 20;
 constexpr unsigned kXcds = 8;  // XCDs of the MI355X (gfx950)
+// The MI355X's Infinity Cache (the memory-side cache in front of HBM).
+constexpr uint64_t kCacheStateMax = 256ULL << 20;
 
 // xm for a launch of `items` work items (one per thread) in `blocks` blocks of
 // 256, over a state of `state_bytes`: log2(items) when the split applies (a
@@ -691,7 +693,11 @@ hipError_t launch_mfma(void* psi, const MfmaArgs& args, hipStream_t st, const vo
             state_bytes <= kXcdStateMax)
                ? a.bits
                : 0u;
-    const bool nt = g_mfma_nt >= 0 ? g_mfma_nt != 0 : mfma_whole_lines(a, f64);
+    // streaming hints only for states larger than the 256 MiB Infinity Cache:
+    // a smaller state stays there between passes when cached (24 random
+    // 5-qubit blocks, complex128: 22 qubits 37.7 -> 32.7 us per pass, 24
+    // qubits 122 -> 101 us; 25 qubits 199 us with the hints, 211 without)
+    const bool nt = g_mfma_nt >= 0 ? g_mfma_nt != 0 : (mfma_whole_lines(a, f64) && state_bytes > kCacheStateMax);
     if (nt)
         hipLaunchKernelGGL((k_gate_mfma<R, UB, WLDS, true>), dim3((unsigned)blocks), dim3(256), 0, st, (cv<R>*)psi, a,
                            (const R*)dmat);
