@@ -524,11 +524,16 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
     constexpr int D = 1 << K;
     const uint32_t xm = xcd_log2(ngroups, blocks, (1ULL << n) * sizeof(cv<R>));
     // streaming when no target sits inside a 128-byte line and the state is
-    // well beyond the caches (measured on a 14-qubit density matrix, 2^28
-    // complex128 entries: one-qubit Kraus channels 5.43 against 5.27 TB/s,
-    // one-qubit unitaries 5.65 against 5.45)
+    // beyond the 256 MiB Infinity Cache (measured on a 14-qubit density
+    // matrix, 2^28 complex128 entries: one-qubit Kraus channels 5.43 against
+    // 5.27 TB/s, one-qubit unitaries 5.65 against 5.45).  A state that fits
+    // stays there between gates when cached: per random one- or two-qubit
+    // gate, complex128 24 qubits 63-67 us against 79-87 with the hints,
+    // complex64 25 qubits 66-73 against 82-93; at 25 qubits complex128 the
+    // hints win, 159-176 against 177-191 us.
     const uint32_t line_bits = std::is_same<R, double>::value ? 3u : 4u;
-    bool whole = n >= 24 && g_direct_nt != 0;
+    const bool stream = ((1ULL << n) * sizeof(cv<R>)) > kCacheStateMax && g_direct_nt != 0;
+    bool whole = stream;
     for (int m = 0; m < K; ++m) whole = whole && g.q[m] >= line_bits;
     if constexpr (K <= 2) {
         // Lane-exchange forms for targets below bit 6, on states beyond the
@@ -548,7 +553,7 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
                 const uint64_t npairs = 1ULL << (n - 1);
                 const unsigned pblocks = grid_for(npairs, 256);
                 const uint32_t tl = low0 ? q0 : q1, th = low0 ? q1 : q0;
-                if (n >= 24 && g_direct_nt != 0)
+                if (stream)
                     hipLaunchKernelGGL((k_gate_lanes_mixed<R, true>), dim3(pblocks), dim3(256), 0, st, (cv<R>*)psi, npairs,
                                        tl, th, low0 ? 1 : 0, mv);
                 else
@@ -574,7 +579,7 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
             pack_gate_matrix(std::is_same<R, double>::value ? kC128 : kC64, K, false, g.re, g.im, mv.v);
             const uint64_t count = 1ULL << n;
             const unsigned lblocks = grid_for(count, 256);
-            if (n >= 24 && g_direct_nt != 0)
+            if (stream)
                 hipLaunchKernelGGL((k_gate_lanes<R, K, true>), dim3(lblocks), dim3(256), 0, st, (cv<R>*)psi, count, la, mv);
             else
                 hipLaunchKernelGGL((k_gate_lanes<R, K, false>), dim3(lblocks), dim3(256), 0, st, (cv<R>*)psi, count, la, mv);
