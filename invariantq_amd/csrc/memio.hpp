@@ -37,8 +37,11 @@ __device__ __forceinline__ void st_stream(cv<R>* p, cv<R> v, bool nt) {
 // diagonal operator (a block takes spans of kPassSpan * blockDim).
 constexpr int kPassSpan = 1;
 
-// States from this many amplitudes up are far beyond L2 and the Infinity
-// Cache, where whole-state streaming passes use the hint.
-constexpr uint64_t kStreamMinCount = 1ULL << 26;
+// The MI355X's Infinity Cache (the memory-side cache in front of HBM).  A
+// state no larger stays there between whole-state passes when they are
+// cached; larger ones take the streaming (non-temporal) hint, where each
+// amplitude is touched once per pass.
+constexpr uint64_t kCacheStateMax = 256ULL << 20;
+inline bool beyond_cache(uint64_t count, uint64_t elem_bytes) { return count * elem_bytes > kCacheStateMax; }
 
 }  // namespace iqg

@@ -25,7 +25,7 @@ inline unsigned blocks_for(uint64_t items, unsigned block, uint64_t cap) {
 
 // Whole-state passes: a grid-stride loop with four independent elements per
 // thread and iteration (four loads in flight), then a tail of single ones.
-// NT adds the streaming hint (states of kStreamMinCount amplitudes and up).
+// NT adds the streaming hint (states larger than kCacheStateMax).
 template <int U, typename Body>
 __device__ __forceinline__ void grid_stride(uint64_t count, Body&& body) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -1194,7 +1194,7 @@ __global__ void __launch_bounds__(256) k_upload(const double2* __restrict__ src,
 // instantiation by state size, complex128 or complex64 by precision.
 #define IQ_LAUNCH_PASS(KERNEL, count, prec, blocks, st, ptr_cast, ...)                                         \
     do {                                                                                                      \
-        const bool nt_ = (count) >= kStreamMinCount && stream_hint_enabled();                                 \
+        const bool nt_ = beyond_cache((count), (prec) == kC128 ? 16 : 8) && stream_hint_enabled();             \
         if ((prec) == kC128) {                                                                                \
             if (nt_) hipLaunchKernelGGL((KERNEL<double, true>), dim3(blocks), dim3(256), 0, st, (ptr_cast<double>)__VA_ARGS__); \
             else hipLaunchKernelGGL((KERNEL<double, false>), dim3(blocks), dim3(256), 0, st, (ptr_cast<double>)__VA_ARGS__); \
@@ -1323,7 +1323,7 @@ hipError_t reduce_pauli_group(const void* psi, int n, int prec, uint64_t xmask, 
     // streaming when each load instruction covers whole 128-byte lines: the
     // pairs' low index runs are 2^hbit amplitudes long
     const int line_bits = prec == kC128 ? 3 : 4;
-    const bool nt = count >= kStreamMinCount && stream_hint_enabled() && (xmask == 0 || hbit >= line_bits);
+    const bool nt = beyond_cache(count, prec == kC128 ? 16 : 8) && stream_hint_enabled() && (xmask == 0 || hbit >= line_bits);
     if (prec == kC128) {
         if (nt)
             hipLaunchKernelGGL((k_red_pauli<double, true>), dim3(b), dim3(kRedThreads), 0, st, (const double2*)psi,
