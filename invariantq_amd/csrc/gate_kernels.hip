@@ -535,10 +535,11 @@ hipError_t run_direct(void* psi, int n, hipStream_t st, const GateDesc& g) {
     for (int m = 0; m < K; ++m) whole = whole && g.q[m] >= line_bits;
     if constexpr (K <= 2) {
         // Lane-exchange forms for targets below bit 6, on states beyond the
-        // caches (from 24 qubits; measured at 14-qubit density matrices and 30
-        // qubits).  On a 20-qubit state, which stays in the Infinity Cache, the
-        // shuffles' latency made them slower (12.3 against 7.9 us per pass).
-        const bool big = n >= 24;
+        // 256 MiB Infinity Cache (measured at 14-qubit density matrices and 30
+        // qubits).  On states that stay in the cache the shuffles' latency
+        // made them slower: 20 qubits 12.3 against 7.9 us per pass, complex64
+        // 25 qubits 71.7 against 65.5 us per one-qubit gate.
+        const bool big = beyond_cache(1ULL << n, sizeof(cv<R>));
         bool lanes = big;  // every target below bit 6
         for (int m = 0; m < K; ++m) lanes = lanes && g.q[m] < kLaneLow;
         if (K == 2 && big) {

@@ -1,8 +1,10 @@
-"""One- and two-qubit gates on qubits 0-5 (the lane-exchange kernel when a
-target lies inside a 128-byte line) against the oracle's restatement of the
-reference's statevector.cpp:26-87, at sizes where it does and does not stream,
-with pairs in both orders and pairs that leave the lane range (strided
-kernel).  Tolerance: 1e-12 absolute for complex128, 2e-6 for complex64."""
+"""One- and two-qubit gates on qubits 0-5 against the oracle's restatement
+of the reference's statevector.cpp:26-87: the direct kernels on states up to
+the 256 MiB Infinity Cache (8, 12, 24 qubits), the lane-exchange kernels
+(a target inside a 128-byte line) and streaming beyond it (25 qubits
+complex128, 26 complex64), with pairs in both orders and pairs that leave
+the lane range (strided kernel).  Tolerance: 1e-12 absolute for
+complex128, 2e-6 for complex64."""
 
 import numpy as np
 import pytest
@@ -25,8 +27,8 @@ def _ops(n, rng):
     return ops
 
 
-@pytest.mark.parametrize("n", [8, 12, 24])
-@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+@pytest.mark.parametrize("n, precision", [(8, "complex128"), (12, "complex128"), (24, "complex128"), (25, "complex128"),
+                                          (8, "complex64"), (12, "complex64"), (24, "complex64"), (26, "complex64")])
 def test_low_target_gates_match_oracle(oracle_lib, n, precision):
     from invariantq_amd.core import binding
     rng = np.random.default_rng(60 + n)
